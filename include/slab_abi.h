@@ -1,0 +1,157 @@
+/*
+ * slab_abi.h -- C ABI of the MI355X depth-averaged MPM step (libslabpro.so).
+ *
+ * The reference (ALMO-SLF-ETH/Slab-Pro) has no FFI: its step is the set of
+ * private member functions of Simulation<T,TV2> (src/simulation.hpp:281-286)
+ * driven by Simulation::simulate() (src/simulation.cpp:484-522).  These entry
+ * points replace that loop body one for one; each declaration cites the
+ * reference code it stands in for.
+ *
+ * Conventions
+ *  - Every call returns slab_status (0 = OK).  No C++ exception crosses the
+ *    ABI; slab_last_error() returns a thread-local message for the last
+ *    failing call on this thread.
+ *  - Host buffers belong to the caller and are copied in/out.  Arrays use the
+ *    reference's layouts: node arrays in ind(i,j,Ny) = i*Ny + j order
+ *    (src/tools.hpp:172-174), particle arrays in creation order.
+ *  - A handle owns its device memory and one HIP stream; it is used by one
+ *    host thread at a time.  Different handles may be used concurrently.
+ *  - Nothing falls back to the CPU: without a usable gfx950 device,
+ *    slab_run_create fails with SLAB_EDEVICE.
+ */
+#ifndef SLAB_ABI_H
+#define SLAB_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct slab_run slab_run;
+
+typedef enum {
+    SLAB_OK = 0,
+    SLAB_EINVAL = 1,  /* bad argument */
+    SLAB_ENOMEM = 2,  /* device allocation failed */
+    SLAB_EDEVICE = 3, /* no device / HIP runtime error */
+    SLAB_EDOMAIN = 4, /* input outside what the step supports */
+    SLAB_ENAN = 5,    /* non-finite velocity detected on the device */
+    SLAB_ESTATE = 6   /* call not valid in the handle's current state */
+} slab_status;
+
+typedef enum { SLAB_FP64 = 0, SLAB_FP32 = 1 } slab_precision;
+
+/* = enum class FrictionType order, src/tools.hpp:37 */
+typedef enum { SLAB_COULOMB = 0, SLAB_VOELLMY = 1, SLAB_MU_I = 2 } slab_friction_type;
+
+/* = struct Friction and its members, src/tools.hpp:62-84 */
+typedef struct {
+    int32_t type;          /* slab_friction_type */
+    double mu;             /* Coulomb mu or Voellmy mu */
+    double xi;             /* Voellmy xi [m s^-2] */
+    double mu1, mu2, I0, d, phi; /* mu(I) */
+} slab_friction;
+
+/* = the scalar members read by the step (src/simulation.hpp:288-310) */
+typedef struct {
+    double gravity, rho, C_cfl, dt_max, flip_ratio, output_dt, total_time;
+    int32_t precision; /* slab_precision */
+} slab_params;
+
+/* = Grid "terrain" as initializeTerrain leaves it (src/simulation.cpp:210-257):
+ * nodes X_i = i*dx, Y_j = j*dx; z[i*ny + j]. */
+typedef struct {
+    int32_t nx, ny;
+    double dx;
+    const double* z;
+} slab_terrain;
+
+/* Progress of the driver (src/simulation.cpp:500-510 state variables). */
+typedef struct {
+    double time;       /* simulated time after the last step */
+    double dt;         /* dt of the last step */
+    int32_t frame;     /* output frame counter */
+    int32_t step;      /* time_step counter */
+    int32_t frame_hit; /* last step reached output_dt*(frame) (frame already advanced) */
+    int32_t finished;  /* time >= total_time */
+    int32_t steps_done;/* steps executed by this call */
+    int32_t reserved;
+} slab_step_info;
+
+/* Creates a run on the current HIP device: uploads the terrain and computes
+ * its gradient / slope tables.  Replaces initializeTerrain's tail,
+ * terrainGradients + terrainTheta (src/complex_terrain.cpp:84-111), and the
+ * zeroing of h_max / v_max2 (src/simulation.cpp:254-256). */
+slab_status slab_run_create(const slab_terrain* terrain, const slab_params* params,
+                            const slab_friction* friction, slab_run** out);
+
+/* Uploads particles as initializeParticles leaves them
+ * (src/simulation.cpp:443-466).  vx, vy, vz may be NULL (zero).  m may be
+ * NULL (then rho*h*A as at :452).  Particle order is the creation order. */
+slab_status slab_run_upload_particles(slab_run* run, int64_t np, const double* x, const double* y,
+                                      const double* vx, const double* vy, const double* vz,
+                                      const double* h, const double* A, const double* m);
+
+/* t = 0 priming: remesh, P2G, computeGridVelocity with dt = 0
+ * (src/simulation.cpp:493-495).  Seeds h_max with the initial height. */
+slab_status slab_run_prime(slab_run* run);
+
+/* Runs up to max_steps passes of the while loop at src/simulation.cpp:500-510
+ * (advanceStep = updateDt, remesh, sort, P2G, computeGridVelocity, G2P;
+ * time += dt; frame check).  Returns early after a step that hits an output
+ * frame or when time >= total_time, so the caller can write outputs.
+ * info may be NULL. */
+slab_status slab_run_step(slab_run* run, int32_t max_steps, slab_step_info* info);
+
+/* Particles in creation order, as saveParticleData reads them
+ * (src/simulation.cpp via src/save_function.cpp:36-76).  Any pointer may be NULL. */
+slab_status slab_run_download_particles(slab_run* run, double* x, double* y, double* vx, double* vy,
+                                        double* vz, double* z, double* h, double* A, double* m);
+
+/* Grid of the last step over the reference's remesh window
+ * (src/simulation.cpp:594-612): i0 = iL, j0 = jB, nx/ny = window size.
+ * Call with all field pointers NULL to query the window first.  Field arrays
+ * (nx*ny, ind order) as saveGridData reads them (src/save_function.cpp:96-226).
+ * Returns SLAB_ESTATE if the last step did not keep grid outputs (only
+ * output-frame steps, the prime and runs with slab_run_keep_grid(1) do). */
+slab_status slab_run_download_grid(slab_run* run, int32_t* i0, int32_t* j0, int32_t* nx, int32_t* ny,
+                                   double* h, double* vx, double* vy, double* vz, double* m);
+
+/* Keep grid output fields after every step (default: only output frames). */
+slab_status slab_run_keep_grid(slab_run* run, int32_t enable);
+
+/* Full-terrain h_max and v_max2 maps (src/save_function.cpp:228-291 inputs). */
+slab_status slab_run_download_max(slab_run* run, double* h_max, double* v_max2);
+
+/* FP64 diagnostics of the last step: out[0]=sum m_i (grid), out[1]=sum m_i vx_i,
+ * out[2]=sum m_i vy_i, out[3]=sum m_p, out[4]=sum h_p A_p, out[5]=dt, out[6]=max|v_p|,
+ * out[7]=number of grid nodes with m > 1e-6.  (No reference counterpart: the
+ * reference only prints the initial volume, src/simulation.cpp:469-475.) */
+slab_status slab_run_diag(slab_run* run, double out[8]);
+
+/* Event timing of each step kernel on the run's stream.  enable=1 starts
+ * accumulating; ms[k] / counts[k] receive totals per kernel (k < n, names by
+ * slab_kernel_name).  No reference counterpart: it replaces the per-phase
+ * timers of advanceStep (src/simulation.cpp:526-562, printed at 512-519). */
+slab_status slab_run_timing(slab_run* run, int32_t enable, double* ms, int64_t* counts, int32_t n);
+const char* slab_kernel_name(int32_t k);
+int32_t slab_kernel_count(void);
+
+/* Overrides the particle velocity (creation order) -- used by closed-form tests
+ * and drivers that restart from saved state (setParticles, simulation.hpp:186). */
+slab_status slab_run_set_velocity(slab_run* run, const double* vx, const double* vy);
+
+/* Overrides time and frame (setFrame/setDt, simulation.hpp:100-113). */
+slab_status slab_run_set_time(slab_run* run, double time, int32_t frame);
+
+const char* slab_last_error(void);
+void slab_run_destroy(slab_run* run);
+
+/* Library / device facts: out[0]=device id, out[1]=CU count, out[2]=is gfx950. */
+slab_status slab_device_info(int32_t out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,635 @@
+// slab_abi.hip -- host side of libslabpro.so: the C ABI declared in
+// include/slab_abi.h, driving the FP64-form kernels on one HIP stream.
+#include "slab_abi.h"
+#include "slab_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace slab;
+
+namespace {
+
+thread_local std::string g_err;
+
+slab_status fail(slab_status s, const std::string& msg) {
+    g_err = msg;
+    return s;
+}
+
+#define HIPCHK(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(SLAB_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+const char* const kKernelNames[] = {"ctl_begin", "tile_reduce", "tile_scan", "tile_offsets", "scatter",
+                                    "tile_sort", "p2g_grid",    "g2p",       "ctl_end"};
+constexpr int kNumKernels = 9;
+constexpr int kChunk = 16;     // steps enqueued between host checks
+constexpr int kPersist = 2048; // workgroups of the persistent tile kernels
+
+} // namespace
+
+struct slab_run {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Ctl* d_ctl = nullptr;
+    Ctl* h_ctl = nullptr; // pinned mirror
+    int nx = 0, ny = 0, ntx = 0, nty = 0, ntiles = 0, nblk = 0;
+    size_t nn = 0;
+    double dx = 0, inv_dx = 0;
+    int precision = SLAB_FP64;
+    // terrain
+    double *tz = nullptr, *tgx = nullptr, *tgy = nullptr, *tcth = nullptr, *hmax = nullptr, *vmax2 = nullptr;
+    // grid
+    double4* vf = nullptr;
+    double *gm = nullptr, *gh = nullptr, *gvz = nullptr;
+    // tiles
+    int *tile_count = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr;
+    int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr, *cell_begin = nullptr;
+    // particles
+    int64_t np = 0;
+    PBuf64 A{}, B{};
+    int *key = nullptr, *tslot = nullptr, *idx = nullptr, *tmp = nullptr;
+    double *diag_part = nullptr, *diag_out = nullptr;
+    bool primed = false;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_free;
+    struct Pending { int kid; int step; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    double ms[kNumKernels] = {};
+    int64_t cnt[kNumKernels] = {};
+    std::vector<void*> allocs;
+    std::vector<void*> part_allocs;
+};
+
+namespace {
+
+template <typename T>
+slab_status dalloc(slab_run* r, T** p, size_t n, std::vector<void*>& list) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SLAB_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                                     hipGetErrorString(e));
+    }
+    list.push_back(q);
+    *p = (T*)q;
+    return SLAB_OK;
+}
+
+#define TRY(expr)                          \
+    do {                                   \
+        slab_status s_ = (expr);           \
+        if (s_ != SLAB_OK) return s_;      \
+    } while (0)
+
+slab_status pull_ctl(slab_run* r) {
+    HIPCHK(hipMemcpyAsync(r->h_ctl, r->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return SLAB_OK;
+}
+
+slab_status push_ctl(slab_run* r) {
+    HIPCHK(hipMemcpyAsync(r->d_ctl, r->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return SLAB_OK;
+}
+
+hipEvent_t get_event(slab_run* r) {
+    if (!r->ev_free.empty()) {
+        hipEvent_t e = r->ev_free.back();
+        r->ev_free.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Launch wrapper: optional event pair per kernel on the run's stream.
+template <typename F>
+void timed(slab_run* r, int kid, int step, F&& launch) {
+    if (!r->timing) {
+        launch();
+        return;
+    }
+    hipEvent_t a = get_event(r), b = get_event(r);
+    (void)hipEventRecord(a, r->stream);
+    launch();
+    (void)hipEventRecord(b, r->stream);
+    r->pending.push_back({kid, step, a, b});
+}
+
+void collect_timing(slab_run* r, int steps_valid) {
+    for (auto& p : r->pending) {
+        if (p.step < steps_valid) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                r->ms[p.kid] += ms;
+                r->cnt[p.kid] += 1;
+            }
+        }
+        r->ev_free.push_back(p.a);
+        r->ev_free.push_back(p.b);
+    }
+    r->pending.clear();
+}
+
+unsigned pblocks(const slab_run* r) { return (unsigned)((r->np + WG - 1) / WG); }
+unsigned tblocks(const slab_run* r) { return (unsigned)std::min(r->ntiles, kPersist); }
+
+// Tile scan + bucketing + canonical sort: the build of the particle order
+// that stands in for remesh + sort_particles_by_grid_block
+// (src/simulation.cpp:584-724).
+void enqueue_sort(slab_run* r, int step) {
+    hipStream_t s = r->stream;
+    timed(r, 1, step, [&] { k_tile_reduce<<<r->nblk, WG, 0, s>>>(r->d_ctl, r->tile_count, r->ntiles, r->bsum); });
+    timed(r, 2, step, [&] { k_tile_scan<<<1, WG, 0, s>>>(r->d_ctl, r->bsum, r->nblk, r->boff); });
+    timed(r, 3, step, [&] {
+        k_tile_offsets<<<r->nblk, WG, 0, s>>>(r->d_ctl, r->tile_count, r->ntiles, r->boff, r->tile_begin, r->ptiles,
+                                               r->gtiles, r->pstamp, r->gstamp);
+    });
+    timed(r, 4, step, [&] { k_scatter<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->key, r->tslot, r->tile_begin, r->tmp); });
+    timed(r, 5, step, [&] {
+        k_tile_sort<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->key, r->ptiles, r->tile_count, r->tile_begin,
+                                              r->tmp, r->idx, r->cell_begin);
+    });
+}
+
+void enqueue_p2g(slab_run* r, int step) {
+    timed(r, 6, step, [&] {
+        k_p2g_grid<<<tblocks(r), WG, 0, r->stream>>>(r->d_ctl, r->A, r->B, r->gtiles, r->pstamp, r->cell_begin, r->idx,
+                                                     r->tgx, r->tgy, r->tcth, r->hmax, r->vmax2, r->vf, r->gm, r->gh,
+                                                     r->gvz);
+    });
+}
+
+void enqueue_step(slab_run* r, int step) {
+    hipStream_t s = r->stream;
+    timed(r, 0, step, [&] { k_ctl_begin<<<1, 1, 0, s>>>(r->d_ctl); });
+    enqueue_sort(r, step);
+    enqueue_p2g(r, step);
+    timed(r, 7, step, [&] {
+        k_g2p<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->vf, r->tz, r->tgx, r->tgy, r->key, r->tslot,
+                                        r->tile_count);
+    });
+    timed(r, 8, step, [&] { k_ctl_end<<<1, 1, 0, s>>>(r->d_ctl); });
+}
+
+// terrainGradients + terrainTheta (src/complex_terrain.cpp:84-111) on the host
+// in FP64 with the host libm, plus cos(theta) as P2G and the grid update use it
+// (src/simulation.cpp:746, 825).
+void terrain_tables(const double* z, int nx, int ny, double inv, std::vector<double>& gx, std::vector<double>& gy,
+                    std::vector<double>& cth) {
+    const size_t nn = (size_t)nx * ny;
+    gx.assign(nn, 0.0);
+    gy.assign(nn, 0.0);
+    cth.assign(nn, 1.0);
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto work = [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            for (int j = 0; j < ny; ++j) {
+                const size_t k = (size_t)i * ny + j;
+                if (i < nx - 1 && j < ny - 1) {
+                    const double z00 = z[k], z01 = z[k + 1], z10 = z[k + ny], z11 = z[k + ny + 1];
+                    gx[k] = (z10 + z11 - z00 - z01) / 2. * inv;
+                    gy[k] = (z01 + z11 - z00 - z10) / 2. * inv;
+                }
+                const double th = std::atan(std::sqrt(gx[k] * gx[k] + gy[k] * gy[k]));
+                cth[k] = std::cos(th);
+            }
+        }
+    };
+    std::vector<std::thread> ts;
+    int per = (nx + (int)nth - 1) / (int)nth;
+    for (unsigned t = 0; t < nth; ++t) {
+        int a = (int)t * per, b = std::min(nx, a + per);
+        if (a < b) ts.emplace_back(work, a, b);
+    }
+    for (auto& t : ts) t.join();
+}
+
+void free_list(std::vector<void*>& l) {
+    for (void* p : l) (void)hipFree(p);
+    l.clear();
+}
+
+} // namespace
+
+extern "C" {
+
+const char* slab_last_error(void) { return g_err.c_str(); }
+
+int32_t slab_kernel_count(void) { return kNumKernels; }
+
+const char* slab_kernel_name(int32_t k) { return (k >= 0 && k < kNumKernels) ? kKernelNames[k] : ""; }
+
+slab_status slab_device_info(int32_t out[3]) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        return fail(SLAB_EDEVICE, "no HIP device visible");
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    out[0] = dev;
+    out[1] = p.multiProcessorCount;
+    out[2] = std::strncmp(p.gcnArchName, "gfx950", 6) == 0;
+    return SLAB_OK;
+}
+
+slab_status slab_run_create(const slab_terrain* terrain, const slab_params* params, const slab_friction* friction,
+                            slab_run** out) {
+    if (!terrain || !params || !friction || !out) return fail(SLAB_EINVAL, "null argument");
+    *out = nullptr;
+    const int nx = terrain->nx, ny = terrain->ny;
+    if (nx < 4 || ny < 4 || !terrain->z) return fail(SLAB_EINVAL, "terrain must have at least 4x4 nodes and z");
+    if (!(terrain->dx > 0.0) || !std::isfinite(terrain->dx)) return fail(SLAB_EINVAL, "dx must be positive");
+    if ((double)nx * (double)ny > 1.0e9) return fail(SLAB_EDOMAIN, "terrain larger than 1e9 nodes");
+    if (params->precision != SLAB_FP64) return fail(SLAB_EINVAL, "this build of the step supports SLAB_FP64");
+    if (friction->type < 0 || friction->type > 2) return fail(SLAB_EINVAL, "unknown friction type");
+    if (!(params->output_dt > 0.0) || !(params->dt_max > 0.0))
+        return fail(SLAB_EINVAL, "output_dt and dt_max must be positive");
+    int32_t info[3];
+    TRY(slab_device_info(info));
+    if (!info[2]) return fail(SLAB_EDEVICE, "device is not gfx950 (MI355X)");
+
+    slab_run* r = new slab_run();
+    r->device = info[0];
+    r->nx = nx;
+    r->ny = ny;
+    r->nn = (size_t)nx * ny;
+    r->dx = terrain->dx;
+    r->inv_dx = 1. / terrain->dx;
+    r->ntx = (nx + TILE - 1) / TILE;
+    r->nty = (ny + TILE - 1) / TILE;
+    r->ntiles = r->ntx * r->nty;
+    r->nblk = (r->ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    auto cleanup = [&](slab_status s) {
+        slab_run_destroy(r);
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(SLAB_EDEVICE, "hipStreamCreate failed"));
+    if (hipHostMalloc((void**)&r->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(SLAB_ENOMEM, "hipHostMalloc failed"));
+    std::memset(r->h_ctl, 0, sizeof(Ctl));
+    slab_status st = SLAB_OK;
+    auto A = [&](auto** p, size_t n) {
+        if (st == SLAB_OK) st = dalloc(r, p, n, r->allocs);
+    };
+    A(&r->d_ctl, 1);
+    A(&r->tz, r->nn);
+    A(&r->tgx, r->nn);
+    A(&r->tgy, r->nn);
+    A(&r->tcth, r->nn);
+    A(&r->hmax, r->nn);
+    A(&r->vmax2, r->nn);
+    A(&r->vf, r->nn);
+    A(&r->gm, r->nn);
+    A(&r->gh, r->nn);
+    A(&r->gvz, r->nn);
+    A(&r->tile_count, r->ntiles);
+    A(&r->tile_begin, r->ntiles);
+    A(&r->pstamp, r->ntiles);
+    A(&r->gstamp, r->ntiles);
+    A(&r->ptiles, r->ntiles);
+    A(&r->gtiles, r->ntiles);
+    A(&r->bsum, 3 * (size_t)r->nblk);
+    A(&r->boff, 3 * (size_t)r->nblk);
+    A(&r->cell_begin, (size_t)r->ntiles * CELL_STRIDE);
+    A(&r->diag_part, 4 * 256 + 2 * 256);
+    A(&r->diag_out, 8);
+    if (st != SLAB_OK) return cleanup(st);
+
+    std::vector<double> gx, gy, cth;
+    terrain_tables(terrain->z, nx, ny, r->inv_dx, gx, gy, cth);
+    hipStream_t s = r->stream;
+    if (hipMemcpyAsync(r->tz, terrain->z, r->nn * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(r->tgx, gx.data(), r->nn * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(r->tgy, gy.data(), r->nn * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(r->tcth, cth.data(), r->nn * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(r->hmax, 0, r->nn * 8, s) != hipSuccess ||
+        hipMemsetAsync(r->vmax2, 0, r->nn * 8, s) != hipSuccess ||
+        hipMemsetAsync(r->vf, 0, r->nn * sizeof(double4), s) != hipSuccess ||
+        hipMemsetAsync(r->gm, 0, r->nn * 8, s) != hipSuccess || hipMemsetAsync(r->gh, 0, r->nn * 8, s) != hipSuccess ||
+        hipMemsetAsync(r->gvz, 0, r->nn * 8, s) != hipSuccess ||
+        hipMemsetAsync(r->tile_count, 0, (size_t)r->ntiles * 4, s) != hipSuccess ||
+        hipMemsetAsync(r->pstamp, 0xff, (size_t)r->ntiles * 4, s) != hipSuccess ||
+        hipMemsetAsync(r->gstamp, 0xff, (size_t)r->ntiles * 4, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return cleanup(fail(SLAB_EDEVICE, "terrain upload failed"));
+
+    Ctl& c = *r->h_ctl;
+    c.gravity = params->gravity;
+    c.rho = params->rho;
+    c.C_cfl = params->C_cfl;
+    c.dt_max = params->dt_max;
+    c.flip_ratio = params->flip_ratio;
+    c.output_dt = params->output_dt;
+    c.total_time = params->total_time;
+    c.dx = r->dx;
+    c.inv_dx = r->inv_dx;
+    c.fric = friction->type;
+    c.mu = friction->mu;
+    c.xi = friction->xi;
+    c.mu1 = friction->mu1;
+    c.mu2 = friction->mu2;
+    c.I0 = friction->I0;
+    c.d = friction->d;
+    c.phi = friction->phi;
+    c.nx = nx;
+    c.ny = ny;
+    c.ntx = r->ntx;
+    c.nty = r->nty;
+    c.stop = RUNNING;
+    c.bb_next[0] = c.bb_next[2] = ~0ull;
+    st = push_ctl(r);
+    if (st != SLAB_OK) return cleanup(st);
+    *out = r;
+    return SLAB_OK;
+}
+
+slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, const double* y, const double* vx,
+                                      const double* vy, const double* vz, const double* h, const double* A,
+                                      const double* m) {
+    if (!r || !x || !y || !h || !A) return fail(SLAB_EINVAL, "null argument");
+    if (np < 1 || np > (int64_t)1 << 30) return fail(SLAB_EINVAL, "particle count must be in [1, 2^30]");
+    free_list(r->part_allocs);
+    r->np = np;
+    r->primed = false;
+    slab_status st = SLAB_OK;
+    auto P = [&](PBuf64& b) {
+        double** f[] = {&b.x, &b.y, &b.vx, &b.vy, &b.vz, &b.h, &b.A, &b.m, &b.z};
+        for (double** q : f)
+            if (st == SLAB_OK) st = dalloc(r, q, (size_t)np, r->part_allocs);
+        if (st == SLAB_OK) st = dalloc(r, &b.pid, (size_t)np, r->part_allocs);
+    };
+    P(r->A);
+    P(r->B);
+    if (st == SLAB_OK) st = dalloc(r, &r->key, (size_t)np, r->part_allocs);
+    if (st == SLAB_OK) st = dalloc(r, &r->tslot, (size_t)np, r->part_allocs);
+    if (st == SLAB_OK) st = dalloc(r, &r->idx, (size_t)np, r->part_allocs);
+    if (st == SLAB_OK) st = dalloc(r, &r->tmp, (size_t)np, r->part_allocs);
+    if (st != SLAB_OK) return st;
+    TRY(pull_ctl(r));
+    const double rho = r->h_ctl->rho;
+    std::vector<double> mm((size_t)np), zero((size_t)np, 0.0);
+    std::vector<int32_t> pid((size_t)np);
+    for (int64_t p = 0; p < np; ++p) {
+        mm[p] = m ? m[p] : rho * h[p] * A[p]; // simulation.cpp:452
+        pid[p] = (int32_t)p;
+    }
+    hipStream_t s = r->stream;
+    const size_t nb = (size_t)np * 8;
+    HIPCHK(hipMemcpyAsync(r->A.x, x, nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.y, y, nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.vx, vx ? vx : zero.data(), nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.vy, vy ? vy : zero.data(), nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.vz, vz ? vz : zero.data(), nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.h, h, nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.A, A, nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.m, mm.data(), nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.pid, pid.data(), (size_t)np * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    Ctl& c = *r->h_ctl;
+    c.np = (int32_t)np;
+    c.time = 0.0;
+    c.dt = 0.0;
+    c.frame = 0;
+    c.step = 0;
+    c.parity = 0;
+    c.stop = RUNNING;
+    c.err = 0;
+    TRY(push_ctl(r));
+    k_init_z<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->tz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return SLAB_OK;
+}
+
+slab_status slab_run_prime(slab_run* r) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    if (r->np < 1) return fail(SLAB_ESTATE, "no particles uploaded");
+    if (r->primed) return fail(SLAB_ESTATE, "run already primed");
+    hipStream_t s = r->stream;
+    HIPCHK(hipMemsetAsync(r->tile_count, 0, (size_t)r->ntiles * 4, s));
+    k_prime_pre<<<1, 1, 0, s>>>(r->d_ctl);
+    k_keys<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->key, r->tslot, r->tile_count);
+    k_ctl_prime<<<1, 1, 0, s>>>(r->d_ctl);
+    bool tm = r->timing;
+    r->timing = false;
+    enqueue_sort(r, 0);
+    enqueue_p2g(r, 0);
+    r->timing = tm;
+    // keys again for step 1 (the tile sort consumed the counts)
+    k_keys<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->key, r->tslot, r->tile_count);
+    HIPCHK(hipGetLastError());
+    TRY(pull_ctl(r));
+    r->primed = true;
+    return SLAB_OK;
+}
+
+slab_status slab_run_step(slab_run* r, int32_t max_steps, slab_step_info* info) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    if (!r->primed) return fail(SLAB_ESTATE, "call slab_run_prime first");
+    if (max_steps < 0) return fail(SLAB_EINVAL, "max_steps < 0");
+    hipStream_t s = r->stream;
+    k_ctl_clear<<<1, 1, 0, s>>>(r->d_ctl);
+    int done = 0;
+    while (done < max_steps) {
+        const int chunk = std::min(kChunk, max_steps - done);
+        // every step of earlier chunks ran (a stop ends the loop), so done + k
+        // is the call-relative index of the step enqueued here
+        for (int k = 0; k < chunk; ++k) enqueue_step(r, done + k);
+        HIPCHK(hipGetLastError());
+        TRY(pull_ctl(r));
+        if (r->timing) collect_timing(r, r->h_ctl->steps_in_call);
+        done += chunk;
+        if (r->h_ctl->stop != RUNNING) break;
+    }
+    const Ctl& c = *r->h_ctl;
+    if (info) {
+        info->time = c.time;
+        info->dt = c.dt;
+        info->frame = c.frame;
+        info->step = c.step;
+        info->frame_hit = c.frame_hit;
+        info->finished = !(c.time < c.total_time);
+        info->steps_done = c.steps_in_call;
+        info->reserved = 0;
+    }
+    if (c.stop == STOP_ERROR) return fail(SLAB_ENAN, "non-finite particle velocity on the device");
+    return SLAB_OK;
+}
+
+slab_status slab_run_download_particles(slab_run* r, double* x, double* y, double* vx, double* vy, double* vz,
+                                        double* z, double* h, double* A, double* m) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    if (r->np < 1) return fail(SLAB_ESTATE, "no particles");
+    TRY(pull_ctl(r));
+    const PBuf64& P = r->h_ctl->parity ? r->B : r->A;
+    const size_t n = (size_t)r->np;
+    std::vector<int32_t> pid(n);
+    std::vector<double> tmp(n);
+    HIPCHK(hipMemcpy(pid.data(), P.pid, n * 4, hipMemcpyDeviceToHost));
+    double* dsts[] = {x, y, vx, vy, vz, z, h, A, m};
+    double* srcs[] = {P.x, P.y, P.vx, P.vy, P.vz, P.z, P.h, P.A, P.m};
+    for (int f = 0; f < 9; ++f) {
+        if (!dsts[f]) continue;
+        HIPCHK(hipMemcpy(tmp.data(), srcs[f], n * 8, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) dsts[f][pid[k]] = tmp[k];
+    }
+    return SLAB_OK;
+}
+
+slab_status slab_run_keep_grid(slab_run* r, int32_t enable) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    TRY(pull_ctl(r));
+    r->h_ctl->keep_grid = enable ? 1 : 0;
+    return push_ctl(r);
+}
+
+slab_status slab_run_download_grid(slab_run* r, int32_t* i0, int32_t* j0, int32_t* wnx, int32_t* wny, double* h,
+                                   double* vx, double* vy, double* vz, double* m) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    if (!r->primed) return fail(SLAB_ESTATE, "run not primed");
+    TRY(pull_ctl(r));
+    const Ctl& c = *r->h_ctl;
+    if (c.last_out_epoch != c.epoch)
+        return fail(SLAB_ESTATE, "last step kept no grid outputs (not an output frame; see slab_run_keep_grid)");
+    // remesh window, src/simulation.cpp:594-606 (terrain.x[0] = terrain.y[0] = 0)
+    const double min_x = ord_value(c.bb_cur[0]), max_x = ord_value(c.bb_cur[1]);
+    const double min_y = ord_value(c.bb_cur[2]), max_y = ord_value(c.bb_cur[3]);
+    const double eps = 1e-8 * r->dx, inv = r->inv_dx;
+    int iL = (int)std::max(-1e9, std::min(1e9, (min_x - 0.0) * inv - 1.0 - eps));
+    int iR = (int)std::max(-1e9, std::min(1e9, (max_x - 0.0) * inv + 2.0 + eps));
+    int jB = (int)std::max(-1e9, std::min(1e9, (min_y - 0.0) * inv - 1.0 - eps));
+    int jT = (int)std::max(-1e9, std::min(1e9, (max_y - 0.0) * inv + 2.0 + eps));
+    iL = std::max(iL, 0);
+    jB = std::max(jB, 0);
+    iR = std::min(iR, r->nx - 1);
+    jT = std::min(jT, r->ny - 1);
+    if (iR < iL || jT < jB) return fail(SLAB_EDOMAIN, "particle window outside the terrain");
+    const int gnx = iR - iL + 1, gny = jT - jB + 1;
+    if (i0) *i0 = iL;
+    if (j0) *j0 = jB;
+    if (wnx) *wnx = gnx;
+    if (wny) *wny = gny;
+    if (!h && !vx && !vy && !vz && !m) return SLAB_OK;
+    const size_t wn = (size_t)gnx * gny;
+    std::vector<void*> tmpl;
+    double* o = nullptr;
+    slab_status st = dalloc(r, &o, 5 * wn, tmpl);
+    if (st != SLAB_OK) return st;
+    k_window<<<(unsigned)((wn + WG - 1) / WG), WG, 0, r->stream>>>(r->d_ctl, iL, jB, gnx, gny, r->gstamp, r->vf, r->gm,
+                                                                   r->gh, r->gvz, o, o + wn, o + 2 * wn, o + 3 * wn,
+                                                                   o + 4 * wn);
+    hipError_t e = hipStreamSynchronize(r->stream);
+    double* dst[] = {h, vx, vy, vz, m};
+    for (int f = 0; f < 5 && e == hipSuccess; ++f)
+        if (dst[f]) e = hipMemcpy(dst[f], o + f * wn, wn * 8, hipMemcpyDeviceToHost);
+    free_list(tmpl);
+    if (e != hipSuccess) return fail(SLAB_EDEVICE, std::string("window download: ") + hipGetErrorString(e));
+    return SLAB_OK;
+}
+
+slab_status slab_run_download_max(slab_run* r, double* h_max, double* v_max2) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    HIPCHK(hipStreamSynchronize(r->stream));
+    if (h_max) HIPCHK(hipMemcpy(h_max, r->hmax, r->nn * 8, hipMemcpyDeviceToHost));
+    if (v_max2) HIPCHK(hipMemcpy(v_max2, r->vmax2, r->nn * 8, hipMemcpyDeviceToHost));
+    return SLAB_OK;
+}
+
+slab_status slab_run_diag(slab_run* r, double out[8]) {
+    if (!r || !out) return fail(SLAB_EINVAL, "null argument");
+    if (!r->primed) return fail(SLAB_ESTATE, "run not primed");
+    TRY(pull_ctl(r));
+    hipStream_t s = r->stream;
+    k_diag_grid<<<256, WG, 0, s>>>(r->d_ctl, r->gtiles, r->vf, r->gm, r->diag_part);
+    k_diag_particles<<<256, WG, 0, s>>>(r->d_ctl, r->A, r->B, r->diag_part + 4 * 256);
+    k_diag_final<<<1, 64, 0, s>>>(r->diag_part, 256, r->diag_part + 4 * 256, 256, r->diag_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, r->diag_out, 8 * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const Ctl& c = *r->h_ctl;
+    out[5] = c.dt;
+    union { unsigned long long u; double d; } v;
+    v.u = c.vmax2_bits;
+    out[6] = std::sqrt(v.d);
+    if (c.last_out_epoch != c.epoch) {
+        out[0] = out[1] = out[2] = out[7] = std::nan("");
+    }
+    return SLAB_OK;
+}
+
+slab_status slab_run_timing(slab_run* r, int32_t enable, double* ms, int64_t* counts, int32_t n) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    if (ms || counts) {
+        for (int k = 0; k < n && k < kNumKernels; ++k) {
+            if (ms) ms[k] = r->ms[k];
+            if (counts) counts[k] = r->cnt[k];
+        }
+    }
+    if (enable && !r->timing) {
+        std::fill(r->ms, r->ms + kNumKernels, 0.0);
+        std::fill(r->cnt, r->cnt + kNumKernels, 0);
+    }
+    r->timing = enable != 0;
+    return SLAB_OK;
+}
+
+slab_status slab_run_set_velocity(slab_run* r, const double* vx, const double* vy) {
+    if (!r || !vx || !vy) return fail(SLAB_EINVAL, "null argument");
+    if (r->primed) return fail(SLAB_ESTATE, "set the velocity before slab_run_prime");
+    TRY(pull_ctl(r));
+    const PBuf64& P = r->h_ctl->parity ? r->B : r->A;
+    const size_t n = (size_t)r->np;
+    std::vector<int32_t> pid(n);
+    std::vector<double> a(n), b(n);
+    HIPCHK(hipMemcpy(pid.data(), P.pid, n * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; ++k) {
+        a[k] = vx[pid[k]];
+        b[k] = vy[pid[k]];
+    }
+    HIPCHK(hipMemcpy(P.vx, a.data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P.vy, b.data(), n * 8, hipMemcpyHostToDevice));
+    return SLAB_OK;
+}
+
+slab_status slab_run_set_time(slab_run* r, double time, int32_t frame) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    TRY(pull_ctl(r));
+    r->h_ctl->time = time;
+    r->h_ctl->frame = frame;
+    return push_ctl(r);
+}
+
+void slab_run_destroy(slab_run* r) {
+    if (!r) return;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    for (auto& p : r->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : r->ev_free) (void)hipEventDestroy(e);
+    free_list(r->part_allocs);
+    free_list(r->allocs);
+    if (r->h_ctl) (void)hipHostFree(r->h_ctl);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+}
+
+} // extern "C"

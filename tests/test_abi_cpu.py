@@ -1,0 +1,64 @@
+"""C-ABI library checks that need no GPU: it loads, exports every function
+include/slab_abi.h declares, and fails cleanly (no CPU fallback) without a device."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from slabpro import abi
+
+
+def _declared_functions():
+    text = open(abi.HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(slab_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exists_and_loads():
+    assert os.path.exists(abi.LIB_PATH), "run __graft_entry__.build() first"
+    abi.lib()
+
+
+def test_exports_every_declared_symbol():
+    declared = _declared_functions()
+    assert len(declared) >= 15
+    out = subprocess.check_output(["nm", "-D", "--defined-only", abi.LIB_PATH], text=True)
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = [n for n in declared if n not in exported]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    assert set(_declared_functions()) == set(abi.SIGNATURES)
+
+
+def test_kernel_names():
+    names = abi.kernel_names()
+    assert "p2g_grid" in names and "g2p" in names and "tile_sort" in names
+
+
+def test_null_and_bad_arguments_are_rejected():
+    import ctypes
+    L = abi.lib()
+    assert L.slab_run_create(None, None, None, None) == abi.EINVAL
+    assert L.slab_run_prime(None) == abi.EINVAL
+    assert L.slab_run_step(None, 1, None) == abi.EINVAL
+    t = abi.Terrain(2, 2, 1.0, None)
+    p = abi.params()
+    f = abi.friction()
+    h = ctypes.c_void_p()
+    assert L.slab_run_create(ctypes.byref(t), ctypes.byref(p), ctypes.byref(f), ctypes.byref(h)) == abi.EINVAL
+    assert b"4x4" in L.slab_last_error()
+
+
+def test_no_device_fails_loudly():
+    """Without a GPU the library reports SLAB_EDEVICE instead of computing on the CPU."""
+    try:
+        info = abi.device_info()
+    except abi.SlabError as e:
+        assert e.status == abi.EDEVICE
+        return
+    if info["gfx950"]:
+        pytest.skip("a gfx950 device is present")

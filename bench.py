@@ -1,0 +1,170 @@
+"""Benchmark of the MI355X MPM step: particle-updates/s on the 8M-particle workload.
+
+Workload (default): 8000x8000-node synthetic DEM at 1 m, 32-vertex star polygon
+release (mean radius 800 m, h0 = 4 m, ~8M UNIFORM particles), mu(I) friction
+(mu1 0.38, mu2 0.64, I0 0.279, d 0.5 m, phi 0.6), rho 2000.  A "step" is one
+pass of the step loop over all particles (sort, P2G + grid update, G2P, dt).
+Inputs are resident in device memory before the timed region starts.
+
+Prints ONE JSON line.  roofline: dominant kernel's algorithmic bytes per launch
+/ its mean HIP-event duration vs the 8 TB/s HBM peak.  cpu_baseline: the FP64
+CPU oracle (single-threaded restatement of the reference step) timed on this
+host on a bounded number of steps of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "slab-pro_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes (compulsory state traffic, each persistent field read once /
+# written once), FP64 form.  Per particle and per active node (m > 1e-6).
+ALG_BYTES_FP64 = {
+    # read x y vx vy vz h A m; per node: terrain Gx Gy cos(theta), h_max v_max2 RMW, write vx vy flipx flipy
+    "p2g_grid": {"particle": 64, "node": 24 + 32 + 32},
+    # read x y vx vy h m, write x y vx vy vz h A; per node: read vx vy flipx flipy + terrain z
+    "g2p": {"particle": 48 + 56, "node": 32 + 8},
+    # idx/key traffic: read key, write idx (4 B each) + cell tables
+    "tile_sort": {"particle": 8, "node": 0},
+    "scatter": {"particle": 12, "node": 0},
+}
+STEP_BYTES_FP64 = {"particle": 104, "node": 120}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, steps, warm):
+    """Time the single-threaded FP64 oracle on `steps` steps of the workload."""
+    from oracle import oracle as orc
+    t0 = time.time()
+    pr = dict(cfg["params"])
+    fr = dict(cfg["friction"])
+    typ = fr.pop("type")
+    s = orc.OracleSim(cfg["z"], cfg["dx"], orc.params(fric_type=typ, **pr, **fr), cfg["x"], cfg["y"], cfg["h"],
+                      cfg["A"], cfg["m"])
+    s.prime()
+    for _ in range(warm):
+        s.step()
+    t1 = time.time()
+    for _ in range(steps):
+        s.step()
+    el = time.time() - t1
+    np_ = len(cfg["x"])
+    log(f"cpu oracle: setup {t1 - t0:.1f}s, {steps} steps {el:.2f}s; phases {s.scalars()['phases']}")
+    s.close()
+    return np_ * steps / el, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2"])
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build's step runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import slabpro
+    from slabpro import abi, configs
+
+    t0 = time.time()
+    cfg = getattr(configs, args.workload)()
+    npart = len(cfg["x"])
+    nx, ny = cfg["z"].shape
+    log(f"workload {args.workload}: {npart} particles, grid {nx}x{ny}, generated in {time.time() - t0:.1f}s")
+
+    t0 = time.time()
+    run = slabpro.make_run(cfg)
+    run.prime()
+    log(f"device setup + prime {time.time() - t0:.1f}s")
+    for _ in range(args.warmup):
+        run.step(1)
+
+    # timed region: K steps, inputs resident, no event instrumentation
+    done = 0
+    t0 = time.perf_counter()
+    while done < args.steps:
+        info = run.step(args.steps - done)  # returns after a stream synchronize
+        done += info["steps_done"]
+        if info["finished"]:
+            break
+    el = time.perf_counter() - t0
+    # per-kernel HIP-event durations on the run's stream, separate short region
+    run.timing(True)
+    ksteps = 0
+    while ksteps < min(args.steps, 20):
+        info = run.step(min(args.steps, 20) - ksteps)
+        ksteps += info["steps_done"]
+        if info["finished"]:
+            break
+    tim = run.timing(False)
+    run.keep_grid(True)
+    run.step(1)
+    d = run.diag()
+    n_act = int(d["n_active"])
+    value = npart * done / el
+    ms_step = 1e3 * el / max(done, 1)
+
+    # dominant kernel by total event time
+    dom = max(tim, key=lambda k: tim[k][0])
+    dom_ms, dom_n = tim[dom]
+    avg_s = dom_ms / max(dom_n, 1) * 1e-3
+    ab = ALG_BYTES_FP64.get(dom, {"particle": 0, "node": 0})
+    alg = ab["particle"] * npart + ab["node"] * n_act
+    achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
+    step_alg = STEP_BYTES_FP64["particle"] * npart + STEP_BYTES_FP64["node"] * n_act
+    log("kernel times (ms total, launches): " + ", ".join(f"{k} {v[0]:.2f}/{v[1]}" for k, v in tim.items()))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu_rate, cpu_el = cpu_baseline(cfg, args.cpu_steps, 1)
+        cpu = {"value": cpu_rate, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps (after prime + 1 warm-up step) of the same {args.workload} "
+                         f"workload, {npart} particles, FP64 oracle, single thread, {cpu_el:.1f} s"}
+
+    out = {
+        "metric": "particle-updates/s at 8M particles",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": done,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded DEM + release generators; no external data)",
+        "config": {"workload": f"{args.workload}: " + {
+            "c0": "601x151 nodes at 2 m, analytical 35deg plane+arc+flat, 4,800 particles, Coulomb",
+            "c1": "2000x2000 DEM at 5 m, raster release of 40 ellipses, Voellmy",
+            "c2": "8000x8000 DEM at 1 m, 32-vertex star polygon r=800 m h0=4 m, mu(I)"}[args.workload],
+            "particles": npart, "grid": f"{nx}x{ny}", "form": "fp64", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3},
+        "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBs": step_alg / (ms_step * 1e-3) / 1e9,
+                          "frac": step_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "n_active_nodes": n_act},
+        "kernels_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in tim.items()},
+        "cpu_baseline": cpu,
+    }
+    run.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -35,6 +35,14 @@ ALG_BYTES_FP64 = {
     "scatter": {"particle": 12, "node": 0},
 }
 STEP_BYTES_FP64 = {"particle": 104, "node": 120}
+# FP32 form: particle = cell int32 + offsets + vx vy vz h m (32 B read, 28 B written)
+ALG_BYTES_FP32 = {
+    "p2g_grid": {"particle": 32, "node": 12 + 16 + 16},
+    "g2p": {"particle": 28 + 28, "node": 16 + 4},
+    "tile_sort": {"particle": 8, "node": 0},
+    "scatter": {"particle": 12, "node": 0},
+}
+STEP_BYTES_FP32 = {"particle": 60, "node": 60}
 
 
 def log(*a):
@@ -71,6 +79,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2"])
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", default="fp64", choices=["fp32", "fp64"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build's step runs on a single MI355X "
@@ -88,7 +97,7 @@ def main():
     log(f"workload {args.workload}: {npart} particles, grid {nx}x{ny}, generated in {time.time() - t0:.1f}s")
 
     t0 = time.time()
-    run = slabpro.make_run(cfg)
+    run = slabpro.make_run(cfg, precision=abi.FP32 if args.form == "fp32" else abi.FP64)
     run.prime()
     log(f"device setup + prime {time.time() - t0:.1f}s")
     for _ in range(args.warmup):
@@ -123,10 +132,12 @@ def main():
     dom = max(tim, key=lambda k: tim[k][0])
     dom_ms, dom_n = tim[dom]
     avg_s = dom_ms / max(dom_n, 1) * 1e-3
-    ab = ALG_BYTES_FP64.get(dom, {"particle": 0, "node": 0})
+    alg_k = ALG_BYTES_FP32 if args.form == "fp32" else ALG_BYTES_FP64
+    step_b = STEP_BYTES_FP32 if args.form == "fp32" else STEP_BYTES_FP64
+    ab = alg_k.get(dom, {"particle": 0, "node": 0})
     alg = ab["particle"] * npart + ab["node"] * n_act
     achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
-    step_alg = STEP_BYTES_FP64["particle"] * npart + STEP_BYTES_FP64["node"] * n_act
+    step_alg = step_b["particle"] * npart + step_b["node"] * n_act
     log("kernel times (ms total, launches): " + ", ".join(f"{k} {v[0]:.2f}/{v[1]}" for k, v in tim.items()))
 
     cpu = None
@@ -147,13 +158,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if args.form == "fp32" else "f64",
         "data": "synthetic (seeded DEM + release generators; no external data)",
         "config": {"workload": f"{args.workload}: " + {
             "c0": "601x151 nodes at 2 m, analytical 35deg plane+arc+flat, 4,800 particles, Coulomb",
             "c1": "2000x2000 DEM at 5 m, raster release of 40 ellipses, Voellmy",
             "c2": "8000x8000 DEM at 1 m, 32-vertex star polygon r=800 m h0=4 m, mu(I)"}[args.workload],
-            "particles": npart, "grid": f"{nx}x{ny}", "form": "fp64", "parallelism": "single GPU"},
+            "particles": npart, "grid": f"{nx}x{ny}", "form": args.form, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3},

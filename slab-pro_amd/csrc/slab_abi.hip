@@ -1,7 +1,9 @@
 // slab_abi.hip -- host side of libslabpro.so: the C ABI declared in
-// include/slab_abi.h, driving the FP64-form kernels on one HIP stream.
+// include/slab_abi.h, driving the FP64-form or FP32-form kernels of one run on
+// one HIP stream.
 #include "slab_abi.h"
 #include "slab_kernels.hpp"
+#include "slab_kernels_f32.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -28,9 +30,15 @@ slab_status fail(slab_status s, const std::string& msg) {
         if (e_ != hipSuccess) return fail(SLAB_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+#define TRY(expr)                          \
+    do {                                   \
+        slab_status s_ = (expr);           \
+        if (s_ != SLAB_OK) return s_;      \
+    } while (0)
+
 const char* const kKernelNames[] = {"ctl_begin", "tile_reduce", "tile_scan", "tile_offsets", "scatter",
-                                    "tile_sort", "p2g_grid",    "g2p",       "ctl_end"};
-constexpr int kNumKernels = 9;
+                                    "tile_sort", "payload",     "p2g_grid",  "g2p",          "ctl_end"};
+constexpr int kNumKernels = 10;
 constexpr int kChunk = 16;     // steps enqueued between host checks
 constexpr int kPersist = 2048; // workgroups of the persistent tile kernels
 
@@ -44,36 +52,51 @@ struct slab_run {
     int nx = 0, ny = 0, ntx = 0, nty = 0, ntiles = 0, nblk = 0;
     size_t nn = 0;
     double dx = 0, inv_dx = 0;
-    int precision = SLAB_FP64;
-    // terrain
+    bool f32 = false;
+    // terrain + grid, FP64 form
     double *tz = nullptr, *tgx = nullptr, *tgy = nullptr, *tcth = nullptr, *hmax = nullptr, *vmax2 = nullptr;
-    // grid
     double4* vf = nullptr;
     double *gm = nullptr, *gh = nullptr, *gvz = nullptr;
+    // terrain + grid, FP32 form
+    float *tz32 = nullptr, *tgx32 = nullptr, *tgy32 = nullptr, *tcth32 = nullptr, *hmax32 = nullptr,
+          *vmax2_32 = nullptr;
+    float4* vf32 = nullptr;
+    float *gm32 = nullptr, *gh32 = nullptr, *gvz32 = nullptr;
     // tiles
     int *tile_count = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr;
     int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr, *cell_begin = nullptr;
     // particles
     int64_t np = 0;
     PBuf64 A{}, B{};
+    Pay64 Y{};
+    PBuf32 A32{}, B32{};
+    Pay32 Y32{};
     int *key = nullptr, *tslot = nullptr, *idx = nullptr, *tmp = nullptr;
+    unsigned long long* part = nullptr; // per-block |v|^2 max + bounding box partials
     double *diag_part = nullptr, *diag_out = nullptr;
     bool primed = false;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_free;
-    struct Pending { int kid; int step; hipEvent_t a, b; };
+    struct Pending {
+        int kid;
+        int step;
+        hipEvent_t a, b;
+    };
     std::vector<Pending> pending;
     double ms[kNumKernels] = {};
     int64_t cnt[kNumKernels] = {};
     std::vector<void*> allocs;
     std::vector<void*> part_allocs;
+
+    const int32_t* pidA() const { return f32 ? A32.pid : A.pid; }
+    const int32_t* pidB() const { return f32 ? B32.pid : B.pid; }
 };
 
 namespace {
 
 template <typename T>
-slab_status dalloc(slab_run* r, T** p, size_t n, std::vector<void*>& list) {
+slab_status dalloc(T** p, size_t n, std::vector<void*>& list) {
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
     if (e != hipSuccess) {
@@ -85,12 +108,6 @@ slab_status dalloc(slab_run* r, T** p, size_t n, std::vector<void*>& list) {
     *p = (T*)q;
     return SLAB_OK;
 }
-
-#define TRY(expr)                          \
-    do {                                   \
-        slab_status s_ = (expr);           \
-        if (s_ != SLAB_OK) return s_;      \
-    } while (0)
 
 slab_status pull_ctl(slab_run* r) {
     HIPCHK(hipMemcpyAsync(r->h_ctl, r->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, r->stream));
@@ -147,9 +164,16 @@ void collect_timing(slab_run* r, int steps_valid) {
 unsigned pblocks(const slab_run* r) { return (unsigned)((r->np + WG - 1) / WG); }
 unsigned tblocks(const slab_run* r) { return (unsigned)std::min(r->ntiles, kPersist); }
 
-// Tile scan + bucketing + canonical sort: the build of the particle order
-// that stands in for remesh + sort_particles_by_grid_block
-// (src/simulation.cpp:584-724).
+void enqueue_keys(slab_run* r) {
+    hipStream_t s = r->stream;
+    if (r->f32)
+        k_keys32<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->key, r->tslot, r->tile_count, r->part);
+    else
+        k_keys<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->key, r->tslot, r->tile_count, r->part);
+}
+
+// Tile scan + bucketing + canonical sort + P2G terms: the particle order that
+// stands in for remesh + sort_particles_by_grid_block (src/simulation.cpp:584-724).
 void enqueue_sort(slab_run* r, int step) {
     hipStream_t s = r->stream;
     timed(r, 1, step, [&] { k_tile_reduce<<<r->nblk, WG, 0, s>>>(r->d_ctl, r->tile_count, r->ntiles, r->bsum); });
@@ -160,29 +184,45 @@ void enqueue_sort(slab_run* r, int step) {
     });
     timed(r, 4, step, [&] { k_scatter<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->key, r->tslot, r->tile_begin, r->tmp); });
     timed(r, 5, step, [&] {
-        k_tile_sort<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->key, r->ptiles, r->tile_count, r->tile_begin,
-                                              r->tmp, r->idx, r->cell_begin);
+        k_tile_sort<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->pidA(), r->pidB(), r->key, r->ptiles, r->tile_count,
+                                              r->tile_begin, r->tmp, r->idx, r->cell_begin);
+    });
+    timed(r, 6, step, [&] {
+        if (r->f32)
+            k_payload32<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->idx, r->tgx32, r->tgy32, r->tcth32,
+                                                  r->Y32);
+        else
+            k_payload<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->tgx, r->tgy, r->tcth, r->Y);
     });
 }
 
 void enqueue_p2g(slab_run* r, int step) {
-    timed(r, 6, step, [&] {
-        k_p2g_grid<<<tblocks(r), WG, 0, r->stream>>>(r->d_ctl, r->A, r->B, r->gtiles, r->pstamp, r->cell_begin, r->idx,
-                                                     r->tgx, r->tgy, r->tcth, r->hmax, r->vmax2, r->vf, r->gm, r->gh,
-                                                     r->gvz);
+    hipStream_t s = r->stream;
+    timed(r, 7, step, [&] {
+        if (r->f32)
+            k_p2g_grid32<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->Y32, r->gtiles, r->pstamp, r->cell_begin, r->tgx32,
+                                                   r->tgy32, r->tcth32, r->hmax32, r->vmax2_32, r->vf32, r->gm32,
+                                                   r->gh32, r->gvz32);
+        else
+            k_p2g_grid<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->Y, r->gtiles, r->pstamp, r->cell_begin, r->tgx, r->tgy,
+                                                 r->tcth, r->hmax, r->vmax2, r->vf, r->gm, r->gh, r->gvz);
     });
 }
 
 void enqueue_step(slab_run* r, int step) {
     hipStream_t s = r->stream;
-    timed(r, 0, step, [&] { k_ctl_begin<<<1, 1, 0, s>>>(r->d_ctl); });
+    timed(r, 0, step, [&] { k_ctl_begin<<<1, WG, 0, s>>>(r->d_ctl, r->part); });
     enqueue_sort(r, step);
     enqueue_p2g(r, step);
-    timed(r, 7, step, [&] {
-        k_g2p<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->vf, r->tz, r->tgx, r->tgy, r->key, r->tslot,
-                                        r->tile_count);
+    timed(r, 8, step, [&] {
+        if (r->f32)
+            k_g2p32<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->idx, r->vf32, r->tz32, r->tgx32, r->tgy32,
+                                              r->key, r->tslot, r->tile_count, r->part);
+        else
+            k_g2p<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->vf, r->tz, r->tgx, r->tgy, r->key,
+                                            r->tslot, r->tile_count, r->part);
     });
-    timed(r, 8, step, [&] { k_ctl_end<<<1, 1, 0, s>>>(r->d_ctl); });
+    timed(r, 9, step, [&] { k_ctl_end<<<1, 1, 0, s>>>(r->d_ctl); });
 }
 
 // terrainGradients + terrainTheta (src/complex_terrain.cpp:84-111) on the host
@@ -218,9 +258,62 @@ void terrain_tables(const double* z, int nx, int ny, double inv, std::vector<dou
     for (auto& t : ts) t.join();
 }
 
+std::vector<float> to_f32(const double* a, size_t n) {
+    std::vector<float> o(n);
+    for (size_t k = 0; k < n; ++k) o[k] = (float)a[k];
+    return o;
+}
+
 void free_list(std::vector<void*>& l) {
     for (void* p : l) (void)hipFree(p);
     l.clear();
+}
+
+// Current-buffer particle state in creation order as FP64 (both forms).
+slab_status gather_particles(slab_run* r, double* const dst[9]) {
+    TRY(pull_ctl(r));
+    const size_t n = (size_t)r->np;
+    const bool par = r->h_ctl->parity != 0;
+    std::vector<int32_t> pid(n);
+    if (!r->f32) {
+        const PBuf64& P = par ? r->B : r->A;
+        HIPCHK(hipMemcpy(pid.data(), P.pid, n * 4, hipMemcpyDeviceToHost));
+        std::vector<double> tmp(n);
+        double* srcs[] = {P.x, P.y, P.vx, P.vy, P.vz, P.z, P.h, P.A, P.m};
+        for (int f = 0; f < 9; ++f) {
+            if (!dst[f]) continue;
+            HIPCHK(hipMemcpy(tmp.data(), srcs[f], n * 8, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < n; ++k) dst[f][pid[k]] = tmp[k];
+        }
+        return SLAB_OK;
+    }
+    const PBuf32& P = par ? r->B32 : r->A32;
+    HIPCHK(hipMemcpy(pid.data(), P.pid, n * 4, hipMemcpyDeviceToHost));
+    std::vector<int32_t> cell(n);
+    std::vector<float> ox(n), oy(n), f(n), h(n), m(n);
+    HIPCHK(hipMemcpy(cell.data(), P.cell, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ox.data(), P.ox, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(oy.data(), P.oy, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h.data(), P.h, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(m.data(), P.m, n * 4, hipMemcpyDeviceToHost));
+    const double rho = r->h_ctl->rho;
+    for (size_t k = 0; k < n; ++k) {
+        const int ci = cell[k] / r->ny, cj = cell[k] - ci * r->ny;
+        const int q = pid[k];
+        if (dst[0]) dst[0][q] = ((double)ci + (double)ox[k]) * r->dx;
+        if (dst[1]) dst[1][q] = ((double)cj + (double)oy[k]) * r->dx;
+        if (dst[6]) dst[6][q] = h[k];
+        if (dst[7]) dst[7][q] = (double)m[k] / ((double)h[k] * rho);
+        if (dst[8]) dst[8][q] = m[k];
+    }
+    float* srcs[] = {P.vx, P.vy, P.vz, P.z};
+    const int slots[] = {2, 3, 4, 5};
+    for (int f2 = 0; f2 < 4; ++f2) {
+        if (!dst[slots[f2]]) continue;
+        HIPCHK(hipMemcpy(f.data(), srcs[f2], n * 4, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) dst[slots[f2]][pid[k]] = f[k];
+    }
+    return SLAB_OK;
 }
 
 } // namespace
@@ -257,7 +350,8 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     if (nx < 4 || ny < 4 || !terrain->z) return fail(SLAB_EINVAL, "terrain must have at least 4x4 nodes and z");
     if (!(terrain->dx > 0.0) || !std::isfinite(terrain->dx)) return fail(SLAB_EINVAL, "dx must be positive");
     if ((double)nx * (double)ny > 1.0e9) return fail(SLAB_EDOMAIN, "terrain larger than 1e9 nodes");
-    if (params->precision != SLAB_FP64) return fail(SLAB_EINVAL, "this build of the step supports SLAB_FP64");
+    if (params->precision != SLAB_FP64 && params->precision != SLAB_FP32)
+        return fail(SLAB_EINVAL, "precision must be SLAB_FP64 or SLAB_FP32");
     if (friction->type < 0 || friction->type > 2) return fail(SLAB_EINVAL, "unknown friction type");
     if (!(params->output_dt > 0.0) || !(params->dt_max > 0.0))
         return fail(SLAB_EINVAL, "output_dt and dt_max must be positive");
@@ -267,6 +361,7 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
 
     slab_run* r = new slab_run();
     r->device = info[0];
+    r->f32 = params->precision == SLAB_FP32;
     r->nx = nx;
     r->ny = ny;
     r->nn = (size_t)nx * ny;
@@ -287,19 +382,32 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     std::memset(r->h_ctl, 0, sizeof(Ctl));
     slab_status st = SLAB_OK;
     auto A = [&](auto** p, size_t n) {
-        if (st == SLAB_OK) st = dalloc(r, p, n, r->allocs);
+        if (st == SLAB_OK) st = dalloc(p, n, r->allocs);
     };
     A(&r->d_ctl, 1);
-    A(&r->tz, r->nn);
-    A(&r->tgx, r->nn);
-    A(&r->tgy, r->nn);
-    A(&r->tcth, r->nn);
-    A(&r->hmax, r->nn);
-    A(&r->vmax2, r->nn);
-    A(&r->vf, r->nn);
-    A(&r->gm, r->nn);
-    A(&r->gh, r->nn);
-    A(&r->gvz, r->nn);
+    if (r->f32) {
+        A(&r->tz32, r->nn);
+        A(&r->tgx32, r->nn);
+        A(&r->tgy32, r->nn);
+        A(&r->tcth32, r->nn);
+        A(&r->hmax32, r->nn);
+        A(&r->vmax2_32, r->nn);
+        A(&r->vf32, r->nn);
+        A(&r->gm32, r->nn);
+        A(&r->gh32, r->nn);
+        A(&r->gvz32, r->nn);
+    } else {
+        A(&r->tz, r->nn);
+        A(&r->tgx, r->nn);
+        A(&r->tgy, r->nn);
+        A(&r->tcth, r->nn);
+        A(&r->hmax, r->nn);
+        A(&r->vmax2, r->nn);
+        A(&r->vf, r->nn);
+        A(&r->gm, r->nn);
+        A(&r->gh, r->nn);
+        A(&r->gvz, r->nn);
+    }
     A(&r->tile_count, r->ntiles);
     A(&r->tile_begin, r->ntiles);
     A(&r->pstamp, r->ntiles);
@@ -316,20 +424,48 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     std::vector<double> gx, gy, cth;
     terrain_tables(terrain->z, nx, ny, r->inv_dx, gx, gy, cth);
     hipStream_t s = r->stream;
-    if (hipMemcpyAsync(r->tz, terrain->z, r->nn * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(r->tgx, gx.data(), r->nn * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(r->tgy, gy.data(), r->nn * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(r->tcth, cth.data(), r->nn * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemsetAsync(r->hmax, 0, r->nn * 8, s) != hipSuccess ||
-        hipMemsetAsync(r->vmax2, 0, r->nn * 8, s) != hipSuccess ||
-        hipMemsetAsync(r->vf, 0, r->nn * sizeof(double4), s) != hipSuccess ||
-        hipMemsetAsync(r->gm, 0, r->nn * 8, s) != hipSuccess || hipMemsetAsync(r->gh, 0, r->nn * 8, s) != hipSuccess ||
-        hipMemsetAsync(r->gvz, 0, r->nn * 8, s) != hipSuccess ||
-        hipMemsetAsync(r->tile_count, 0, (size_t)r->ntiles * 4, s) != hipSuccess ||
-        hipMemsetAsync(r->pstamp, 0xff, (size_t)r->ntiles * 4, s) != hipSuccess ||
-        hipMemsetAsync(r->gstamp, 0xff, (size_t)r->ntiles * 4, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return cleanup(fail(SLAB_EDEVICE, "terrain upload failed"));
+    bool ok = true;
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        ok = ok && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+    };
+    auto zero = [&](void* dst, size_t bytes, int v) { ok = ok && hipMemsetAsync(dst, v, bytes, s) == hipSuccess; };
+    auto sync = [&]() { ok = ok && hipStreamSynchronize(s) == hipSuccess; };
+    if (r->f32) {
+        std::vector<float> f = to_f32(terrain->z, r->nn);
+        up(r->tz32, f.data(), r->nn * 4);
+        sync();
+        f = to_f32(gx.data(), r->nn);
+        up(r->tgx32, f.data(), r->nn * 4);
+        sync();
+        f = to_f32(gy.data(), r->nn);
+        up(r->tgy32, f.data(), r->nn * 4);
+        sync();
+        f = to_f32(cth.data(), r->nn);
+        up(r->tcth32, f.data(), r->nn * 4);
+        sync();
+        zero(r->hmax32, r->nn * 4, 0);
+        zero(r->vmax2_32, r->nn * 4, 0);
+        zero(r->vf32, r->nn * sizeof(float4), 0);
+        zero(r->gm32, r->nn * 4, 0);
+        zero(r->gh32, r->nn * 4, 0);
+        zero(r->gvz32, r->nn * 4, 0);
+    } else {
+        up(r->tz, terrain->z, r->nn * 8);
+        up(r->tgx, gx.data(), r->nn * 8);
+        up(r->tgy, gy.data(), r->nn * 8);
+        up(r->tcth, cth.data(), r->nn * 8);
+        zero(r->hmax, r->nn * 8, 0);
+        zero(r->vmax2, r->nn * 8, 0);
+        zero(r->vf, r->nn * sizeof(double4), 0);
+        zero(r->gm, r->nn * 8, 0);
+        zero(r->gh, r->nn * 8, 0);
+        zero(r->gvz, r->nn * 8, 0);
+    }
+    zero(r->tile_count, (size_t)r->ntiles * 4, 0);
+    zero(r->pstamp, (size_t)r->ntiles * 4, 0xff);
+    zero(r->gstamp, (size_t)r->ntiles * 4, 0xff);
+    sync();
+    if (!ok) return cleanup(fail(SLAB_EDEVICE, "terrain upload failed"));
 
     Ctl& c = *r->h_ctl;
     c.gravity = params->gravity;
@@ -354,7 +490,6 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     c.ntx = r->ntx;
     c.nty = r->nty;
     c.stop = RUNNING;
-    c.bb_next[0] = c.bb_next[2] = ~0ull;
     st = push_ctl(r);
     if (st != SLAB_OK) return cleanup(st);
     *out = r;
@@ -369,40 +504,95 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
     free_list(r->part_allocs);
     r->np = np;
     r->primed = false;
+    const size_t n = (size_t)np;
     slab_status st = SLAB_OK;
-    auto P = [&](PBuf64& b) {
-        double** f[] = {&b.x, &b.y, &b.vx, &b.vy, &b.vz, &b.h, &b.A, &b.m, &b.z};
-        for (double** q : f)
-            if (st == SLAB_OK) st = dalloc(r, q, (size_t)np, r->part_allocs);
-        if (st == SLAB_OK) st = dalloc(r, &b.pid, (size_t)np, r->part_allocs);
+    auto P = [&](auto** q) {
+        if (st == SLAB_OK) st = dalloc(q, n, r->part_allocs);
     };
-    P(r->A);
-    P(r->B);
-    if (st == SLAB_OK) st = dalloc(r, &r->key, (size_t)np, r->part_allocs);
-    if (st == SLAB_OK) st = dalloc(r, &r->tslot, (size_t)np, r->part_allocs);
-    if (st == SLAB_OK) st = dalloc(r, &r->idx, (size_t)np, r->part_allocs);
-    if (st == SLAB_OK) st = dalloc(r, &r->tmp, (size_t)np, r->part_allocs);
+    if (r->f32) {
+        for (PBuf32* b : {&r->A32, &r->B32}) {
+            P(&b->cell);
+            for (float** q : {&b->ox, &b->oy, &b->vx, &b->vy, &b->vz, &b->h, &b->m, &b->z}) P(q);
+            P(&b->pid);
+        }
+        for (float** q : {&r->Y32.fx, &r->Y32.fy, &r->Y32.m, &r->Y32.vmx, &r->Y32.vmy, &r->Y32.vzm, &r->Y32.hm,
+                          &r->Y32.mbx, &r->Y32.mby, &r->Y32.ps})
+            P(q);
+    } else {
+        for (PBuf64* b : {&r->A, &r->B}) {
+            for (double** q : {&b->x, &b->y, &b->vx, &b->vy, &b->vz, &b->h, &b->A, &b->m, &b->z}) P(q);
+            P(&b->pid);
+        }
+        for (double** q : {&r->Y.x, &r->Y.y, &r->Y.m, &r->Y.vmx, &r->Y.vmy, &r->Y.vzm, &r->Y.hm, &r->Y.hA, &r->Y.mbx,
+                           &r->Y.mby, &r->Y.s00})
+            P(q);
+    }
+    P(&r->key);
+    P(&r->tslot);
+    P(&r->idx);
+    P(&r->tmp);
+    if (st == SLAB_OK) st = dalloc(&r->part, 5 * (size_t)((np + WG - 1) / WG), r->part_allocs);
     if (st != SLAB_OK) return st;
     TRY(pull_ctl(r));
     const double rho = r->h_ctl->rho;
-    std::vector<double> mm((size_t)np), zero((size_t)np, 0.0);
-    std::vector<int32_t> pid((size_t)np);
-    for (int64_t p = 0; p < np; ++p) {
+    std::vector<double> mm(n), zero(n, 0.0);
+    std::vector<int32_t> pid(n);
+    for (size_t p = 0; p < n; ++p) {
         mm[p] = m ? m[p] : rho * h[p] * A[p]; // simulation.cpp:452
         pid[p] = (int32_t)p;
     }
     hipStream_t s = r->stream;
-    const size_t nb = (size_t)np * 8;
-    HIPCHK(hipMemcpyAsync(r->A.x, x, nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(r->A.y, y, nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(r->A.vx, vx ? vx : zero.data(), nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(r->A.vy, vy ? vy : zero.data(), nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(r->A.vz, vz ? vz : zero.data(), nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(r->A.h, h, nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(r->A.A, A, nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(r->A.m, mm.data(), nb, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(r->A.pid, pid.data(), (size_t)np * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
+    if (r->f32) {
+        std::vector<int32_t> cell(n);
+        std::vector<float> ox(n), oy(n);
+        for (size_t p = 0; p < n; ++p) {
+            const double fx = x[p] * r->inv_dx, fy = y[p] * r->inv_dx;
+            double ci = std::floor(fx), cj = std::floor(fy);
+            if (!std::isfinite(ci)) ci = 0;
+            if (!std::isfinite(cj)) cj = 0;
+            ci = std::min(std::max(ci, 0.0), (double)(r->nx - 1));
+            cj = std::min(std::max(cj, 0.0), (double)(r->ny - 1));
+            float a = (float)(fx - ci), b = (float)(fy - cj);
+            int ii = (int)ci, jj = (int)cj;
+            if (a >= 1.f) {
+                a = 0.f;
+                ii = std::min(ii + 1, r->nx - 1);
+            }
+            if (b >= 1.f) {
+                b = 0.f;
+                jj = std::min(jj + 1, r->ny - 1);
+            }
+            if (!(a >= 0.f)) a = 0.f;
+            if (!(b >= 0.f)) b = 0.f;
+            cell[p] = ii * r->ny + jj;
+            ox[p] = a;
+            oy[p] = b;
+        }
+        std::vector<float> fvx = to_f32(vx ? vx : zero.data(), n), fvy = to_f32(vy ? vy : zero.data(), n);
+        std::vector<float> fvz = to_f32(vz ? vz : zero.data(), n), fh = to_f32(h, n), fm = to_f32(mm.data(), n);
+        HIPCHK(hipMemcpyAsync(r->A32.cell, cell.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A32.ox, ox.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A32.oy, oy.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A32.vx, fvx.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A32.vy, fvy.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A32.vz, fvz.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A32.h, fh.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A32.m, fm.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A32.pid, pid.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    } else {
+        const size_t nb = n * 8;
+        HIPCHK(hipMemcpyAsync(r->A.x, x, nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A.y, y, nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A.vx, vx ? vx : zero.data(), nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A.vy, vy ? vy : zero.data(), nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A.vz, vz ? vz : zero.data(), nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A.h, h, nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A.A, A, nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A.m, mm.data(), nb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(r->A.pid, pid.data(), n * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
     Ctl& c = *r->h_ctl;
     c.np = (int32_t)np;
     c.time = 0.0;
@@ -413,7 +603,10 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
     c.stop = RUNNING;
     c.err = 0;
     TRY(push_ctl(r));
-    k_init_z<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->tz);
+    if (r->f32)
+        k_init_z32<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->tz32);
+    else
+        k_init_z<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->tz);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     return SLAB_OK;
@@ -426,15 +619,15 @@ slab_status slab_run_prime(slab_run* r) {
     hipStream_t s = r->stream;
     HIPCHK(hipMemsetAsync(r->tile_count, 0, (size_t)r->ntiles * 4, s));
     k_prime_pre<<<1, 1, 0, s>>>(r->d_ctl);
-    k_keys<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->key, r->tslot, r->tile_count);
-    k_ctl_prime<<<1, 1, 0, s>>>(r->d_ctl);
+    enqueue_keys(r);
+    k_ctl_prime<<<1, WG, 0, s>>>(r->d_ctl, r->part);
     bool tm = r->timing;
     r->timing = false;
     enqueue_sort(r, 0);
     enqueue_p2g(r, 0);
     r->timing = tm;
     // keys again for step 1 (the tile sort consumed the counts)
-    k_keys<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->key, r->tslot, r->tile_count);
+    enqueue_keys(r);
     HIPCHK(hipGetLastError());
     TRY(pull_ctl(r));
     r->primed = true;
@@ -478,20 +671,8 @@ slab_status slab_run_download_particles(slab_run* r, double* x, double* y, doubl
                                         double* z, double* h, double* A, double* m) {
     if (!r) return fail(SLAB_EINVAL, "null run");
     if (r->np < 1) return fail(SLAB_ESTATE, "no particles");
-    TRY(pull_ctl(r));
-    const PBuf64& P = r->h_ctl->parity ? r->B : r->A;
-    const size_t n = (size_t)r->np;
-    std::vector<int32_t> pid(n);
-    std::vector<double> tmp(n);
-    HIPCHK(hipMemcpy(pid.data(), P.pid, n * 4, hipMemcpyDeviceToHost));
-    double* dsts[] = {x, y, vx, vy, vz, z, h, A, m};
-    double* srcs[] = {P.x, P.y, P.vx, P.vy, P.vz, P.z, P.h, P.A, P.m};
-    for (int f = 0; f < 9; ++f) {
-        if (!dsts[f]) continue;
-        HIPCHK(hipMemcpy(tmp.data(), srcs[f], n * 8, hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < n; ++k) dsts[f][pid[k]] = tmp[k];
-    }
-    return SLAB_OK;
+    double* const dst[9] = {x, y, vx, vy, vz, z, h, A, m};
+    return gather_particles(r, dst);
 }
 
 slab_status slab_run_keep_grid(slab_run* r, int32_t enable) {
@@ -513,10 +694,11 @@ slab_status slab_run_download_grid(slab_run* r, int32_t* i0, int32_t* j0, int32_
     const double min_x = ord_value(c.bb_cur[0]), max_x = ord_value(c.bb_cur[1]);
     const double min_y = ord_value(c.bb_cur[2]), max_y = ord_value(c.bb_cur[3]);
     const double eps = 1e-8 * r->dx, inv = r->inv_dx;
-    int iL = (int)std::max(-1e9, std::min(1e9, (min_x - 0.0) * inv - 1.0 - eps));
-    int iR = (int)std::max(-1e9, std::min(1e9, (max_x - 0.0) * inv + 2.0 + eps));
-    int jB = (int)std::max(-1e9, std::min(1e9, (min_y - 0.0) * inv - 1.0 - eps));
-    int jT = (int)std::max(-1e9, std::min(1e9, (max_y - 0.0) * inv + 2.0 + eps));
+    auto clampi = [](double v) { return (int)std::max(-1e9, std::min(1e9, v)); };
+    int iL = clampi((min_x - 0.0) * inv - 1.0 - eps);
+    int iR = clampi((max_x - 0.0) * inv + 2.0 + eps);
+    int jB = clampi((min_y - 0.0) * inv - 1.0 - eps);
+    int jT = clampi((max_y - 0.0) * inv + 2.0 + eps);
     iL = std::max(iL, 0);
     jB = std::max(jB, 0);
     iR = std::min(iR, r->nx - 1);
@@ -531,11 +713,15 @@ slab_status slab_run_download_grid(slab_run* r, int32_t* i0, int32_t* j0, int32_
     const size_t wn = (size_t)gnx * gny;
     std::vector<void*> tmpl;
     double* o = nullptr;
-    slab_status st = dalloc(r, &o, 5 * wn, tmpl);
+    slab_status st = dalloc(&o, 5 * wn, tmpl);
     if (st != SLAB_OK) return st;
-    k_window<<<(unsigned)((wn + WG - 1) / WG), WG, 0, r->stream>>>(r->d_ctl, iL, jB, gnx, gny, r->gstamp, r->vf, r->gm,
-                                                                   r->gh, r->gvz, o, o + wn, o + 2 * wn, o + 3 * wn,
-                                                                   o + 4 * wn);
+    const unsigned nb = (unsigned)((wn + WG - 1) / WG);
+    if (r->f32)
+        k_window32<<<nb, WG, 0, r->stream>>>(r->d_ctl, iL, jB, gnx, gny, r->gstamp, r->vf32, r->gm32, r->gh32,
+                                             r->gvz32, o, o + wn, o + 2 * wn, o + 3 * wn, o + 4 * wn);
+    else
+        k_window<<<nb, WG, 0, r->stream>>>(r->d_ctl, iL, jB, gnx, gny, r->gstamp, r->vf, r->gm, r->gh, r->gvz, o,
+                                           o + wn, o + 2 * wn, o + 3 * wn, o + 4 * wn);
     hipError_t e = hipStreamSynchronize(r->stream);
     double* dst[] = {h, vx, vy, vz, m};
     for (int f = 0; f < 5 && e == hipSuccess; ++f)
@@ -548,8 +734,19 @@ slab_status slab_run_download_grid(slab_run* r, int32_t* i0, int32_t* j0, int32_
 slab_status slab_run_download_max(slab_run* r, double* h_max, double* v_max2) {
     if (!r) return fail(SLAB_EINVAL, "null run");
     HIPCHK(hipStreamSynchronize(r->stream));
-    if (h_max) HIPCHK(hipMemcpy(h_max, r->hmax, r->nn * 8, hipMemcpyDeviceToHost));
-    if (v_max2) HIPCHK(hipMemcpy(v_max2, r->vmax2, r->nn * 8, hipMemcpyDeviceToHost));
+    if (!r->f32) {
+        if (h_max) HIPCHK(hipMemcpy(h_max, r->hmax, r->nn * 8, hipMemcpyDeviceToHost));
+        if (v_max2) HIPCHK(hipMemcpy(v_max2, r->vmax2, r->nn * 8, hipMemcpyDeviceToHost));
+        return SLAB_OK;
+    }
+    std::vector<float> f(r->nn);
+    double* dst[] = {h_max, v_max2};
+    float* src[] = {r->hmax32, r->vmax2_32};
+    for (int k = 0; k < 2; ++k) {
+        if (!dst[k]) continue;
+        HIPCHK(hipMemcpy(f.data(), src[k], r->nn * 4, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < r->nn; ++q) dst[k][q] = f[q];
+    }
     return SLAB_OK;
 }
 
@@ -558,20 +755,20 @@ slab_status slab_run_diag(slab_run* r, double out[8]) {
     if (!r->primed) return fail(SLAB_ESTATE, "run not primed");
     TRY(pull_ctl(r));
     hipStream_t s = r->stream;
-    k_diag_grid<<<256, WG, 0, s>>>(r->d_ctl, r->gtiles, r->vf, r->gm, r->diag_part);
-    k_diag_particles<<<256, WG, 0, s>>>(r->d_ctl, r->A, r->B, r->diag_part + 4 * 256);
-    k_diag_final<<<1, 64, 0, s>>>(r->diag_part, 256, r->diag_part + 4 * 256, 256, r->diag_out);
+    if (r->f32) {
+        k_diag_grid32<<<256, WG, 0, s>>>(r->d_ctl, r->gtiles, r->vf32, r->gm32, r->diag_part);
+        k_diag_particles32<<<256, WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->diag_part + 4 * 256);
+    } else {
+        k_diag_grid<<<256, WG, 0, s>>>(r->d_ctl, r->gtiles, r->vf, r->gm, r->diag_part);
+        k_diag_particles<<<256, WG, 0, s>>>(r->d_ctl, r->A, r->B, r->diag_part + 4 * 256);
+    }
+    k_diag_final<<<1, WG, 0, s>>>(r->d_ctl, r->diag_part, 256, r->diag_part + 4 * 256, 256, r->part, r->diag_out);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, r->diag_out, 8 * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     const Ctl& c = *r->h_ctl;
     out[5] = c.dt;
-    union { unsigned long long u; double d; } v;
-    v.u = c.vmax2_bits;
-    out[6] = std::sqrt(v.d);
-    if (c.last_out_epoch != c.epoch) {
-        out[0] = out[1] = out[2] = out[7] = std::nan("");
-    }
+    if (c.last_out_epoch != c.epoch) out[0] = out[1] = out[2] = out[7] = std::nan("");
     return SLAB_OK;
 }
 
@@ -594,18 +791,31 @@ slab_status slab_run_timing(slab_run* r, int32_t enable, double* ms, int64_t* co
 slab_status slab_run_set_velocity(slab_run* r, const double* vx, const double* vy) {
     if (!r || !vx || !vy) return fail(SLAB_EINVAL, "null argument");
     if (r->primed) return fail(SLAB_ESTATE, "set the velocity before slab_run_prime");
+    if (r->np < 1) return fail(SLAB_ESTATE, "no particles");
     TRY(pull_ctl(r));
-    const PBuf64& P = r->h_ctl->parity ? r->B : r->A;
+    const bool par = r->h_ctl->parity != 0;
     const size_t n = (size_t)r->np;
     std::vector<int32_t> pid(n);
-    std::vector<double> a(n), b(n);
-    HIPCHK(hipMemcpy(pid.data(), P.pid, n * 4, hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < n; ++k) {
-        a[k] = vx[pid[k]];
-        b[k] = vy[pid[k]];
+    HIPCHK(hipMemcpy(pid.data(), par ? r->pidB() : r->pidA(), n * 4, hipMemcpyDeviceToHost));
+    if (r->f32) {
+        std::vector<float> a(n), b(n);
+        for (size_t k = 0; k < n; ++k) {
+            a[k] = (float)vx[pid[k]];
+            b[k] = (float)vy[pid[k]];
+        }
+        const PBuf32& P = par ? r->B32 : r->A32;
+        HIPCHK(hipMemcpy(P.vx, a.data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(P.vy, b.data(), n * 4, hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> a(n), b(n);
+        for (size_t k = 0; k < n; ++k) {
+            a[k] = vx[pid[k]];
+            b[k] = vy[pid[k]];
+        }
+        const PBuf64& P = par ? r->B : r->A;
+        HIPCHK(hipMemcpy(P.vx, a.data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(P.vy, b.data(), n * 8, hipMemcpyHostToDevice));
     }
-    HIPCHK(hipMemcpy(P.vx, a.data(), n * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(P.vy, b.data(), n * 8, hipMemcpyHostToDevice));
     return SLAB_OK;
 }
 

@@ -1,0 +1,128 @@
+// slab_device.hpp -- device helpers shared by the FP64 and FP32 forms:
+// wavefront (64-lane) reductions, per-block partials of max |v|^2 and the
+// particle bounding box, and run-aggregated tile slot allocation.
+#pragma once
+#include "slab_common.hpp"
+
+namespace slab {
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Block-wide (256 threads = 4 waves) reduction of |v|^2 max and the bounding
+// box of new positions into one partial per block (no global atomics: the
+// consumer reduces the partials).  |v|^2 >= 0, so its IEEE bits order like
+// integers; NaN bits sort above +inf and are caught by ctl_begin.
+__device__ __forceinline__ void block_partials(double v2, unsigned long long bx0, unsigned long long bx1, unsigned long long by0,
+                               unsigned long long by1, unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long s_b[4][5];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long vbits = wave_max_u64((unsigned long long)__double_as_longlong(v2));
+    bx0 = wave_min_u64(bx0);
+    bx1 = wave_max_u64(bx1);
+    by0 = wave_min_u64(by0);
+    by1 = wave_max_u64(by1);
+    if (lane == 0) {
+        s_b[wv][0] = bx0;
+        s_b[wv][1] = bx1;
+        s_b[wv][2] = by0;
+        s_b[wv][3] = by1;
+        s_b[wv][4] = vbits;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long vmax = 0, a0 = ~0ull, a1 = 0, a2 = ~0ull, a3 = 0;
+        for (int k = 0; k < 4; ++k) {
+            vmax = s_b[k][4] > vmax ? s_b[k][4] : vmax;
+            a0 = s_b[k][0] < a0 ? s_b[k][0] : a0;
+            a1 = s_b[k][1] > a1 ? s_b[k][1] : a1;
+            a2 = s_b[k][2] < a2 ? s_b[k][2] : a2;
+            a3 = s_b[k][3] > a3 ? s_b[k][3] : a3;
+        }
+        unsigned long long* q = part + 5 * (size_t)blockIdx.x;
+        q[0] = a0;
+        q[1] = a1;
+        q[2] = a2;
+        q[3] = a3;
+        q[4] = vmax;
+    }
+}
+
+// Reduces the per-block partials (all 256 threads of a 1-block launch);
+// result in out[5] on thread 0.
+__device__ __forceinline__ void reduce_partials(const unsigned long long* __restrict__ part, int nb, unsigned long long out[5]) {
+    __shared__ unsigned long long s_r[4][5];
+    unsigned long long a0 = ~0ull, a1 = 0, a2 = ~0ull, a3 = 0, v = 0;
+    for (int b = threadIdx.x; b < nb; b += WG) {
+        const unsigned long long* q = part + 5 * (size_t)b;
+        a0 = q[0] < a0 ? q[0] : a0;
+        a1 = q[1] > a1 ? q[1] : a1;
+        a2 = q[2] < a2 ? q[2] : a2;
+        a3 = q[3] > a3 ? q[3] : a3;
+        v = q[4] > v ? q[4] : v;
+    }
+    a0 = wave_min_u64(a0);
+    a1 = wave_max_u64(a1);
+    a2 = wave_min_u64(a2);
+    a3 = wave_max_u64(a3);
+    v = wave_max_u64(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_r[wv][0] = a0;
+        s_r[wv][1] = a1;
+        s_r[wv][2] = a2;
+        s_r[wv][3] = a3;
+        s_r[wv][4] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = ~0ull;
+        out[1] = 0;
+        out[2] = ~0ull;
+        out[3] = 0;
+        out[4] = 0;
+        for (int k = 0; k < 4; ++k) {
+            out[0] = s_r[k][0] < out[0] ? s_r[k][0] : out[0];
+            out[1] = s_r[k][1] > out[1] ? s_r[k][1] : out[1];
+            out[2] = s_r[k][2] < out[2] ? s_r[k][2] : out[2];
+            out[3] = s_r[k][3] > out[3] ? s_r[k][3] : out[3];
+            out[4] = s_r[k][4] > out[4] ? s_r[k][4] : out[4];
+        }
+    }
+}
+
+// Slot of each lane's particle inside its tile.  Lanes of a wave hold
+// consecutive sorted particles, so equal tiles form runs: one atomicAdd per run
+// (by its first lane) instead of one per particle.  Valid lanes must be a
+// prefix of the wave; every lane of the wave must call this.
+__device__ __forceinline__ int wave_tile_slot(int t, bool valid, int* __restrict__ tile_count) {
+    const int lane = threadIdx.x & 63;
+    const int tprev = __shfl_up(t, 1, 64);
+    const bool head = valid && (lane == 0 || tprev != t);
+    const unsigned long long vmask = __ballot(valid);
+    const unsigned long long hmask = __ballot(head);
+    const int nvalid = __popcll(vmask);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long hle = hmask & upto;
+    const unsigned long long hgt = hmask & ~upto;
+    const int start = hle ? 63 - __clzll((long long)hle) : 0;
+    const int end = hgt ? (__ffsll((long long)hgt) - 1) : nvalid;
+    int base = 0;
+    if (head) base = atomicAdd(&tile_count[t], end - start);
+    base = __shfl(base, start, 64);
+    return base + (lane - start);
+}
+
+} // namespace slab

@@ -15,6 +15,7 @@
 // creation indices -- so results are reproducible run to run.
 #include "slab_common.hpp"
 #include "slab_kernels.hpp"
+#include "slab_device.hpp"
 
 #pragma clang fp contract(off)
 
@@ -60,60 +61,6 @@ __device__ __forceinline__ PBuf64 nxt_buf(const Ctl* c, const PBuf64& A, const P
     return c->parity ? A : B;
 }
 
-__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
-// Block-wide (256 threads = 4 waves) reduction of |v|^2 max and the bounding
-// box of new positions, then one atomic per block into Ctl.  |v|^2 >= 0, so its
-// IEEE bits order like integers; NaN bits sort above +inf and are caught by
-// ctl_begin.
-__device__ void block_publish(Ctl* c, double v2, unsigned long long bx0, unsigned long long bx1,
-                              unsigned long long by0, unsigned long long by1) {
-    __shared__ unsigned long long s_b[4][5];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // |v|^2 >= 0 (or NaN, whose bits sort above +inf): integer max of the bits
-    unsigned long long vbits = wave_max_u64((unsigned long long)__double_as_longlong(v2));
-    bx0 = wave_min_u64(bx0);
-    bx1 = wave_max_u64(bx1);
-    by0 = wave_min_u64(by0);
-    by1 = wave_max_u64(by1);
-    if (lane == 0) {
-        s_b[wv][0] = bx0;
-        s_b[wv][1] = bx1;
-        s_b[wv][2] = by0;
-        s_b[wv][3] = by1;
-        s_b[wv][4] = vbits;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long vmax = 0, a0 = ~0ull, a1 = 0, a2 = ~0ull, a3 = 0;
-        for (int k = 0; k < 4; ++k) {
-            vmax = s_b[k][4] > vmax ? s_b[k][4] : vmax;
-            a0 = s_b[k][0] < a0 ? s_b[k][0] : a0;
-            a1 = s_b[k][1] > a1 ? s_b[k][1] : a1;
-            a2 = s_b[k][2] < a2 ? s_b[k][2] : a2;
-            a3 = s_b[k][3] > a3 ? s_b[k][3] : a3;
-        }
-        atomicMax(&c->vmax2_bits, vmax);
-        atomicMin(&c->bb_next[0], a0);
-        atomicMax(&c->bb_next[1], a1);
-        atomicMin(&c->bb_next[2], a2);
-        atomicMax(&c->bb_next[3], a3);
-    }
-}
-
 // Key of a position: cell = stencil base node, grouped by 16x16 tile.
 __device__ __forceinline__ int cell_key(const Ctl* c, double x, double y, int* tile_out) {
     const int ib = base_index(x, c->inv_dx, c->nx - 3);
@@ -132,13 +79,20 @@ __global__ void k_ctl_clear(Ctl* c) {
 }
 
 // updateDt, src/simulation.cpp:565-582, plus the bookkeeping of one step.
-__global__ void k_ctl_begin(Ctl* c) {
+// One block of 256 threads: reduces the partials of the previous g2p (or of
+// k_keys after upload) to max |v_p|^2 and the bounding box of the positions
+// this step's P2G uses.
+__global__ void __launch_bounds__(WG) k_ctl_begin(Ctl* c, const unsigned long long* __restrict__ part) {
     if (c->stop) return;
+    __shared__ unsigned long long s_out[5];
+    reduce_partials(part, (c->np + WG - 1) / WG, s_out);
+    if (threadIdx.x != 0) return;
     if (!(c->time < c->total_time)) { // while condition, simulation.cpp:500
         c->stop = STOP_DONE;
         return;
     }
-    const double v2 = __longlong_as_double(c->vmax2_bits);
+    c->vmax2_bits = s_out[4];
+    const double v2 = __longlong_as_double(s_out[4]);
     if (!(v2 <= 1.7976931348623157e308)) {
         c->err = 5; // SLAB_ENAN
         c->stop = STOP_ERROR;
@@ -161,12 +115,7 @@ __global__ void k_ctl_begin(Ctl* c) {
     const int hit = (c->output_dt * (double)(c->frame + 1) - tnew < 1e-13);
     c->out_step = hit || c->keep_grid;
     if (c->out_step) c->last_out_epoch = c->epoch;
-    c->vmax2_bits = 0ull;
-    for (int k = 0; k < 4; ++k) c->bb_cur[k] = c->bb_next[k];
-    c->bb_next[0] = ~0ull;
-    c->bb_next[1] = 0ull;
-    c->bb_next[2] = ~0ull;
-    c->bb_next[3] = 0ull;
+    for (int k = 0; k < 4; ++k) c->bb_cur[k] = s_out[k];
 }
 
 // time += dt and the frame trigger, src/simulation.cpp:503-509.
@@ -182,43 +131,46 @@ __global__ void k_ctl_end(Ctl* c) {
     }
 }
 
-__global__ void k_prime_pre(Ctl* c) {
-    c->vmax2_bits = 0ull;
-    c->bb_next[0] = ~0ull;
-    c->bb_next[1] = 0ull;
-    c->bb_next[2] = ~0ull;
-    c->bb_next[3] = 0ull;
-    c->stop = RUNNING;
-}
+__global__ void k_prime_pre(Ctl* c) { c->stop = RUNNING; }
 
 // simulate() priming with dt = 0 (src/simulation.cpp:493-495, simulation.hpp:39).
-__global__ void k_ctl_prime(Ctl* c) {
+__global__ void __launch_bounds__(WG) k_ctl_prime(Ctl* c, const unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long s_out[5];
+    reduce_partials(part, (c->np + WG - 1) / WG, s_out);
+    if (threadIdx.x != 0) return;
     c->dt = 0.0;
     c->epoch += 1;
     c->out_step = 1;
     c->last_out_epoch = c->epoch;
-    for (int k = 0; k < 4; ++k) c->bb_cur[k] = c->bb_next[k];
+    c->vmax2_bits = s_out[4];
+    for (int k = 0; k < 4; ++k) c->bb_cur[k] = s_out[k];
 }
 
 // Keys, tile counts, |v|^2 max and bounding box of the current particles
 // (used once after upload; afterwards g2p produces them).
-__global__ void k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* __restrict__ key, int* __restrict__ tslot,
-                       int* __restrict__ tile_count) {
+__global__ void __launch_bounds__(WG) k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* __restrict__ key,
+                                             int* __restrict__ tslot, int* __restrict__ tile_count,
+                                             unsigned long long* __restrict__ part) {
     const PBuf64 P = cur_buf(c, A, B);
     const int s = blockIdx.x * WG + threadIdx.x;
+    const bool valid = s < c->np;
     double v2 = 0.0;
     unsigned long long bx0 = ~0ull, bx1 = 0, by0 = ~0ull, by1 = 0;
-    if (s < c->np) {
+    int t = 0, k = 0;
+    if (valid) {
         const double x = P.x[s], y = P.y[s];
-        int t;
-        key[s] = cell_key(c, x, y, &t);
-        tslot[s] = atomicAdd(&tile_count[t], 1);
+        k = cell_key(c, x, y, &t);
         const double vx = P.vx[s], vy = P.vy[s];
         v2 = vx * vx + vy * vy;
         bx0 = bx1 = ord_bits(x);
         by0 = by1 = ord_bits(y);
     }
-    block_publish(c, v2, bx0, bx1, by0, by1);
+    const int slot = wave_tile_slot(t, valid, tile_count);
+    if (valid) {
+        key[s] = k;
+        tslot[s] = slot;
+    }
+    block_partials(v2, bx0, bx1, by0, by1, part);
 }
 
 // Initial particle z: 4x4 quadratic interpolation of terrain z,
@@ -372,7 +324,8 @@ __global__ void k_scatter(Ctl* c, const int* __restrict__ key, const int* __rest
 // by creation index, giving the canonical order (tile, cell, creation index).
 // Writes idx (sorted position -> storage slot) and cell_begin, and clears the
 // tile's count for the next step's g2p.
-__global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, PBuf64 A, PBuf64 B, const int* __restrict__ key,
+__global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, const int32_t* __restrict__ pidA,
+                                                  const int32_t* __restrict__ pidB, const int* __restrict__ key,
                                                   const int* __restrict__ ptiles, int* __restrict__ tile_count,
                                                   const int* __restrict__ tile_begin, const int* __restrict__ tmp,
                                                   int* __restrict__ idx, int* __restrict__ cell_begin) {
@@ -380,7 +333,7 @@ __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, PBuf64 A, PBuf64 B, co
     __shared__ int s_hist[TILE_CELLS];
     __shared__ int s_off[TILE_CELLS];
     __shared__ int s_n;
-    const PBuf64 P = cur_buf(c, A, B);
+    const int32_t* __restrict__ pid = c->parity ? pidB : pidA;
     const int nt = c->n_ptiles;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         const int t = ptiles[w];
@@ -428,9 +381,9 @@ __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, PBuf64 A, PBuf64 B, co
             const int hi = b + s_hist[threadIdx.x];
             for (int q = lo + 1; q < hi; ++q) {
                 const int sv = idx[q];
-                const int pv = P.pid[sv];
+                const int pv = pid[sv];
                 int k = q - 1;
-                while (k >= lo && P.pid[idx[k]] > pv) {
+                while (k >= lo && pid[idx[k]] > pv) {
                     idx[k + 1] = idx[k];
                     --k;
                 }
@@ -443,22 +396,50 @@ __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, PBuf64 A, PBuf64 B, co
 
 // ---------------------------------------------------------------- P2G + grid
 
+// Per-particle P2G terms of src/simulation.cpp:732-753 (the quantities that do
+// not depend on the node), computed once per particle in sorted order so the
+// node gather reads them contiguously by cell.
+__global__ void __launch_bounds__(WG) k_payload(Ctl* c, PBuf64 A, PBuf64 B, const int* __restrict__ idx,
+                                                const double* __restrict__ tgx, const double* __restrict__ tgy,
+                                                const double* __restrict__ tcth, Pay64 Y) {
+    if (c->stop) return;
+    const int r = blockIdx.x * WG + threadIdx.x;
+    if (r >= c->np) return;
+    const PBuf64 P = cur_buf(c, A, B);
+    const int s = idx[r];
+    const double inv = c->inv_dx, g = c->gravity, rho = c->rho;
+    const double xp = P.x[s], yp = P.y[s], hp = P.h[s], Ap = P.A[s], mp = P.m[s];
+    const size_t fc = (size_t)floor_index(xp, inv, c->nx - 1) * c->ny + floor_index(yp, inv, c->ny - 1);
+    const double cth = tcth[fc];
+    const double coeff = -mp * g * cth * cth;
+    Y.x[r] = xp;
+    Y.y[r] = yp;
+    Y.m[r] = mp;
+    Y.vmx[r] = P.vx[s] * mp;
+    Y.vmy[r] = P.vy[s] * mp;
+    Y.vzm[r] = P.vz[s] * mp;
+    Y.hm[r] = hp * mp;
+    Y.hA[r] = hp * Ap;
+    Y.mbx[r] = coeff * tgx[fc];
+    Y.mby[r] = coeff * tgy[fc];
+    Y.s00[r] = -0.5 * rho * g * cth * cth * hp;
+}
+
 // One workgroup per grid tile (persistent over the list), one thread per node.
 // P2G (src/simulation.cpp:726-798) as a gather: the node sums the
 // contributions of the particles of the 9 cells whose stencil covers it, in
 // canonical order -- no atomics, reproducible.  Then computeGridVelocity
 // (src/simulation.cpp:800-875) for that node in registers.
-__global__ void __launch_bounds__(WG) k_p2g_grid(Ctl* c, PBuf64 A, PBuf64 B, const int* __restrict__ gtiles,
+__global__ void __launch_bounds__(WG) k_p2g_grid(Ctl* c, Pay64 Y, const int* __restrict__ gtiles,
                                                  const int* __restrict__ pstamp, const int* __restrict__ cell_begin,
-                                                 const int* __restrict__ idx, const double* __restrict__ tgx,
-                                                 const double* __restrict__ tgy, const double* __restrict__ tcth,
-                                                 double* __restrict__ h_max, double* __restrict__ v_max2,
-                                                 double4* __restrict__ vf, double* __restrict__ out_m,
-                                                 double* __restrict__ out_h, double* __restrict__ out_vz) {
+                                                 const double* __restrict__ tgx, const double* __restrict__ tgy,
+                                                 const double* __restrict__ tcth, double* __restrict__ h_max,
+                                                 double* __restrict__ v_max2, double4* __restrict__ vf,
+                                                 double* __restrict__ out_m, double* __restrict__ out_h,
+                                                 double* __restrict__ out_vz) {
     if (c->stop) return;
-    const PBuf64 P = cur_buf(c, A, B);
     const int nt = c->n_gtiles, ep = c->epoch, nx = c->nx, ny = c->ny, nty = c->nty;
-    const double inv = c->inv_dx, dx = c->dx, g = c->gravity, rho = c->rho, dt = c->dt;
+    const double inv = c->inv_dx, dx = c->dx, g = c->gravity, dt = c->dt;
     const int out = c->out_step;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         const int t = gtiles[w];
@@ -466,7 +447,7 @@ __global__ void __launch_bounds__(WG) k_p2g_grid(Ctl* c, PBuf64 A, PBuf64 B, con
         const int i = tx * TILE + (threadIdx.x >> 4);
         const int j = ty * TILE + (threadIdx.x & 15);
         if (i >= nx || j >= ny) continue;
-        const double X = (double)i * dx, Y = (double)j * dx;
+        const double X = (double)i * dx, Y0 = (double)j * dx;
         double am = 0.0, apx = 0.0, apy = 0.0, apz = 0.0, ah = 0.0, afx = 0.0, afy = 0.0;
         for (int ci = i - 2; ci <= i; ++ci) {
             if (ci < 0 || ci > nx - 3) continue;
@@ -478,29 +459,19 @@ __global__ void __launch_bounds__(WG) k_p2g_grid(Ctl* c, PBuf64 A, PBuf64 B, con
                 const int r0 = cell_begin[(size_t)tc * CELL_STRIDE + lc];
                 const int r1 = cell_begin[(size_t)tc * CELL_STRIDE + lc + 1];
                 for (int r = r0; r < r1; ++r) {
-                    const int s = idx[r];
-                    const double xp = P.x[s], yp = P.y[s];
-                    const double ux = (xp - X) * inv, uy = (yp - Y) * inv;
+                    const double ux = (Y.x[r] - X) * inv, uy = (Y.y[r] - Y0) * inv;
                     const double wx = Nq(ux), wy = Nq(uy);
                     const double wgt = wx * wy;
                     if (!(wgt > 3e-3)) continue; // simulation.cpp:780
-                    const double dwx = dNq(ux) * inv, dwy = dNq(uy) * inv;
-                    const double hp = P.h[s], Ap = P.A[s], mp = P.m[s];
-                    const double vmx = P.vx[s] * mp, vmy = P.vy[s] * mp;
-                    const double vzm = P.vz[s] * mp, hm = hp * mp, hA = hp * Ap;
-                    const size_t fc = (size_t)floor_index(xp, inv, nx - 1) * ny + floor_index(yp, inv, ny - 1);
-                    const double cth = tcth[fc];
-                    const double coeff = -mp * g * cth * cth;
-                    const double mbx = coeff * tgx[fc], mby = coeff * tgy[fc];
-                    const double s00 = -0.5 * rho * g * cth * cth * hp;
-                    const double gwx = dwx * wy, gwy = dwy * wx;
-                    am += mp * wgt;
-                    apx += vmx * wgt;
-                    apy += vmy * wgt;
-                    apz += vzm * wgt;
-                    ah += hm * wgt;
-                    afx += mbx * wgt - hA * (s00 * gwx + 0.0 * gwy);
-                    afy += mby * wgt - hA * (s00 * gwy + 0.0 * gwx);
+                    const double gwx = dNq(ux) * inv * wy, gwy = dNq(uy) * inv * wx;
+                    const double hA = Y.hA[r], s00 = Y.s00[r];
+                    am += Y.m[r] * wgt;
+                    apx += Y.vmx[r] * wgt;
+                    apy += Y.vmy[r] * wgt;
+                    apz += Y.vzm[r] * wgt;
+                    ah += Y.hm[r] * wgt;
+                    afx += Y.mbx[r] * wgt - hA * (s00 * gwx + 0.0 * gwy);
+                    afy += Y.mby[r] * wgt - hA * (s00 * gwy + 0.0 * gwx);
                 }
             }
         }
@@ -562,14 +533,16 @@ __global__ void __launch_bounds__(WG) k_g2p(Ctl* c, PBuf64 A, PBuf64 B, const in
                                             const double4* __restrict__ vf, const double* __restrict__ tz,
                                             const double* __restrict__ tgx, const double* __restrict__ tgy,
                                             int* __restrict__ key, int* __restrict__ tslot,
-                                            int* __restrict__ tile_count) {
+                                            int* __restrict__ tile_count, unsigned long long* __restrict__ part) {
     if (c->stop) return;
     const PBuf64 P = cur_buf(c, A, B);
     const PBuf64 Q = nxt_buf(c, A, B);
     const int r = blockIdx.x * WG + threadIdx.x;
+    const bool valid = r < c->np;
     double v2 = 0.0;
     unsigned long long bx0 = ~0ull, bx1 = 0, by0 = ~0ull, by1 = 0;
-    if (r < c->np) {
+    int t = 0, kk = 0;
+    if (valid) {
         const int nx = c->nx, ny = c->ny;
         const double inv = c->inv_dx, dx = c->dx, dt = c->dt, fr = c->flip_ratio;
         const int s = idx[r];
@@ -618,14 +591,17 @@ __global__ void __launch_bounds__(WG) k_g2p(Ctl* c, PBuf64 A, PBuf64 B, const in
         Q.A[r] = mp / (hn * c->rho);
         Q.m[r] = mp;
         Q.pid[r] = P.pid[s];
-        int t;
-        key[r] = cell_key(c, xn, yn, &t);
-        tslot[r] = atomicAdd(&tile_count[t], 1);
+        kk = cell_key(c, xn, yn, &t);
         v2 = nvx * nvx + nvy * nvy;
         bx0 = bx1 = ord_bits(xn);
         by0 = by1 = ord_bits(yn);
     }
-    block_publish(c, v2, bx0, bx1, by0, by1);
+    const int slot = wave_tile_slot(t, valid, tile_count);
+    if (valid) {
+        key[r] = kk;
+        tslot[r] = slot;
+    }
+    block_partials(v2, bx0, bx1, by0, by1, part);
 }
 
 // ---------------------------------------------------------------- downloads / diagnostics
@@ -717,9 +693,13 @@ __global__ void __launch_bounds__(WG) k_diag_particles(const Ctl* c, PBuf64 A, P
         for (int q = 0; q < 2; ++q) part[2 * blockIdx.x + q] = s[q][0];
 }
 
-__global__ void k_diag_final(const double* __restrict__ pg, int ng, const double* __restrict__ pp, int npb,
-                             double* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(WG) k_diag_final(const Ctl* c, const double* __restrict__ pg, int ng,
+                                                   const double* __restrict__ pp, int npb,
+                                                   const unsigned long long* __restrict__ part, double* __restrict__ out) {
+    __shared__ unsigned long long s_out[5];
+    reduce_partials(part, (c->np + WG - 1) / WG, s_out);
+    if (threadIdx.x != 0) return;
+    out[6] = sqrt(__longlong_as_double(s_out[4]));
     double g[4] = {0, 0, 0, 0}, p[2] = {0, 0};
     for (int b = 0; b < ng; ++b)
         for (int q = 0; q < 4; ++q) g[q] += pg[4 * b + q];

@@ -11,31 +11,40 @@ struct PBuf64 {
     int32_t* pid; // creation index
 };
 
+// Per-particle P2G terms in sorted order (node-independent part of
+// src/simulation.cpp:732-753).
+struct Pay64 {
+    double *x, *y, *m, *vmx, *vmy, *vzm, *hm, *hA, *mbx, *mby, *s00;
+};
+
 __global__ void k_ctl_clear(Ctl* c);
-__global__ void k_ctl_begin(Ctl* c);
+__global__ void k_ctl_begin(Ctl* c, const unsigned long long* part);
 __global__ void k_ctl_end(Ctl* c);
 __global__ void k_prime_pre(Ctl* c);
-__global__ void k_ctl_prime(Ctl* c);
-__global__ void k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* key, int* tslot, int* tile_count);
+__global__ void k_ctl_prime(Ctl* c, const unsigned long long* part);
+__global__ void k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* key, int* tslot, int* tile_count, unsigned long long* part);
 __global__ void k_init_z(Ctl* c, PBuf64 A, const double* tz);
 __global__ void k_tile_reduce(Ctl* c, const int* tile_count, int ntiles, int* bsum);
 __global__ void k_tile_scan(Ctl* c, const int* bsum, int nblk, int* boff);
 __global__ void k_tile_offsets(Ctl* c, const int* tile_count, int ntiles, const int* boff, int* tile_begin,
                                int* ptiles, int* gtiles, int* pstamp, int* gstamp);
 __global__ void k_scatter(Ctl* c, const int* key, const int* tslot, const int* tile_begin, int* tmp);
-__global__ void k_tile_sort(Ctl* c, PBuf64 A, PBuf64 B, const int* key, const int* ptiles, int* tile_count,
-                            const int* tile_begin, const int* tmp, int* idx, int* cell_begin);
-__global__ void k_p2g_grid(Ctl* c, PBuf64 A, PBuf64 B, const int* gtiles, const int* pstamp,
-                           const int* cell_begin, const int* idx, const double* tgx, const double* tgy,
-                           const double* tcth, double* h_max, double* v_max2, double4* vf, double* out_m,
-                           double* out_h, double* out_vz);
+__global__ void k_tile_sort(Ctl* c, const int32_t* pidA, const int32_t* pidB, const int* key, const int* ptiles,
+                            int* tile_count, const int* tile_begin, const int* tmp, int* idx, int* cell_begin);
+__global__ void k_payload(Ctl* c, PBuf64 A, PBuf64 B, const int* idx, const double* tgx, const double* tgy,
+                          const double* tcth, Pay64 Y);
+__global__ void k_p2g_grid(Ctl* c, Pay64 Y, const int* gtiles, const int* pstamp, const int* cell_begin,
+                           const double* tgx, const double* tgy, const double* tcth, double* h_max, double* v_max2,
+                           double4* vf, double* out_m, double* out_h, double* out_vz);
 __global__ void k_g2p(Ctl* c, PBuf64 A, PBuf64 B, const int* idx, const double4* vf, const double* tz,
-                      const double* tgx, const double* tgy, int* key, int* tslot, int* tile_count);
+                      const double* tgx, const double* tgy, int* key, int* tslot, int* tile_count,
+                      unsigned long long* part);
 __global__ void k_window(const Ctl* c, int i0, int j0, int wnx, int wny, const int* gstamp, const double4* vf,
                          const double* gm, const double* gh, const double* gvz, double* o_h, double* o_vx,
                          double* o_vy, double* o_vz, double* o_m);
 __global__ void k_diag_grid(const Ctl* c, const int* gtiles, const double4* vf, const double* gm, double* part);
 __global__ void k_diag_particles(const Ctl* c, PBuf64 A, PBuf64 B, double* part);
-__global__ void k_diag_final(const double* pg, int ng, const double* pp, int npb, double* out);
+__global__ void k_diag_final(const Ctl* c, const double* pg, int ng, const double* pp, int npb,
+                             const unsigned long long* part, double* out);
 
 } // namespace slab

@@ -24,20 +24,25 @@ sys.path.insert(0, os.path.join(ROOT, "slab-pro_amd"))
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes (compulsory state traffic, each persistent field read once /
-# written once), FP64 form.  Per particle and per active node (m > 1e-6).
+# written once), per particle and per active node (m > 1e-6), for each kernel.
+# FP64 form: "p2g" = per-particle P2G terms (reads x y vx vy vz h A m),
+#            "grid" = node gather + grid update (terrain Gx Gy cos, h_max v_max2
+#            read+write, write vx vy flipx flipy).
 ALG_BYTES_FP64 = {
-    # read x y vx vy vz h A m; per node: terrain Gx Gy cos(theta), h_max v_max2 RMW, write vx vy flipx flipy
-    "p2g_grid": {"particle": 64, "node": 24 + 32 + 32},
+    "p2g": {"particle": 64, "node": 0},
+    "grid": {"particle": 0, "node": 24 + 32 + 32},
     # read x y vx vy h m, write x y vx vy vz h A; per node: read vx vy flipx flipy + terrain z
     "g2p": {"particle": 48 + 56, "node": 32 + 8},
-    # idx/key traffic: read key, write idx (4 B each) + cell tables
     "tile_sort": {"particle": 8, "node": 0},
     "scatter": {"particle": 12, "node": 0},
 }
 STEP_BYTES_FP64 = {"particle": 104, "node": 120}
-# FP32 form: particle = cell int32 + offsets + vx vy vz h m (32 B read, 28 B written)
+# FP32 form: particle = cell int32 + offsets + vx vy vz h m (32 B read, 28 B
+# written by G2P); "p2g" = per-tile LDS reduction of the particle state,
+# "grid" = partial merge + grid update (terrain 12 B, max maps 16 B, vf 16 B).
 ALG_BYTES_FP32 = {
-    "p2g_grid": {"particle": 32, "node": 12 + 16 + 16},
+    "p2g": {"particle": 32, "node": 0},
+    "grid": {"particle": 0, "node": 12 + 16 + 16},
     "g2p": {"particle": 28 + 28, "node": 16 + 4},
     "tile_sort": {"particle": 8, "node": 0},
     "scatter": {"particle": 12, "node": 0},

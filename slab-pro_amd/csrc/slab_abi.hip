@@ -36,8 +36,11 @@ slab_status fail(slab_status s, const std::string& msg) {
         if (s_ != SLAB_OK) return s_;      \
     } while (0)
 
+// FP64 form: "p2g" is the per-particle payload pass, "grid" the node gather +
+// grid update.  FP32 form: "p2g" is the per-tile LDS reduction, "grid" merges
+// the tile partials and updates the grid.
 const char* const kKernelNames[] = {"ctl_begin", "tile_reduce", "tile_scan", "tile_offsets", "scatter",
-                                    "tile_sort", "payload",     "p2g_grid",  "g2p",          "ctl_end"};
+                                    "tile_sort", "p2g",         "grid",      "g2p",          "ctl_end"};
 constexpr int kNumKernels = 10;
 constexpr int kChunk = 16;     // steps enqueued between host checks
 constexpr int kPersist = 2048; // workgroups of the persistent tile kernels
@@ -65,12 +68,13 @@ struct slab_run {
     // tiles
     int *tile_count = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr;
     int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr, *cell_begin = nullptr;
+    int* tile_pos = nullptr;  // tile -> position in the particle-tile list
+    float* tpart32 = nullptr; // FP32 P2G partial sums, one region per listed particle tile
     // particles
     int64_t np = 0;
     PBuf64 A{}, B{};
     Pay64 Y{};
     PBuf32 A32{}, B32{};
-    Pay32 Y32{};
     int *key = nullptr, *tslot = nullptr, *idx = nullptr, *tmp = nullptr;
     unsigned long long* part = nullptr; // per-block |v|^2 max + bounding box partials
     double *diag_part = nullptr, *diag_out = nullptr;
@@ -180,7 +184,7 @@ void enqueue_sort(slab_run* r, int step) {
     timed(r, 2, step, [&] { k_tile_scan<<<1, WG, 0, s>>>(r->d_ctl, r->bsum, r->nblk, r->boff); });
     timed(r, 3, step, [&] {
         k_tile_offsets<<<r->nblk, WG, 0, s>>>(r->d_ctl, r->tile_count, r->ntiles, r->boff, r->tile_begin, r->ptiles,
-                                               r->gtiles, r->pstamp, r->gstamp);
+                                               r->gtiles, r->pstamp, r->gstamp, r->tile_pos);
     });
     timed(r, 4, step, [&] { k_scatter<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->key, r->tslot, r->tile_begin, r->tmp); });
     timed(r, 5, step, [&] {
@@ -189,8 +193,8 @@ void enqueue_sort(slab_run* r, int step) {
     });
     timed(r, 6, step, [&] {
         if (r->f32)
-            k_payload32<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->idx, r->tgx32, r->tgy32, r->tcth32,
-                                                  r->Y32);
+            k_p2g_cells32<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->idx, r->ptiles, r->cell_begin,
+                                                    r->tgx32, r->tgy32, r->tcth32, r->tpart32);
         else
             k_payload<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->tgx, r->tgy, r->tcth, r->Y);
     });
@@ -200,9 +204,9 @@ void enqueue_p2g(slab_run* r, int step) {
     hipStream_t s = r->stream;
     timed(r, 7, step, [&] {
         if (r->f32)
-            k_p2g_grid32<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->Y32, r->gtiles, r->pstamp, r->cell_begin, r->tgx32,
-                                                   r->tgy32, r->tcth32, r->hmax32, r->vmax2_32, r->vf32, r->gm32,
-                                                   r->gh32, r->gvz32);
+            k_grid32<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->gtiles, r->pstamp, r->tile_pos, r->tpart32, r->tgx32,
+                                               r->tgy32, r->tcth32, r->hmax32, r->vmax2_32, r->vf32, r->gm32, r->gh32,
+                                               r->gvz32);
         else
             k_p2g_grid<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->Y, r->gtiles, r->pstamp, r->cell_begin, r->tgx, r->tgy,
                                                  r->tcth, r->hmax, r->vmax2, r->vf, r->gm, r->gh, r->gvz);
@@ -417,6 +421,8 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     A(&r->bsum, 3 * (size_t)r->nblk);
     A(&r->boff, 3 * (size_t)r->nblk);
     A(&r->cell_begin, (size_t)r->ntiles * CELL_STRIDE);
+    A(&r->tile_pos, r->ntiles);
+    if (r->f32) A(&r->tpart32, (size_t)r->ntiles * P2G_FIELDS * REGION_NODES);
     A(&r->diag_part, 4 * 256 + 2 * 256);
     A(&r->diag_out, 8);
     if (st != SLAB_OK) return cleanup(st);
@@ -515,9 +521,6 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
             for (float** q : {&b->ox, &b->oy, &b->vx, &b->vy, &b->vz, &b->h, &b->m, &b->z}) P(q);
             P(&b->pid);
         }
-        for (float** q : {&r->Y32.fx, &r->Y32.fy, &r->Y32.m, &r->Y32.vmx, &r->Y32.vmy, &r->Y32.vzm, &r->Y32.hm,
-                          &r->Y32.mbx, &r->Y32.mby, &r->Y32.ps})
-            P(q);
     } else {
         for (PBuf64* b : {&r->A, &r->B}) {
             for (double** q : {&b->x, &b->y, &b->vx, &b->vy, &b->vz, &b->h, &b->A, &b->m, &b->z}) P(q);

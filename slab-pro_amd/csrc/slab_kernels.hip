@@ -278,7 +278,7 @@ __global__ void k_tile_scan(Ctl* c, const int* __restrict__ bsum, int nblk, int*
 // tile_begin, particle-tile list, grid-tile list (both in tile-id order) and stamps.
 __global__ void k_tile_offsets(Ctl* c, const int* __restrict__ tile_count, int ntiles, const int* __restrict__ boff,
                                int* __restrict__ tile_begin, int* __restrict__ ptiles, int* __restrict__ gtiles,
-                               int* __restrict__ pstamp, int* __restrict__ gstamp) {
+                               int* __restrict__ pstamp, int* __restrict__ gstamp, int* __restrict__ tile_pos) {
     if (c->stop) return;
     const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
     int f[SCAN_ITEMS][3];
@@ -298,6 +298,7 @@ __global__ void k_tile_offsets(Ctl* c, const int* __restrict__ tile_count, int n
         tile_begin[t] = run[0];
         if (f[k][1]) {
             ptiles[run[1]] = t;
+            tile_pos[t] = run[1];
             pstamp[t] = ep;
         }
         if (f[k][2]) {

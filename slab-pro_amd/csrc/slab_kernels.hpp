@@ -27,7 +27,7 @@ __global__ void k_init_z(Ctl* c, PBuf64 A, const double* tz);
 __global__ void k_tile_reduce(Ctl* c, const int* tile_count, int ntiles, int* bsum);
 __global__ void k_tile_scan(Ctl* c, const int* bsum, int nblk, int* boff);
 __global__ void k_tile_offsets(Ctl* c, const int* tile_count, int ntiles, const int* boff, int* tile_begin,
-                               int* ptiles, int* gtiles, int* pstamp, int* gstamp);
+                               int* ptiles, int* gtiles, int* pstamp, int* gstamp, int* tile_pos);
 __global__ void k_scatter(Ctl* c, const int* key, const int* tslot, const int* tile_begin, int* tmp);
 __global__ void k_tile_sort(Ctl* c, const int32_t* pidA, const int32_t* pidB, const int* key, const int* ptiles,
                             int* tile_count, const int* tile_begin, const int* tmp, int* idx, int* cell_begin);

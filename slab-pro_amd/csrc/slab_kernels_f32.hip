@@ -93,94 +93,152 @@ __global__ void k_init_z32(Ctl* c, PBuf32 A, const float* __restrict__ tz) {
     A.z[s] = z;
 }
 
-// Node-independent P2G terms (src/simulation.cpp:732-753) in sorted order;
-// offsets relative to the stencil base so u = f - d for node base + d.
-__global__ void __launch_bounds__(WG) k_payload32(Ctl* c, PBuf32 A, PBuf32 B, const int* __restrict__ idx,
-                                                  const float* __restrict__ tgx, const float* __restrict__ tgy,
-                                                  const float* __restrict__ tcth, Pay32 Y) {
-    if (c->stop) return;
-    const int r = blockIdx.x * WG + threadIdx.x;
-    if (r >= c->np) return;
-    const PBuf32 P = cur32(c, A, B);
-    const int s = idx[r];
-    const int cell = P.cell[s];
-    const int ci = cell / c->ny, cj = cell - ci * c->ny;
-    const float ox = P.ox[s], oy = P.oy[s];
-    const int ib = base32(ci, ox, c->nx - 3), jb = base32(cj, oy, c->ny - 3);
-    const float g = (float)c->gravity;
-    const float hp = P.h[s], mp = P.m[s];
-    const float cth = tcth[cell];
-    const float coeff = -mp * g * cth * cth;
-    Y.fx[r] = (float)(ci - ib) + ox;
-    Y.fy[r] = (float)(cj - jb) + oy;
-    Y.m[r] = mp;
-    Y.vmx[r] = P.vx[s] * mp;
-    Y.vmy[r] = P.vy[s] * mp;
-    Y.vzm[r] = P.vz[s] * mp;
-    Y.hm[r] = hp * mp;
-    Y.mbx[r] = coeff * tgx[cell];
-    Y.mby[r] = coeff * tgy[cell];
-    // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
-    Y.ps[r] = -0.5f * mp * g * cth * cth * hp;
+// Quadratic B-spline weights and derivatives of the 3 stencil nodes of one
+// axis for a position f in [0.5, 1.5) relative to the base node (u = f - d),
+// src/tools.hpp:119-140; general branches keep the edge-clamped cases exact.
+__device__ __forceinline__ void weights3f(float f, float inv, float w[3], float dw[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        w[d] = Nf(f - (float)d);
+        dw[d] = dNf(f - (float)d) * inv;
+    }
 }
 
-// P2G gather + computeGridVelocity per node (src/simulation.cpp:726-875).
-__global__ void __launch_bounds__(WG) k_p2g_grid32(Ctl* c, Pay32 Y, const int* __restrict__ gtiles,
-                                                   const int* __restrict__ pstamp, const int* __restrict__ cell_begin,
-                                                   const float* __restrict__ tgx, const float* __restrict__ tgy,
-                                                   const float* __restrict__ tcth, float* __restrict__ h_max,
-                                                   float* __restrict__ v_max2, float4* __restrict__ vf,
-                                                   float* __restrict__ out_m, float* __restrict__ out_h,
-                                                   float* __restrict__ out_vz) {
+// P2G, src/simulation.cpp:726-798, as a per-tile segmented reduction without
+// atomics.  One workgroup per particle tile (persistent over the list), one
+// thread per stencil-base cell: the thread sums its cell's particles (sorted,
+// so in creation order) into 9 stencil slots x 7 fields in registers; then 9
+// phases add slot (d,e) of every cell into node cell+(d,e) of an 18x18-node
+// LDS region -- within a phase the cell -> node map is one-to-one, so there
+// are no conflicts and the order is fixed.  The region (tile + 2-node halo) is
+// written to this tile's partial; k_grid32 merges the <= 4 partials of a node.
+__global__ void __launch_bounds__(WG) k_p2g_cells32(Ctl* c, PBuf32 A, PBuf32 B, const int* __restrict__ idx,
+                                                    const int* __restrict__ ptiles,
+                                                    const int* __restrict__ cell_begin,
+                                                    const float* __restrict__ tgx, const float* __restrict__ tgy,
+                                                    const float* __restrict__ tcth, float* __restrict__ tpart) {
     if (c->stop) return;
-    const int nt = c->n_gtiles, ep = c->epoch, nx = c->nx, ny = c->ny, nty = c->nty;
-    const float inv = (float)c->inv_dx, g = (float)c->gravity, dt = (float)c->dt;
-    const float mu = (float)c->mu, xi = (float)c->xi, mu1 = (float)c->mu1, mu2 = (float)c->mu2;
-    const float I0 = (float)c->I0, dd = (float)c->d, phi = (float)c->phi;
-    const int fric = c->fric, out = c->out_step;
+    __shared__ float s_acc[P2G_FIELDS * REGION_NODES];
+    const PBuf32 P = cur32(c, A, B);
+    const int nt = c->n_ptiles, ny = c->ny, nty = c->nty;
+    const float g = (float)c->gravity, inv = (float)c->inv_dx;
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
-        const int t = gtiles[w];
+        const int t = ptiles[w];
         const int tx = t / nty, ty = t - tx * nty;
-        const int i = tx * TILE + (threadIdx.x >> 4);
-        const int j = ty * TILE + (threadIdx.x & 15);
-        if (i >= nx || j >= ny) continue;
-        float am = 0.f, apx = 0.f, apy = 0.f, apz = 0.f, ah = 0.f, afx = 0.f, afy = 0.f;
-        for (int ci = i - 2; ci <= i; ++ci) {
-            if (ci < 0 || ci > nx - 3) continue;
-            const float d = (float)(i - ci);
-            for (int cj = j - 2; cj <= j; ++cj) {
-                if (cj < 0 || cj > ny - 3) continue;
-                const int tc = (ci >> 4) * nty + (cj >> 4);
-                if (pstamp[tc] != ep) continue;
-                const float e = (float)(j - cj);
-                const int lc = ((ci & 15) << 4) | (cj & 15);
-                const int r0 = cell_begin[(size_t)tc * CELL_STRIDE + lc];
-                const int r1 = cell_begin[(size_t)tc * CELL_STRIDE + lc + 1];
-                for (int r = r0; r < r1; ++r) {
-                    const float ux = Y.fx[r] - d, uy = Y.fy[r] - e;
-                    const float wx = Nf(ux), wy = Nf(uy);
-                    const float wgt = wx * wy;
-                    if (!(wgt > 3e-3f)) continue;
-                    const float gwx = dNf(ux) * inv * wy, gwy = dNf(uy) * inv * wx;
-                    const float ps = Y.ps[r];
-                    am += Y.m[r] * wgt;
-                    apx += Y.vmx[r] * wgt;
-                    apy += Y.vmy[r] * wgt;
-                    apz += Y.vzm[r] * wgt;
-                    ah += Y.hm[r] * wgt;
-                    afx += Y.mbx[r] * wgt - ps * gwx;
-                    afy += Y.mby[r] * wgt - ps * gwy;
+        const int ib = tx * TILE + lx, jb = ty * TILE + ly;
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
+        float acc[9][P2G_FIELDS];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
+        const int r0 = cell_begin[(size_t)t * CELL_STRIDE + threadIdx.x];
+        const int r1 = cell_begin[(size_t)t * CELL_STRIDE + threadIdx.x + 1];
+        for (int r = r0; r < r1; ++r) {
+            const int s = idx[r];
+            const int cell = P.cell[s];
+            const int ci = cell / ny, cj = cell - ci * ny;
+            const float fx = (float)(ci - ib) + P.ox[s], fy = (float)(cj - jb) + P.oy[s];
+            const float mp = P.m[s], hp = P.h[s];
+            const float cth = tcth[cell];
+            const float coeff = -mp * g * cth * cth;
+            const float vmx = P.vx[s] * mp, vmy = P.vy[s] * mp, vzm = P.vz[s] * mp, hm = hp * mp;
+            const float mbx = coeff * tgx[cell], mby = coeff * tgy[cell];
+            // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
+            const float ps = -0.5f * mp * g * cth * cth * hp;
+            float wx[3], wy[3], dwx[3], dwy[3];
+            weights3f(fx, inv, wx, dwx);
+            weights3f(fy, inv, wy, dwy);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    const float wgt = wx[d] * wy[e];
+                    if (wgt > 3e-3f) { // simulation.cpp:780
+                        float* a = acc[d * 3 + e];
+                        a[0] += mp * wgt;
+                        a[1] += vmx * wgt;
+                        a[2] += vmy * wgt;
+                        a[3] += vzm * wgt;
+                        a[4] += hm * wgt;
+                        a[5] += mbx * wgt - ps * (dwx[d] * wy[e]);
+                        a[6] += mby * wgt - ps * (dwy[e] * wx[d]);
+                    }
                 }
             }
         }
+        __syncthreads();
+#pragma unroll
+        for (int ph = 0; ph < 9; ++ph) {
+            const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) s_acc[f * REGION_NODES + node] += acc[ph][f];
+            __syncthreads();
+        }
+        float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) dst[q] = s_acc[q];
+        __syncthreads();
+    }
+}
+
+// computeGridVelocity (src/simulation.cpp:800-875) for one grid tile per
+// workgroup, one thread per node, after merging the node's partial sums from
+// the particle tiles that reach it (own, left, below, diagonal; fixed order).
+__global__ void __launch_bounds__(WG) k_grid32(Ctl* c, const int* __restrict__ gtiles, const int* __restrict__ pstamp,
+                                               const int* __restrict__ tile_pos, const float* __restrict__ tpart,
+                                               const float* __restrict__ tgx, const float* __restrict__ tgy,
+                                               const float* __restrict__ tcth, float* __restrict__ h_max,
+                                               float* __restrict__ v_max2, float4* __restrict__ vf,
+                                               float* __restrict__ out_m, float* __restrict__ out_h,
+                                               float* __restrict__ out_vz) {
+    if (c->stop) return;
+    const int nt = c->n_gtiles, ep = c->epoch, nx = c->nx, ny = c->ny, nty = c->nty;
+    const float g = (float)c->gravity, dt = (float)c->dt;
+    const float mu = (float)c->mu, xi = (float)c->xi, mu1 = (float)c->mu1, mu2 = (float)c->mu2;
+    const float I0 = (float)c->I0, dd = (float)c->d, phi = (float)c->phi;
+    const int fric = c->fric, out = c->out_step;
+    const int a = threadIdx.x >> 4, b = threadIdx.x & 15;
+    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        const int t = gtiles[w];
+        const int tx = t / nty, ty = t - tx * nty;
+        const int i = tx * TILE + a, j = ty * TILE + b;
+        if (i >= nx || j >= ny) continue;
+        float acc[P2G_FIELDS];
+#pragma unroll
+        for (int f = 0; f < P2G_FIELDS; ++f) acc[f] = 0.f;
+        // (source tile, region node) pairs in fixed order
+        int src[4], rn[4];
+        int ns = 0;
+        src[ns] = t;
+        rn[ns++] = a * REGION + b;
+        if (tx > 0 && a < 2) {
+            src[ns] = t - nty;
+            rn[ns++] = (a + TILE) * REGION + b;
+        }
+        if (ty > 0 && b < 2) {
+            src[ns] = t - 1;
+            rn[ns++] = a * REGION + (b + TILE);
+        }
+        if (tx > 0 && ty > 0 && a < 2 && b < 2) {
+            src[ns] = t - nty - 1;
+            rn[ns++] = (a + TILE) * REGION + (b + TILE);
+        }
+        for (int k = 0; k < ns; ++k) {
+            if (pstamp[src[k]] != ep) continue;
+            const float* q = tpart + (size_t)tile_pos[src[k]] * (P2G_FIELDS * REGION_NODES) + rn[k];
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) acc[f] += q[f * REGION_NODES];
+        }
+        const float am = acc[0];
         const size_t k = (size_t)i * ny + j;
         float nvx = 0.f, nvy = 0.f, flx = 0.f, fly = 0.f, hh = 0.f, vz = 0.f;
         if (am > 1e-6f) {
             const float rm = 1.0f / am;
-            const float vx = apx * rm, vy = apy * rm;
-            vz = apz * rm;
-            hh = ah * rm;
-            const float vtx = vx + dt * (afx * rm), vty = vy + dt * (afy * rm);
+            const float vx = acc[1] * rm, vy = acc[2] * rm;
+            vz = acc[3] * rm;
+            hh = acc[4] * rm;
+            const float vtx = vx + dt * (acc[5] * rm), vty = vy + dt * (acc[6] * rm);
             const float vsq = vtx * vtx + vty * vty;
             if (vsq > 1e-16f) {
                 const float Gx = tgx[k], Gy = tgy[k], cth = tcth[k];

@@ -11,19 +11,19 @@ struct PBuf32 {
     int32_t* pid;
 };
 
-// Node-independent P2G terms in sorted order; fx, fy = position relative to
-// the stencil base node, in cell units (in [0.5, 1.5) away from the edges).
-struct Pay32 {
-    float *fx, *fy, *m, *vmx, *vmy, *vzm, *hm, *mbx, *mby, *ps;
-};
+// P2G partial sums per particle tile: 7 fields over the tile's 16x16 nodes
+// plus the 2-node halo its stencils reach (18x18 region).
+constexpr int REGION = TILE + 2;
+constexpr int REGION_NODES = REGION * REGION;
+constexpr int P2G_FIELDS = 7; // m, P_x, P_y, P_z, H, F_x, F_y
 
 __global__ void k_keys32(Ctl* c, PBuf32 A, PBuf32 B, int* key, int* tslot, int* tile_count, unsigned long long* part);
 __global__ void k_init_z32(Ctl* c, PBuf32 A, const float* tz);
-__global__ void k_payload32(Ctl* c, PBuf32 A, PBuf32 B, const int* idx, const float* tgx, const float* tgy,
-                            const float* tcth, Pay32 Y);
-__global__ void k_p2g_grid32(Ctl* c, Pay32 Y, const int* gtiles, const int* pstamp, const int* cell_begin,
-                             const float* tgx, const float* tgy, const float* tcth, float* h_max, float* v_max2,
-                             float4* vf, float* out_m, float* out_h, float* out_vz);
+__global__ void k_p2g_cells32(Ctl* c, PBuf32 A, PBuf32 B, const int* idx, const int* ptiles, const int* cell_begin,
+                              const float* tgx, const float* tgy, const float* tcth, float* tpart);
+__global__ void k_grid32(Ctl* c, const int* gtiles, const int* pstamp, const int* tile_pos, const float* tpart,
+                         const float* tgx, const float* tgy, const float* tcth, float* h_max, float* v_max2,
+                         float4* vf, float* out_m, float* out_h, float* out_vz);
 __global__ void k_g2p32(Ctl* c, PBuf32 A, PBuf32 B, const int* idx, const float4* vf, const float* tz,
                         const float* tgx, const float* tgy, int* key, int* tslot, int* tile_count,
                         unsigned long long* part);

@@ -88,7 +88,8 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
 
 /* Uploads particles as initializeParticles leaves them
  * (src/simulation.cpp:443-466).  vx, vy, vz may be NULL (zero).  m may be
- * NULL (then rho*h*A as at :452).  Particle order is the creation order. */
+ * NULL (then rho*h*A as at :452).  Particle order is the creation order.
+ * Non-finite inputs are rejected (SLAB_EINVAL). */
 slab_status slab_run_upload_particles(slab_run* run, int64_t np, const double* x, const double* y,
                                       const double* vx, const double* vy, const double* vz,
                                       const double* h, const double* A, const double* m);
@@ -150,6 +151,55 @@ void slab_run_destroy(slab_run* run);
 
 /* Library / device facts: out[0]=device id, out[1]=CU count, out[2]=is gfx950. */
 slab_status slab_device_info(int32_t out[3]);
+
+/* ---------------------------------------------------------------------------
+ * Batches: many scenarios on one shared DEM, advanced together (FP32 form).
+ * No reference counterpart -- the reference runs one scenario per process
+ * (src/main.cpp:29-58).  Each slot is one Simulation's step state: its own
+ * friction (src/tools.hpp:62-84), time / dt / frame / total_time
+ * (src/simulation.cpp:484-582) and max maps.  Slots advance by their own dt;
+ * a slot that reaches total_time is parked (its particles are kept but not
+ * computed) until released.  The batch owns one HIP stream.
+ * ------------------------------------------------------------------------- */
+typedef struct slab_batch slab_batch;
+
+enum { SLAB_SLOT_FREE = 0, SLAB_SLOT_PRIMING = 1, SLAB_SLOT_RUNNING = 2, SLAB_SLOT_DONE = 3, SLAB_SLOT_FAILED = 4 };
+
+typedef struct {
+    double time, dt;
+    int32_t status, frame, step, np;
+    int32_t frame_hit, err;
+} slab_slot_info;
+
+/* Shared terrain (complex_terrain.cpp:84-111 precompute) + capacity for
+ * max_slots scenarios and max_particles particles in total. */
+slab_status slab_batch_create(const slab_terrain* terrain, int32_t max_slots, int64_t max_particles,
+                              slab_batch** out);
+/* Admits one scenario (particles as initializeParticles leaves them; vx, vy
+ * and m may be NULL); it is primed (dt = 0 pass, simulation.cpp:493-495) by
+ * the next step.  SLAB_ENOMEM when no slot or particle capacity is free,
+ * SLAB_EINVAL for non-finite inputs or parameters; *slot = -1 then. */
+slab_status slab_batch_admit(slab_batch* b, const slab_params* params, const slab_friction* friction, int64_t np,
+                             const double* x, const double* y, const double* vx, const double* vy, const double* h,
+                             const double* A, const double* m, int32_t* slot);
+/* Runs up to max_steps batch steps (every active slot advances one of its own
+ * steps per batch step); returns early when no slot is active. */
+slab_status slab_batch_step(slab_batch* b, int32_t max_steps, int32_t* n_active);
+slab_status slab_batch_slot_info(slab_batch* b, int32_t slot, slab_slot_info* out);
+/* status[max_slots] and time[max_slots] of every slot (either may be NULL). */
+slab_status slab_batch_statuses(slab_batch* b, int32_t* status, double* time);
+slab_status slab_batch_download_max(slab_batch* b, int32_t slot, double* h_max, double* v_max2);
+slab_status slab_batch_download_particles(slab_batch* b, int32_t slot, double* x, double* y, double* vx, double* vy,
+                                          double* vz, double* z, double* h, double* A, double* m);
+/* Frees a slot; its particles leave the device state at the next step. */
+slab_status slab_batch_release(slab_batch* b, int32_t slot);
+/* out[0] = device bytes per particle, out[1] = device bytes per slot,
+ * out[2] = free particle capacity, out[3] = free slots (admission by footprint). */
+slab_status slab_batch_footprint(slab_batch* b, int64_t out[4]);
+/* Replay the step's launch sequence from a captured hipGraph (on = 1). */
+slab_status slab_batch_use_graph(slab_batch* b, int32_t on);
+slab_status slab_batch_timing(slab_batch* b, int32_t enable, double* ms, int64_t* counts, int32_t n);
+void slab_batch_destroy(slab_batch* b);
 
 #ifdef __cplusplus
 }

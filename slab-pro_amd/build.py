@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # contraction so its arithmetic is the reference's operation for operation.
 SOURCES = {
     "slab_kernels.hip": ["-ffp-contract=off"],
-    "slab_kernels_f32.hip": ["-ffp-contract=fast"],
+    "slab_engine32.hip": ["-ffp-contract=fast"],
     "slab_abi.hip": ["-ffp-contract=off"],
 }
 

@@ -1,9 +1,9 @@
 // slab_abi.hip -- host side of libslabpro.so: the C ABI declared in
-// include/slab_abi.h, driving the FP64-form or FP32-form kernels of one run on
-// one HIP stream.
+// include/slab_abi.h.  FP64 runs drive the FP64-form kernels on one HIP stream;
+// FP32 runs and batches drive the FP32 engine (slab_engine32.hip).
 #include "slab_abi.h"
+#include "slab_engine32.hpp"
 #include "slab_kernels.hpp"
-#include "slab_kernels_f32.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -60,21 +60,18 @@ struct slab_run {
     double *tz = nullptr, *tgx = nullptr, *tgy = nullptr, *tcth = nullptr, *hmax = nullptr, *vmax2 = nullptr;
     double4* vf = nullptr;
     double *gm = nullptr, *gh = nullptr, *gvz = nullptr;
-    // terrain + grid, FP32 form
-    float *tz32 = nullptr, *tgx32 = nullptr, *tgy32 = nullptr, *tcth32 = nullptr, *hmax32 = nullptr,
-          *vmax2_32 = nullptr;
-    float4* vf32 = nullptr;
-    float *gm32 = nullptr, *gh32 = nullptr, *gvz32 = nullptr;
+    // FP32 form: the engine (one slot), created at upload; terrain kept until then
+    Engine32* eng = nullptr;
+    std::vector<double> z32;
+    Slot slot_prm{};
     // tiles
     int *tile_count = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr;
     int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr, *cell_begin = nullptr;
-    int* tile_pos = nullptr;  // tile -> position in the particle-tile list
-    float* tpart32 = nullptr; // FP32 P2G partial sums, one region per listed particle tile
+    int* tile_pos = nullptr; // tile -> position in the particle-tile list
     // particles
     int64_t np = 0;
     PBuf64 A{}, B{};
     Pay64 Y{};
-    PBuf32 A32{}, B32{};
     int *key = nullptr, *tslot = nullptr, *idx = nullptr, *tmp = nullptr;
     unsigned long long* part = nullptr; // per-block |v|^2 max + bounding box partials
     double *diag_part = nullptr, *diag_out = nullptr;
@@ -92,9 +89,6 @@ struct slab_run {
     int64_t cnt[kNumKernels] = {};
     std::vector<void*> allocs;
     std::vector<void*> part_allocs;
-
-    const int32_t* pidA() const { return f32 ? A32.pid : A.pid; }
-    const int32_t* pidB() const { return f32 ? B32.pid : B.pid; }
 };
 
 namespace {
@@ -169,11 +163,7 @@ unsigned pblocks(const slab_run* r) { return (unsigned)((r->np + WG - 1) / WG); 
 unsigned tblocks(const slab_run* r) { return (unsigned)std::min(r->ntiles, kPersist); }
 
 void enqueue_keys(slab_run* r) {
-    hipStream_t s = r->stream;
-    if (r->f32)
-        k_keys32<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->key, r->tslot, r->tile_count, r->part);
-    else
-        k_keys<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->key, r->tslot, r->tile_count, r->part);
+    k_keys<<<pblocks(r), WG, 0, r->stream>>>(r->d_ctl, r->A, r->B, r->key, r->tslot, r->tile_count, r->part);
 }
 
 // Tile scan + bucketing + canonical sort + P2G terms: the particle order that
@@ -188,28 +178,19 @@ void enqueue_sort(slab_run* r, int step) {
     });
     timed(r, 4, step, [&] { k_scatter<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->key, r->tslot, r->tile_begin, r->tmp); });
     timed(r, 5, step, [&] {
-        k_tile_sort<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->pidA(), r->pidB(), r->key, r->ptiles, r->tile_count,
+        k_tile_sort<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->A.pid, r->B.pid, r->key, r->ptiles, r->tile_count,
                                               r->tile_begin, r->tmp, r->idx, r->cell_begin);
     });
     timed(r, 6, step, [&] {
-        if (r->f32)
-            k_p2g_cells32<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->idx, r->ptiles, r->cell_begin,
-                                                    r->tgx32, r->tgy32, r->tcth32, r->tpart32);
-        else
-            k_payload<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->tgx, r->tgy, r->tcth, r->Y);
+        k_payload<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->tgx, r->tgy, r->tcth, r->Y);
     });
 }
 
 void enqueue_p2g(slab_run* r, int step) {
     hipStream_t s = r->stream;
     timed(r, 7, step, [&] {
-        if (r->f32)
-            k_grid32<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->gtiles, r->pstamp, r->tile_pos, r->tpart32, r->tgx32,
-                                               r->tgy32, r->tcth32, r->hmax32, r->vmax2_32, r->vf32, r->gm32, r->gh32,
-                                               r->gvz32);
-        else
-            k_p2g_grid<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->Y, r->gtiles, r->pstamp, r->cell_begin, r->tgx, r->tgy,
-                                                 r->tcth, r->hmax, r->vmax2, r->vf, r->gm, r->gh, r->gvz);
+        k_p2g_grid<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->Y, r->gtiles, r->pstamp, r->cell_begin, r->tgx, r->tgy,
+                                             r->tcth, r->hmax, r->vmax2, r->vf, r->gm, r->gh, r->gvz);
     });
 }
 
@@ -219,12 +200,8 @@ void enqueue_step(slab_run* r, int step) {
     enqueue_sort(r, step);
     enqueue_p2g(r, step);
     timed(r, 8, step, [&] {
-        if (r->f32)
-            k_g2p32<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->idx, r->vf32, r->tz32, r->tgx32, r->tgy32,
-                                              r->key, r->tslot, r->tile_count, r->part);
-        else
-            k_g2p<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->vf, r->tz, r->tgx, r->tgy, r->key,
-                                            r->tslot, r->tile_count, r->part);
+        k_g2p<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->vf, r->tz, r->tgx, r->tgy, r->key, r->tslot,
+                                        r->tile_count, r->part);
     });
     timed(r, 9, step, [&] { k_ctl_end<<<1, 1, 0, s>>>(r->d_ctl); });
 }
@@ -262,61 +239,70 @@ void terrain_tables(const double* z, int nx, int ny, double inv, std::vector<dou
     for (auto& t : ts) t.join();
 }
 
-std::vector<float> to_f32(const double* a, size_t n) {
-    std::vector<float> o(n);
-    for (size_t k = 0; k < n; ++k) o[k] = (float)a[k];
-    return o;
-}
-
 void free_list(std::vector<void*>& l) {
     for (void* p : l) (void)hipFree(p);
     l.clear();
 }
 
-// Current-buffer particle state in creation order as FP64 (both forms).
+// Current-buffer particle state in creation order (FP64 form).
 slab_status gather_particles(slab_run* r, double* const dst[9]) {
     TRY(pull_ctl(r));
     const size_t n = (size_t)r->np;
-    const bool par = r->h_ctl->parity != 0;
+    const PBuf64& P = r->h_ctl->parity ? r->B : r->A;
     std::vector<int32_t> pid(n);
-    if (!r->f32) {
-        const PBuf64& P = par ? r->B : r->A;
-        HIPCHK(hipMemcpy(pid.data(), P.pid, n * 4, hipMemcpyDeviceToHost));
-        std::vector<double> tmp(n);
-        double* srcs[] = {P.x, P.y, P.vx, P.vy, P.vz, P.z, P.h, P.A, P.m};
-        for (int f = 0; f < 9; ++f) {
-            if (!dst[f]) continue;
-            HIPCHK(hipMemcpy(tmp.data(), srcs[f], n * 8, hipMemcpyDeviceToHost));
-            for (size_t k = 0; k < n; ++k) dst[f][pid[k]] = tmp[k];
-        }
-        return SLAB_OK;
-    }
-    const PBuf32& P = par ? r->B32 : r->A32;
     HIPCHK(hipMemcpy(pid.data(), P.pid, n * 4, hipMemcpyDeviceToHost));
-    std::vector<int32_t> cell(n);
-    std::vector<float> ox(n), oy(n), f(n), h(n), m(n);
-    HIPCHK(hipMemcpy(cell.data(), P.cell, n * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ox.data(), P.ox, n * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(oy.data(), P.oy, n * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(h.data(), P.h, n * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(m.data(), P.m, n * 4, hipMemcpyDeviceToHost));
-    const double rho = r->h_ctl->rho;
-    for (size_t k = 0; k < n; ++k) {
-        const int ci = cell[k] / r->ny, cj = cell[k] - ci * r->ny;
-        const int q = pid[k];
-        if (dst[0]) dst[0][q] = ((double)ci + (double)ox[k]) * r->dx;
-        if (dst[1]) dst[1][q] = ((double)cj + (double)oy[k]) * r->dx;
-        if (dst[6]) dst[6][q] = h[k];
-        if (dst[7]) dst[7][q] = (double)m[k] / ((double)h[k] * rho);
-        if (dst[8]) dst[8][q] = m[k];
+    std::vector<double> tmp(n);
+    double* srcs[] = {P.x, P.y, P.vx, P.vy, P.vz, P.z, P.h, P.A, P.m};
+    for (int f = 0; f < 9; ++f) {
+        if (!dst[f]) continue;
+        HIPCHK(hipMemcpy(tmp.data(), srcs[f], n * 8, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) dst[f][pid[k]] = tmp[k];
     }
-    float* srcs[] = {P.vx, P.vy, P.vz, P.z};
-    const int slots[] = {2, 3, 4, 5};
-    for (int f2 = 0; f2 < 4; ++f2) {
-        if (!dst[slots[f2]]) continue;
-        HIPCHK(hipMemcpy(f.data(), srcs[f2], n * 4, hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < n; ++k) dst[slots[f2]][pid[k]] = f[k];
-    }
+    return SLAB_OK;
+}
+
+Slot slot_params(const slab_params* p, const slab_friction* f) {
+    Slot S{};
+    S.gravity = p->gravity;
+    S.rho = p->rho;
+    S.C_cfl = p->C_cfl;
+    S.dt_max = p->dt_max;
+    S.flip_ratio = p->flip_ratio;
+    S.output_dt = p->output_dt;
+    S.total_time = p->total_time;
+    S.fric = f->type;
+    S.mu = f->mu;
+    S.xi = f->xi;
+    S.mu1 = f->mu1;
+    S.mu2 = f->mu2;
+    S.I0 = f->I0;
+    S.d = f->d;
+    S.phi = f->phi;
+    return S;
+}
+
+slab_status eng_fail(const std::string& msg, slab_status st = SLAB_EDEVICE) { return fail(st, msg); }
+
+// Non-finite inputs are rejected before they reach the device.
+bool all_finite(const double* a, int64_t n) {
+    if (!a) return true;
+    for (int64_t k = 0; k < n; ++k)
+        if (!std::isfinite(a[k])) return false;
+    return true;
+}
+
+slab_status check_particles(int64_t np, const double* x, const double* y, const double* vx, const double* vy,
+                            const double* vz, const double* h, const double* A, const double* m) {
+    if (!all_finite(x, np) || !all_finite(y, np) || !all_finite(vx, np) || !all_finite(vy, np) ||
+        !all_finite(vz, np) || !all_finite(h, np) || !all_finite(A, np) || !all_finite(m, np))
+        return fail(SLAB_EINVAL, "non-finite particle input");
+    return SLAB_OK;
+}
+
+slab_status check_params(const slab_params* p, const slab_friction* f) {
+    const double v[] = {p->gravity, p->rho, p->C_cfl, p->dt_max, p->flip_ratio, p->output_dt, p->total_time,
+                        f->mu, f->xi, f->mu1, f->mu2, f->I0, f->d, f->phi};
+    if (!all_finite(v, 14)) return fail(SLAB_EINVAL, "non-finite parameter");
     return SLAB_OK;
 }
 
@@ -346,19 +332,26 @@ slab_status slab_device_info(int32_t out[3]) {
     return SLAB_OK;
 }
 
-slab_status slab_run_create(const slab_terrain* terrain, const slab_params* params, const slab_friction* friction,
-                            slab_run** out) {
-    if (!terrain || !params || !friction || !out) return fail(SLAB_EINVAL, "null argument");
-    *out = nullptr;
-    const int nx = terrain->nx, ny = terrain->ny;
-    if (nx < 4 || ny < 4 || !terrain->z) return fail(SLAB_EINVAL, "terrain must have at least 4x4 nodes and z");
+static slab_status check_inputs(const slab_terrain* terrain, const slab_params* params, const slab_friction* friction) {
+    if (!terrain || !params || !friction) return fail(SLAB_EINVAL, "null argument");
+    if (terrain->nx < 4 || terrain->ny < 4 || !terrain->z)
+        return fail(SLAB_EINVAL, "terrain must have at least 4x4 nodes and z");
     if (!(terrain->dx > 0.0) || !std::isfinite(terrain->dx)) return fail(SLAB_EINVAL, "dx must be positive");
-    if ((double)nx * (double)ny > 1.0e9) return fail(SLAB_EDOMAIN, "terrain larger than 1e9 nodes");
+    if ((double)terrain->nx * (double)terrain->ny > 1.0e9) return fail(SLAB_EDOMAIN, "terrain larger than 1e9 nodes");
     if (params->precision != SLAB_FP64 && params->precision != SLAB_FP32)
         return fail(SLAB_EINVAL, "precision must be SLAB_FP64 or SLAB_FP32");
     if (friction->type < 0 || friction->type > 2) return fail(SLAB_EINVAL, "unknown friction type");
     if (!(params->output_dt > 0.0) || !(params->dt_max > 0.0))
         return fail(SLAB_EINVAL, "output_dt and dt_max must be positive");
+    return check_params(params, friction);
+}
+
+slab_status slab_run_create(const slab_terrain* terrain, const slab_params* params, const slab_friction* friction,
+                            slab_run** out) {
+    if (!out) return fail(SLAB_EINVAL, "null argument");
+    *out = nullptr;
+    TRY(check_inputs(terrain, params, friction));
+    const int nx = terrain->nx, ny = terrain->ny;
     int32_t info[3];
     TRY(slab_device_info(info));
     if (!info[2]) return fail(SLAB_EDEVICE, "device is not gfx950 (MI355X)");
@@ -371,6 +364,16 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     r->nn = (size_t)nx * ny;
     r->dx = terrain->dx;
     r->inv_dx = 1. / terrain->dx;
+    if (r->f32) { // the FP32 engine is sized at upload, when the particle count is known
+        if (nx > 65535 || ny > 65535) {
+            delete r;
+            return fail(SLAB_EDOMAIN, "the FP32 form supports up to 65535 nodes per side");
+        }
+        r->z32.assign(terrain->z, terrain->z + r->nn);
+        r->slot_prm = slot_params(params, friction);
+        *out = r;
+        return SLAB_OK;
+    }
     r->ntx = (nx + TILE - 1) / TILE;
     r->nty = (ny + TILE - 1) / TILE;
     r->ntiles = r->ntx * r->nty;
@@ -389,29 +392,16 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
         if (st == SLAB_OK) st = dalloc(p, n, r->allocs);
     };
     A(&r->d_ctl, 1);
-    if (r->f32) {
-        A(&r->tz32, r->nn);
-        A(&r->tgx32, r->nn);
-        A(&r->tgy32, r->nn);
-        A(&r->tcth32, r->nn);
-        A(&r->hmax32, r->nn);
-        A(&r->vmax2_32, r->nn);
-        A(&r->vf32, r->nn);
-        A(&r->gm32, r->nn);
-        A(&r->gh32, r->nn);
-        A(&r->gvz32, r->nn);
-    } else {
-        A(&r->tz, r->nn);
-        A(&r->tgx, r->nn);
-        A(&r->tgy, r->nn);
-        A(&r->tcth, r->nn);
-        A(&r->hmax, r->nn);
-        A(&r->vmax2, r->nn);
-        A(&r->vf, r->nn);
-        A(&r->gm, r->nn);
-        A(&r->gh, r->nn);
-        A(&r->gvz, r->nn);
-    }
+    A(&r->tz, r->nn);
+    A(&r->tgx, r->nn);
+    A(&r->tgy, r->nn);
+    A(&r->tcth, r->nn);
+    A(&r->hmax, r->nn);
+    A(&r->vmax2, r->nn);
+    A(&r->vf, r->nn);
+    A(&r->gm, r->nn);
+    A(&r->gh, r->nn);
+    A(&r->gvz, r->nn);
     A(&r->tile_count, r->ntiles);
     A(&r->tile_begin, r->ntiles);
     A(&r->pstamp, r->ntiles);
@@ -422,7 +412,6 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     A(&r->boff, 3 * (size_t)r->nblk);
     A(&r->cell_begin, (size_t)r->ntiles * CELL_STRIDE);
     A(&r->tile_pos, r->ntiles);
-    if (r->f32) A(&r->tpart32, (size_t)r->ntiles * P2G_FIELDS * REGION_NODES);
     A(&r->diag_part, 4 * 256 + 2 * 256);
     A(&r->diag_out, 8);
     if (st != SLAB_OK) return cleanup(st);
@@ -435,42 +424,20 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
         ok = ok && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
     };
     auto zero = [&](void* dst, size_t bytes, int v) { ok = ok && hipMemsetAsync(dst, v, bytes, s) == hipSuccess; };
-    auto sync = [&]() { ok = ok && hipStreamSynchronize(s) == hipSuccess; };
-    if (r->f32) {
-        std::vector<float> f = to_f32(terrain->z, r->nn);
-        up(r->tz32, f.data(), r->nn * 4);
-        sync();
-        f = to_f32(gx.data(), r->nn);
-        up(r->tgx32, f.data(), r->nn * 4);
-        sync();
-        f = to_f32(gy.data(), r->nn);
-        up(r->tgy32, f.data(), r->nn * 4);
-        sync();
-        f = to_f32(cth.data(), r->nn);
-        up(r->tcth32, f.data(), r->nn * 4);
-        sync();
-        zero(r->hmax32, r->nn * 4, 0);
-        zero(r->vmax2_32, r->nn * 4, 0);
-        zero(r->vf32, r->nn * sizeof(float4), 0);
-        zero(r->gm32, r->nn * 4, 0);
-        zero(r->gh32, r->nn * 4, 0);
-        zero(r->gvz32, r->nn * 4, 0);
-    } else {
-        up(r->tz, terrain->z, r->nn * 8);
-        up(r->tgx, gx.data(), r->nn * 8);
-        up(r->tgy, gy.data(), r->nn * 8);
-        up(r->tcth, cth.data(), r->nn * 8);
-        zero(r->hmax, r->nn * 8, 0);
-        zero(r->vmax2, r->nn * 8, 0);
-        zero(r->vf, r->nn * sizeof(double4), 0);
-        zero(r->gm, r->nn * 8, 0);
-        zero(r->gh, r->nn * 8, 0);
-        zero(r->gvz, r->nn * 8, 0);
-    }
+    up(r->tz, terrain->z, r->nn * 8);
+    up(r->tgx, gx.data(), r->nn * 8);
+    up(r->tgy, gy.data(), r->nn * 8);
+    up(r->tcth, cth.data(), r->nn * 8);
+    zero(r->hmax, r->nn * 8, 0);
+    zero(r->vmax2, r->nn * 8, 0);
+    zero(r->vf, r->nn * sizeof(double4), 0);
+    zero(r->gm, r->nn * 8, 0);
+    zero(r->gh, r->nn * 8, 0);
+    zero(r->gvz, r->nn * 8, 0);
     zero(r->tile_count, (size_t)r->ntiles * 4, 0);
     zero(r->pstamp, (size_t)r->ntiles * 4, 0xff);
     zero(r->gstamp, (size_t)r->ntiles * 4, 0xff);
-    sync();
+    ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) return cleanup(fail(SLAB_EDEVICE, "terrain upload failed"));
 
     Ctl& c = *r->h_ctl;
@@ -507,29 +474,30 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
                                       const double* m) {
     if (!r || !x || !y || !h || !A) return fail(SLAB_EINVAL, "null argument");
     if (np < 1 || np > (int64_t)1 << 30) return fail(SLAB_EINVAL, "particle count must be in [1, 2^30]");
-    free_list(r->part_allocs);
+    TRY(check_particles(np, x, y, vx, vy, vz, h, A, m));
     r->np = np;
     r->primed = false;
+    if (r->f32) {
+        if (r->eng) engine_destroy(r->eng);
+        std::string why;
+        r->eng = engine_create(r->nx, r->ny, r->dx, r->z32.data(), 1, np, true, &why);
+        if (!r->eng) return fail(SLAB_ENOMEM, why);
+        if (engine_admit(r->eng, r->slot_prm, np, x, y, vx, vy, vz, h, A, m, &why) != 0) return eng_fail(why);
+        return SLAB_OK;
+    }
+    free_list(r->part_allocs);
     const size_t n = (size_t)np;
     slab_status st = SLAB_OK;
     auto P = [&](auto** q) {
         if (st == SLAB_OK) st = dalloc(q, n, r->part_allocs);
     };
-    if (r->f32) {
-        for (PBuf32* b : {&r->A32, &r->B32}) {
-            P(&b->cell);
-            for (float** q : {&b->ox, &b->oy, &b->vx, &b->vy, &b->vz, &b->h, &b->m, &b->z}) P(q);
-            P(&b->pid);
-        }
-    } else {
-        for (PBuf64* b : {&r->A, &r->B}) {
-            for (double** q : {&b->x, &b->y, &b->vx, &b->vy, &b->vz, &b->h, &b->A, &b->m, &b->z}) P(q);
-            P(&b->pid);
-        }
-        for (double** q : {&r->Y.x, &r->Y.y, &r->Y.m, &r->Y.vmx, &r->Y.vmy, &r->Y.vzm, &r->Y.hm, &r->Y.hA, &r->Y.mbx,
-                           &r->Y.mby, &r->Y.s00})
-            P(q);
+    for (PBuf64* b : {&r->A, &r->B}) {
+        for (double** q : {&b->x, &b->y, &b->vx, &b->vy, &b->vz, &b->h, &b->A, &b->m, &b->z}) P(q);
+        P(&b->pid);
     }
+    for (double** q : {&r->Y.x, &r->Y.y, &r->Y.m, &r->Y.vmx, &r->Y.vmy, &r->Y.vzm, &r->Y.hm, &r->Y.hA, &r->Y.mbx,
+                       &r->Y.mby, &r->Y.s00})
+        P(q);
     P(&r->key);
     P(&r->tslot);
     P(&r->idx);
@@ -545,57 +513,17 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
         pid[p] = (int32_t)p;
     }
     hipStream_t s = r->stream;
-    if (r->f32) {
-        std::vector<int32_t> cell(n);
-        std::vector<float> ox(n), oy(n);
-        for (size_t p = 0; p < n; ++p) {
-            const double fx = x[p] * r->inv_dx, fy = y[p] * r->inv_dx;
-            double ci = std::floor(fx), cj = std::floor(fy);
-            if (!std::isfinite(ci)) ci = 0;
-            if (!std::isfinite(cj)) cj = 0;
-            ci = std::min(std::max(ci, 0.0), (double)(r->nx - 1));
-            cj = std::min(std::max(cj, 0.0), (double)(r->ny - 1));
-            float a = (float)(fx - ci), b = (float)(fy - cj);
-            int ii = (int)ci, jj = (int)cj;
-            if (a >= 1.f) {
-                a = 0.f;
-                ii = std::min(ii + 1, r->nx - 1);
-            }
-            if (b >= 1.f) {
-                b = 0.f;
-                jj = std::min(jj + 1, r->ny - 1);
-            }
-            if (!(a >= 0.f)) a = 0.f;
-            if (!(b >= 0.f)) b = 0.f;
-            cell[p] = ii * r->ny + jj;
-            ox[p] = a;
-            oy[p] = b;
-        }
-        std::vector<float> fvx = to_f32(vx ? vx : zero.data(), n), fvy = to_f32(vy ? vy : zero.data(), n);
-        std::vector<float> fvz = to_f32(vz ? vz : zero.data(), n), fh = to_f32(h, n), fm = to_f32(mm.data(), n);
-        HIPCHK(hipMemcpyAsync(r->A32.cell, cell.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A32.ox, ox.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A32.oy, oy.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A32.vx, fvx.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A32.vy, fvy.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A32.vz, fvz.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A32.h, fh.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A32.m, fm.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A32.pid, pid.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
-    } else {
-        const size_t nb = n * 8;
-        HIPCHK(hipMemcpyAsync(r->A.x, x, nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A.y, y, nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A.vx, vx ? vx : zero.data(), nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A.vy, vy ? vy : zero.data(), nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A.vz, vz ? vz : zero.data(), nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A.h, h, nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A.A, A, nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A.m, mm.data(), nb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(r->A.pid, pid.data(), n * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
-    }
+    const size_t nb = n * 8;
+    HIPCHK(hipMemcpyAsync(r->A.x, x, nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.y, y, nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.vx, vx ? vx : zero.data(), nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.vy, vy ? vy : zero.data(), nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.vz, vz ? vz : zero.data(), nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.h, h, nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.A, A, nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.m, mm.data(), nb, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(r->A.pid, pid.data(), n * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
     Ctl& c = *r->h_ctl;
     c.np = (int32_t)np;
     c.time = 0.0;
@@ -606,10 +534,7 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
     c.stop = RUNNING;
     c.err = 0;
     TRY(push_ctl(r));
-    if (r->f32)
-        k_init_z32<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A32, r->tz32);
-    else
-        k_init_z<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->tz);
+    k_init_z<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->tz);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     return SLAB_OK;
@@ -619,6 +544,13 @@ slab_status slab_run_prime(slab_run* r) {
     if (!r) return fail(SLAB_EINVAL, "null run");
     if (r->np < 1) return fail(SLAB_ESTATE, "no particles uploaded");
     if (r->primed) return fail(SLAB_ESTATE, "run already primed");
+    if (r->f32) {
+        std::string why;
+        if (!engine_step(r->eng, 1, &why)) return eng_fail(why);
+        if (engine_slot(r->eng, 0).status != SLOT_RUNNING) return eng_fail("priming step did not run");
+        r->primed = true;
+        return SLAB_OK;
+    }
     hipStream_t s = r->stream;
     HIPCHK(hipMemsetAsync(r->tile_count, 0, (size_t)r->ntiles * 4, s));
     k_prime_pre<<<1, 1, 0, s>>>(r->d_ctl);
@@ -641,6 +573,23 @@ slab_status slab_run_step(slab_run* r, int32_t max_steps, slab_step_info* info) 
     if (!r) return fail(SLAB_EINVAL, "null run");
     if (!r->primed) return fail(SLAB_ESTATE, "call slab_run_prime first");
     if (max_steps < 0) return fail(SLAB_EINVAL, "max_steps < 0");
+    if (r->f32) {
+        std::string why;
+        if (!engine_step(r->eng, max_steps, &why)) return eng_fail(why);
+        const Slot& S = engine_slot(r->eng, 0);
+        if (info) {
+            info->time = S.time;
+            info->dt = S.dt;
+            info->frame = S.frame;
+            info->step = S.step;
+            info->frame_hit = S.frame_hit;
+            info->finished = !(S.time < S.total_time);
+            info->steps_done = S.steps_in_call;
+            info->reserved = 0;
+        }
+        if (S.status == SLOT_FAILED) return fail(SLAB_ENAN, "non-finite particle velocity on the device");
+        return SLAB_OK;
+    }
     hipStream_t s = r->stream;
     k_ctl_clear<<<1, 1, 0, s>>>(r->d_ctl);
     int done = 0;
@@ -675,11 +624,20 @@ slab_status slab_run_download_particles(slab_run* r, double* x, double* y, doubl
     if (!r) return fail(SLAB_EINVAL, "null run");
     if (r->np < 1) return fail(SLAB_ESTATE, "no particles");
     double* const dst[9] = {x, y, vx, vy, vz, z, h, A, m};
+    if (r->f32) {
+        std::string why;
+        return engine_particles(r->eng, 0, dst, &why) ? SLAB_OK : eng_fail(why);
+    }
     return gather_particles(r, dst);
 }
 
 slab_status slab_run_keep_grid(slab_run* r, int32_t enable) {
     if (!r) return fail(SLAB_EINVAL, "null run");
+    if (r->f32) {
+        if (!r->eng) return fail(SLAB_ESTATE, "no particles uploaded");
+        std::string why;
+        return engine_keep_grid(r->eng, 0, enable, &why) ? SLAB_OK : eng_fail(why);
+    }
     TRY(pull_ctl(r));
     r->h_ctl->keep_grid = enable ? 1 : 0;
     return push_ctl(r);
@@ -689,6 +647,18 @@ slab_status slab_run_download_grid(slab_run* r, int32_t* i0, int32_t* j0, int32_
                                    double* vx, double* vy, double* vz, double* m) {
     if (!r) return fail(SLAB_EINVAL, "null run");
     if (!r->primed) return fail(SLAB_ESTATE, "run not primed");
+    if (r->f32) {
+        std::string why;
+        int a = 0, b = 0, c = 0, d = 0;
+        double* const dst[5] = {h, vx, vy, vz, m};
+        if (!engine_window(r->eng, 0, &a, &b, &c, &d, dst, &why))
+            return fail(why.find("kept no grid") != std::string::npos ? SLAB_ESTATE : SLAB_EDEVICE, why);
+        if (i0) *i0 = a;
+        if (j0) *j0 = b;
+        if (wnx) *wnx = c;
+        if (wny) *wny = d;
+        return SLAB_OK;
+    }
     TRY(pull_ctl(r));
     const Ctl& c = *r->h_ctl;
     if (c.last_out_epoch != c.epoch)
@@ -719,12 +689,8 @@ slab_status slab_run_download_grid(slab_run* r, int32_t* i0, int32_t* j0, int32_
     slab_status st = dalloc(&o, 5 * wn, tmpl);
     if (st != SLAB_OK) return st;
     const unsigned nb = (unsigned)((wn + WG - 1) / WG);
-    if (r->f32)
-        k_window32<<<nb, WG, 0, r->stream>>>(r->d_ctl, iL, jB, gnx, gny, r->gstamp, r->vf32, r->gm32, r->gh32,
-                                             r->gvz32, o, o + wn, o + 2 * wn, o + 3 * wn, o + 4 * wn);
-    else
-        k_window<<<nb, WG, 0, r->stream>>>(r->d_ctl, iL, jB, gnx, gny, r->gstamp, r->vf, r->gm, r->gh, r->gvz, o,
-                                           o + wn, o + 2 * wn, o + 3 * wn, o + 4 * wn);
+    k_window<<<nb, WG, 0, r->stream>>>(r->d_ctl, iL, jB, gnx, gny, r->gstamp, r->vf, r->gm, r->gh, r->gvz, o, o + wn,
+                                       o + 2 * wn, o + 3 * wn, o + 4 * wn);
     hipError_t e = hipStreamSynchronize(r->stream);
     double* dst[] = {h, vx, vy, vz, m};
     for (int f = 0; f < 5 && e == hipSuccess; ++f)
@@ -736,35 +702,28 @@ slab_status slab_run_download_grid(slab_run* r, int32_t* i0, int32_t* j0, int32_
 
 slab_status slab_run_download_max(slab_run* r, double* h_max, double* v_max2) {
     if (!r) return fail(SLAB_EINVAL, "null run");
+    if (r->f32) {
+        if (!r->eng) return fail(SLAB_ESTATE, "no particles uploaded");
+        std::string why;
+        return engine_max_maps(r->eng, 0, h_max, v_max2, &why) ? SLAB_OK : eng_fail(why);
+    }
     HIPCHK(hipStreamSynchronize(r->stream));
-    if (!r->f32) {
-        if (h_max) HIPCHK(hipMemcpy(h_max, r->hmax, r->nn * 8, hipMemcpyDeviceToHost));
-        if (v_max2) HIPCHK(hipMemcpy(v_max2, r->vmax2, r->nn * 8, hipMemcpyDeviceToHost));
-        return SLAB_OK;
-    }
-    std::vector<float> f(r->nn);
-    double* dst[] = {h_max, v_max2};
-    float* src[] = {r->hmax32, r->vmax2_32};
-    for (int k = 0; k < 2; ++k) {
-        if (!dst[k]) continue;
-        HIPCHK(hipMemcpy(f.data(), src[k], r->nn * 4, hipMemcpyDeviceToHost));
-        for (size_t q = 0; q < r->nn; ++q) dst[k][q] = f[q];
-    }
+    if (h_max) HIPCHK(hipMemcpy(h_max, r->hmax, r->nn * 8, hipMemcpyDeviceToHost));
+    if (v_max2) HIPCHK(hipMemcpy(v_max2, r->vmax2, r->nn * 8, hipMemcpyDeviceToHost));
     return SLAB_OK;
 }
 
 slab_status slab_run_diag(slab_run* r, double out[8]) {
     if (!r || !out) return fail(SLAB_EINVAL, "null argument");
     if (!r->primed) return fail(SLAB_ESTATE, "run not primed");
+    if (r->f32) {
+        std::string why;
+        return engine_diag(r->eng, 0, out, &why) ? SLAB_OK : eng_fail(why);
+    }
     TRY(pull_ctl(r));
     hipStream_t s = r->stream;
-    if (r->f32) {
-        k_diag_grid32<<<256, WG, 0, s>>>(r->d_ctl, r->gtiles, r->vf32, r->gm32, r->diag_part);
-        k_diag_particles32<<<256, WG, 0, s>>>(r->d_ctl, r->A32, r->B32, r->diag_part + 4 * 256);
-    } else {
-        k_diag_grid<<<256, WG, 0, s>>>(r->d_ctl, r->gtiles, r->vf, r->gm, r->diag_part);
-        k_diag_particles<<<256, WG, 0, s>>>(r->d_ctl, r->A, r->B, r->diag_part + 4 * 256);
-    }
+    k_diag_grid<<<256, WG, 0, s>>>(r->d_ctl, r->gtiles, r->vf, r->gm, r->diag_part);
+    k_diag_particles<<<256, WG, 0, s>>>(r->d_ctl, r->A, r->B, r->diag_part + 4 * 256);
     k_diag_final<<<1, WG, 0, s>>>(r->d_ctl, r->diag_part, 256, r->diag_part + 4 * 256, 256, r->part, r->diag_out);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, r->diag_out, 8 * 8, hipMemcpyDeviceToHost, s));
@@ -777,6 +736,11 @@ slab_status slab_run_diag(slab_run* r, double out[8]) {
 
 slab_status slab_run_timing(slab_run* r, int32_t enable, double* ms, int64_t* counts, int32_t n) {
     if (!r) return fail(SLAB_EINVAL, "null run");
+    if (r->f32) {
+        if (!r->eng) return fail(SLAB_ESTATE, "no particles uploaded");
+        engine_timing(r->eng, enable, ms, counts, n);
+        return SLAB_OK;
+    }
     if (ms || counts) {
         for (int k = 0; k < n && k < kNumKernels; ++k) {
             if (ms) ms[k] = r->ms[k];
@@ -795,35 +759,33 @@ slab_status slab_run_set_velocity(slab_run* r, const double* vx, const double* v
     if (!r || !vx || !vy) return fail(SLAB_EINVAL, "null argument");
     if (r->primed) return fail(SLAB_ESTATE, "set the velocity before slab_run_prime");
     if (r->np < 1) return fail(SLAB_ESTATE, "no particles");
+    if (r->f32) {
+        std::string why;
+        return engine_set_velocity(r->eng, 0, vx, vy, &why) ? SLAB_OK : fail(SLAB_ESTATE, why);
+    }
     TRY(pull_ctl(r));
     const bool par = r->h_ctl->parity != 0;
     const size_t n = (size_t)r->np;
     std::vector<int32_t> pid(n);
-    HIPCHK(hipMemcpy(pid.data(), par ? r->pidB() : r->pidA(), n * 4, hipMemcpyDeviceToHost));
-    if (r->f32) {
-        std::vector<float> a(n), b(n);
-        for (size_t k = 0; k < n; ++k) {
-            a[k] = (float)vx[pid[k]];
-            b[k] = (float)vy[pid[k]];
-        }
-        const PBuf32& P = par ? r->B32 : r->A32;
-        HIPCHK(hipMemcpy(P.vx, a.data(), n * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(P.vy, b.data(), n * 4, hipMemcpyHostToDevice));
-    } else {
-        std::vector<double> a(n), b(n);
-        for (size_t k = 0; k < n; ++k) {
-            a[k] = vx[pid[k]];
-            b[k] = vy[pid[k]];
-        }
-        const PBuf64& P = par ? r->B : r->A;
-        HIPCHK(hipMemcpy(P.vx, a.data(), n * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(P.vy, b.data(), n * 8, hipMemcpyHostToDevice));
+    const PBuf64& P = par ? r->B : r->A;
+    HIPCHK(hipMemcpy(pid.data(), P.pid, n * 4, hipMemcpyDeviceToHost));
+    std::vector<double> a(n), b(n);
+    for (size_t k = 0; k < n; ++k) {
+        a[k] = vx[pid[k]];
+        b[k] = vy[pid[k]];
     }
+    HIPCHK(hipMemcpy(P.vx, a.data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P.vy, b.data(), n * 8, hipMemcpyHostToDevice));
     return SLAB_OK;
 }
 
 slab_status slab_run_set_time(slab_run* r, double time, int32_t frame) {
     if (!r) return fail(SLAB_EINVAL, "null run");
+    if (r->f32) {
+        if (!r->eng) return fail(SLAB_ESTATE, "no particles uploaded");
+        std::string why;
+        return engine_set_time(r->eng, 0, time, frame, &why) ? SLAB_OK : eng_fail(why);
+    }
     TRY(pull_ctl(r));
     r->h_ctl->time = time;
     r->h_ctl->frame = frame;
@@ -832,6 +794,7 @@ slab_status slab_run_set_time(slab_run* r, double time, int32_t frame) {
 
 void slab_run_destroy(slab_run* r) {
     if (!r) return;
+    if (r->eng) engine_destroy(r->eng);
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (auto& p : r->pending) {
         (void)hipEventDestroy(p.a);
@@ -843,6 +806,151 @@ void slab_run_destroy(slab_run* r) {
     if (r->h_ctl) (void)hipHostFree(r->h_ctl);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
+}
+
+// ---------------------------------------------------------------- batches
+
+struct slab_batch {
+    Engine32* eng = nullptr;
+    int nslots = 0;
+};
+
+slab_status slab_batch_create(const slab_terrain* terrain, int32_t max_slots, int64_t max_particles,
+                              slab_batch** out) {
+    if (!terrain || !out || !terrain->z) return fail(SLAB_EINVAL, "null argument");
+    *out = nullptr;
+    if (max_slots < 1 || max_slots > 65536) return fail(SLAB_EINVAL, "max_slots must be in [1, 65536]");
+    if (max_particles < 1 || max_particles > ((int64_t)1 << 30))
+        return fail(SLAB_EINVAL, "max_particles must be in [1, 2^30]");
+    if (!(terrain->dx > 0.0)) return fail(SLAB_EINVAL, "dx must be positive");
+    int32_t info[3];
+    TRY(slab_device_info(info));
+    if (!info[2]) return fail(SLAB_EDEVICE, "device is not gfx950 (MI355X)");
+    std::string why;
+    Engine32* e = engine_create(terrain->nx, terrain->ny, terrain->dx, terrain->z, max_slots, max_particles, false,
+                                &why);
+    if (!e) return fail(SLAB_ENOMEM, why);
+    slab_batch* b = new slab_batch();
+    b->eng = e;
+    b->nslots = max_slots;
+    *out = b;
+    return SLAB_OK;
+}
+
+slab_status slab_batch_admit(slab_batch* b, const slab_params* params, const slab_friction* friction, int64_t np,
+                             const double* x, const double* y, const double* vx, const double* vy, const double* h,
+                             const double* A, const double* m, int32_t* slot) {
+    if (!b || !params || !friction || !x || !y || !h || !A || !slot) return fail(SLAB_EINVAL, "null argument");
+    *slot = -1;
+    if (friction->type < 0 || friction->type > 2) return fail(SLAB_EINVAL, "unknown friction type");
+    if (!(params->output_dt > 0.0) || !(params->dt_max > 0.0))
+        return fail(SLAB_EINVAL, "output_dt and dt_max must be positive");
+    if (np < 1) return fail(SLAB_EINVAL, "np must be >= 1");
+    TRY(check_params(params, friction));
+    TRY(check_particles(np, x, y, vx, vy, nullptr, h, A, m));
+    std::string why;
+    const int s = engine_admit(b->eng, slot_params(params, friction), np, x, y, vx, vy, nullptr, h, A, m, &why);
+    if (s < 0) {
+        *slot = -1;
+        return fail((why == "no free slot" || why == "particle capacity exhausted") ? SLAB_ENOMEM : SLAB_EDEVICE, why);
+    }
+    *slot = s;
+    return SLAB_OK;
+}
+
+slab_status slab_batch_step(slab_batch* b, int32_t max_steps, int32_t* n_active) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    if (max_steps < 0) return fail(SLAB_EINVAL, "max_steps < 0");
+    std::string why;
+    if (!engine_step(b->eng, max_steps, &why))
+        return fail(why.find("capacity") != std::string::npos ? SLAB_EDOMAIN : SLAB_EDEVICE, why);
+    if (n_active) {
+        int a = 0;
+        for (int s = 0; s < b->nslots; ++s) {
+            const int st = engine_slot(b->eng, s).status;
+            a += (st == SLOT_PRIMING || st == SLOT_RUNNING);
+        }
+        *n_active = a;
+    }
+    return SLAB_OK;
+}
+
+slab_status slab_batch_slot_info(slab_batch* b, int32_t slot, slab_slot_info* out) {
+    if (!b || !out) return fail(SLAB_EINVAL, "null argument");
+    if (slot < 0 || slot >= b->nslots) return fail(SLAB_EINVAL, "bad slot");
+    const Slot& S = engine_slot(b->eng, slot);
+    out->time = S.time;
+    out->dt = S.dt;
+    out->status = S.status;
+    out->frame = S.frame;
+    out->step = S.step;
+    out->np = S.np;
+    out->frame_hit = S.frame_hit;
+    out->err = S.err;
+    return SLAB_OK;
+}
+
+slab_status slab_batch_statuses(slab_batch* b, int32_t* status, double* time) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    for (int s = 0; s < b->nslots; ++s) {
+        const Slot& S = engine_slot(b->eng, s);
+        if (status) status[s] = S.status;
+        if (time) time[s] = S.time;
+    }
+    return SLAB_OK;
+}
+
+slab_status slab_batch_download_max(slab_batch* b, int32_t slot, double* h_max, double* v_max2) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    if (slot < 0 || slot >= b->nslots) return fail(SLAB_EINVAL, "bad slot");
+    std::string why;
+    return engine_max_maps(b->eng, slot, h_max, v_max2, &why) ? SLAB_OK : eng_fail(why);
+}
+
+slab_status slab_batch_download_particles(slab_batch* b, int32_t slot, double* x, double* y, double* vx, double* vy,
+                                          double* vz, double* z, double* h, double* A, double* m) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    if (slot < 0 || slot >= b->nslots) return fail(SLAB_EINVAL, "bad slot");
+    double* const dst[9] = {x, y, vx, vy, vz, z, h, A, m};
+    std::string why;
+    return engine_particles(b->eng, slot, dst, &why) ? SLAB_OK : eng_fail(why);
+}
+
+slab_status slab_batch_release(slab_batch* b, int32_t slot) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    std::string why;
+    return engine_release(b->eng, slot, &why) ? SLAB_OK : fail(SLAB_EINVAL, why);
+}
+
+slab_status slab_batch_footprint(slab_batch* b, int64_t out[4]) {
+    if (!b || !out) return fail(SLAB_EINVAL, "null argument");
+    std::string why;
+    if (!engine_pull(b->eng, &why)) return eng_fail(why);
+    int free_slots = 0;
+    for (int s = 0; s < b->nslots; ++s) free_slots += engine_slot(b->eng, s).status == SLOT_FREE;
+    out[0] = engine_particle_bytes();
+    out[1] = engine_slot_bytes(b->eng);
+    out[2] = engine_free_particles(b->eng);
+    out[3] = free_slots;
+    return SLAB_OK;
+}
+
+slab_status slab_batch_use_graph(slab_batch* b, int32_t on) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    engine_use_graph(b->eng, on);
+    return SLAB_OK;
+}
+
+slab_status slab_batch_timing(slab_batch* b, int32_t enable, double* ms, int64_t* counts, int32_t n) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    engine_timing(b->eng, enable, ms, counts, n);
+    return SLAB_OK;
+}
+
+void slab_batch_destroy(slab_batch* b) {
+    if (!b) return;
+    engine_destroy(b->eng);
+    delete b;
 }
 
 } // extern "C"

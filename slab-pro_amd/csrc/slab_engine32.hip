@@ -1,0 +1,1628 @@
+// slab_engine32.hip -- FP32 step engine for one or many scenarios on a shared
+// DEM (single FP32 runs, the batched ensemble and the request server).
+//
+// Step launch sequence (fixed grid sizes; every kernel reads its counts from
+// device memory, so a step can be replayed from a captured hipGraph):
+//   begin -> tile_reduce -> tile_scan -> tile_offsets -> scatter -> tile_sort
+//   -> p2g -> grid -> g2p -> end
+// Positions are a packed floor cell (ci << 16 | cj) plus FP32 offsets in cell
+// units; h*A = m / rho (A is not stored); time and dt are FP64 per slot.
+// Multiply-add contraction is allowed in this translation unit.
+#include "slab_engine32.hpp"
+#include "slab_device.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+namespace slab {
+
+// ============================================================== device code
+
+__device__ __forceinline__ float Nf(float u) {
+    const float a = fabsf(u);
+    if (a < 0.5f) return 0.75f - a * a;
+    if (a < 1.5f) return 0.5f * (1.5f - a) * (1.5f - a);
+    return 0.0f;
+}
+__device__ __forceinline__ float dNf(float u) {
+    const float a = fabsf(u);
+    if (a < 0.5f) return -2.0f * u;
+    if (a < 1.5f) return u - 1.5f * (float)((0.0f < u) - (u < 0.0f));
+    return 0.0f;
+}
+// Weights / derivatives of the 3 stencil nodes of one axis for a position f
+// relative to the base node (u = f - d), src/tools.hpp:119-140.
+__device__ __forceinline__ void weights3f(float f, float inv, float w[3], float dw[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        w[d] = Nf(f - (float)d);
+        dw[d] = dNf(f - (float)d) * inv;
+    }
+}
+
+__device__ __forceinline__ PB32 ecur(const ECtl* c, const PB32& A, const PB32& B) { return c->parity ? B : A; }
+__device__ __forceinline__ PB32 enxt(const ECtl* c, const PB32& A, const PB32& B) { return c->parity ? A : B; }
+
+// Stencil base trunc(x/dx - 0.5) = ci - (o < 0.5) of src/simulation.cpp:741-742,
+// clamped so the stencil stays on the terrain.
+__device__ __forceinline__ int ebase(int ci, float o, int nmax) {
+    int b = ci - (o < 0.5f ? 1 : 0);
+    return b < 0 ? 0 : (b > nmax ? nmax : b);
+}
+
+__device__ __forceinline__ bool slot_active(int st) { return st == SLOT_PRIMING || st == SLOT_RUNNING; }
+
+// Key of a particle: live (slot, tile, cell), parking tile of a finished slot,
+// or -1 (released slot: the particle drops out of the next sort).
+__device__ __forceinline__ int ekey(const ECtl* c, int st, int slot, int cell, float ox, float oy, int* tile) {
+    if (slot_active(st)) {
+        const int ci = cell >> 16, cj = cell & 0xffff;
+        const int ib = ebase(ci, ox, c->nx - 3), jb = ebase(cj, oy, c->ny - 3);
+        const int t = slot * c->ntd + (ib >> 4) * c->nty + (jb >> 4);
+        *tile = t;
+        return t * TILE_CELLS + (((ib & 15) << 4) | (jb & 15));
+    }
+    if (st == SLOT_DONE || st == SLOT_FAILED) {
+        const int t = c->nslots * c->ntd + slot;
+        *tile = t;
+        return t * TILE_CELLS;
+    }
+    *tile = 0;
+    return -1;
+}
+
+// Per-slot max |v|^2 as a sharded integer max of the IEEE bits (|v|^2 >= 0;
+// NaN sorts above +inf).  Particles are sorted by slot, so a block almost
+// always holds one slot: it reduces in LDS and issues one atomic on shard
+// blockIdx % VMAX_SHARDS; blocks that straddle slots fall back to one atomic
+// per wave (or per lane for a wave that straddles).  All threads must call.
+__device__ __forceinline__ void publish_vmax(int slot, bool valid, float v2, unsigned long long* __restrict__ vmx) {
+    __shared__ int s_slot[4];
+    __shared__ int s_uni[4];
+    __shared__ unsigned long long s_max[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long bits = valid ? (unsigned long long)__double_as_longlong((double)v2) : 0ull;
+    const unsigned long long vmask = __ballot(valid);
+    const int first = vmask ? __ffsll((long long)vmask) - 1 : 0;
+    const int s0 = __shfl(slot, first, 64);
+    const unsigned long long same = __ballot(valid && slot == s0);
+    const unsigned long long m = wave_max_u64(bits);
+    const bool uni = same == vmask;
+    if (lane == 0) {
+        s_slot[wv] = vmask ? s0 : -1;
+        s_uni[wv] = uni;
+        s_max[wv] = m;
+    }
+    if (!uni && valid) atomicMax(&vmx[(size_t)slot * VMAX_SHARDS + (blockIdx.x & (VMAX_SHARDS - 1))], bits);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned sh = blockIdx.x & (VMAX_SHARDS - 1);
+        int bs = -1;
+        bool block_uni = true;
+        for (int k = 0; k < 4; ++k) {
+            if (s_slot[k] < 0) continue;
+            if (!s_uni[k] || (bs >= 0 && s_slot[k] != bs)) block_uni = false;
+            if (bs < 0) bs = s_slot[k];
+        }
+        if (bs >= 0 && block_uni) {
+            unsigned long long mm = 0;
+            for (int k = 0; k < 4; ++k)
+                if (s_slot[k] >= 0) mm = s_max[k] > mm ? s_max[k] : mm;
+            atomicMax(&vmx[(size_t)bs * VMAX_SHARDS + sh], mm);
+        } else {
+            for (int k = 0; k < 4; ++k)
+                if (s_slot[k] >= 0 && s_uni[k]) atomicMax(&vmx[(size_t)s_slot[k] * VMAX_SHARDS + sh], s_max[k]);
+        }
+    }
+}
+
+// Tile slots for arbitrary valid masks: one atomic per run of equal tiles
+// among consecutive valid lanes.  Every lane of the wave must call this.
+__device__ __forceinline__ int wave_slot_any(int t, bool valid, int* __restrict__ tile_count) {
+    const int lane = threadIdx.x & 63;
+    const int tprev = __shfl_up(t, 1, 64);
+    const bool vprev = __shfl_up((int)valid, 1, 64) != 0;
+    const bool head = valid && (lane == 0 || !vprev || tprev != t);
+    const unsigned long long vmask = __ballot(valid);
+    const unsigned long long hmask = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long hle = hmask & upto;
+    const unsigned long long brk = (hmask | ~vmask) & ~upto; // next head or next invalid lane
+    const int start = hle ? 63 - __clzll((long long)hle) : 0;
+    const int end = brk ? (__ffsll((long long)brk) - 1) : 64;
+    int base = 0;
+    if (head) base = atomicAdd(&tile_count[t], end - start);
+    base = __shfl(base, start, 64);
+    return base + (lane - start);
+}
+
+// --- control ---------------------------------------------------------------
+
+__global__ void k_eclear(ECtl* c, Slot* __restrict__ slots) {
+    if (threadIdx.x == 0) {
+        if (c->stop == STOP_FRAME || c->stop == STOP_DONE) c->stop = RUNNING;
+        c->steps_in_call = 0;
+    }
+    for (int s = threadIdx.x; s < c->nslots; s += blockDim.x) {
+        slots[s].frame_hit = 0;
+        slots[s].steps_in_call = 0;
+    }
+}
+
+// Start of a step: per slot, the while condition (src/simulation.cpp:500) and
+// updateDt (565-582) from the slot's own max |v|; priming slots get dt = 0
+// (simulation.hpp:39, simulation.cpp:493-495).
+__global__ void __launch_bounds__(WG) k_ebegin(ECtl* c, Slot* __restrict__ slots,
+                                               unsigned long long* __restrict__ vmx) {
+    if (c->stop) return;
+    if (c->compact) { // re-sort pass: no slot advances
+        if (threadIdx.x == 0) {
+            c->epoch += 1;
+            c->p2g_parity = c->parity;
+        }
+        return;
+    }
+    __shared__ int s_act;
+    if (threadIdx.x == 0) s_act = 0;
+    __syncthreads();
+    const int ep = c->epoch + 1;
+    int act = 0;
+    for (int s = threadIdx.x; s < c->nslots; s += WG) {
+        Slot& S = slots[s];
+        unsigned long long v = 0;
+        for (int k = 0; k < VMAX_SHARDS; ++k) {
+            const unsigned long long q = vmx[(size_t)s * VMAX_SHARDS + k];
+            v = q > v ? q : v;
+            vmx[(size_t)s * VMAX_SHARDS + k] = 0ull;
+        }
+        if (S.status == SLOT_PRIMING) {
+            S.dt = 0.0;
+            S.out_step = 1;
+            S.last_out_epoch = ep;
+            ++act;
+            continue;
+        }
+        if (S.status != SLOT_RUNNING) continue;
+        if (!(S.time < S.total_time)) {
+            S.status = SLOT_DONE;
+            continue;
+        }
+        const double v2 = __longlong_as_double(v);
+        if (!(v2 <= 1.7976931348623157e308)) {
+            S.status = SLOT_FAILED;
+            S.err = 5; // SLAB_ENAN
+            continue;
+        }
+        const double max_speed = sqrt(v2);
+        double dt;
+        if (max_speed > 0.0) {
+            const double a = S.C_cfl * c->dx / max_speed;
+            dt = (S.dt_max < a) ? S.dt_max : a;
+        } else {
+            dt = S.dt_max;
+        }
+        const double next_frame_time = S.output_dt * (double)(S.frame + 1);
+        if (S.time + dt > next_frame_time - 1e-12) dt = next_frame_time - S.time;
+        S.dt = dt;
+        S.step += 1;
+        const double tnew = S.time + dt;
+        const int hit = (S.output_dt * (double)(S.frame + 1) - tnew < 1e-13);
+        S.out_step = hit || S.keep_grid;
+        if (S.out_step) S.last_out_epoch = ep;
+        ++act;
+    }
+    if (act) atomicAdd(&s_act, act);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c->epoch = ep;
+        c->p2g_parity = c->parity;
+        c->n_active = s_act;
+        if (s_act == 0) c->stop = STOP_DONE; // nothing to advance
+    }
+}
+
+// End of a step: time += dt and the frame trigger (src/simulation.cpp:503-509).
+__global__ void __launch_bounds__(WG) k_eend(ECtl* c, Slot* __restrict__ slots) {
+    if (c->stop) return;
+    if (c->compact) {
+        if (threadIdx.x == 0) {
+            c->parity ^= 1;
+            c->n_store = c->n_sorted;
+            c->compact = 0;
+        }
+        return;
+    }
+    __shared__ int s_hit;
+    if (threadIdx.x == 0) s_hit = 0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < c->nslots; s += WG) {
+        Slot& S = slots[s];
+        if (S.status == SLOT_PRIMING) {
+            S.status = SLOT_RUNNING;
+            continue;
+        }
+        if (S.status != SLOT_RUNNING) continue;
+        S.time += S.dt;
+        S.steps_in_call += 1;
+        if (S.output_dt * (double)(S.frame + 1) - S.time < 1e-13) {
+            S.frame += 1;
+            S.frame_hit = 1;
+            atomicAdd(&s_hit, 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c->parity ^= 1;
+        c->n_store = c->n_sorted;
+        if (c->n_active > 0) c->steps_in_call += 1;
+        if (c->single && s_hit) c->stop = STOP_FRAME;
+    }
+}
+
+// --- keys of admitted particles ---------------------------------------------
+
+__global__ void __launch_bounds__(WG) k_ekeys(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B, int s0,
+                                              int s1, int* __restrict__ key, int* __restrict__ tslot,
+                                              int* __restrict__ tile_count) {
+    const PB32 P = ecur(c, A, B);
+    const int s = s0 + blockIdx.x * WG + threadIdx.x;
+    const bool in = s < s1;
+    int t = 0, k = -1;
+    if (in) {
+        const int sl = P.slot[s];
+        k = ekey(c, slots[sl].status, sl, P.cell[s], P.ox[s], P.oy[s], &t);
+    }
+    const bool valid = in && k >= 0;
+    const int q = wave_slot_any(t, valid, tile_count);
+    if (in) {
+        key[s] = k;
+        tslot[s] = q;
+    }
+}
+
+// Initial particle z, src/simulation.cpp:455-465 (4x4 quadratic weights).
+__global__ void k_ez(ECtl* c, PB32 A, PB32 B, int s0, int s1, const float* __restrict__ tz) {
+    const PB32 P = ecur(c, A, B);
+    const int s = s0 + blockIdx.x * WG + threadIdx.x;
+    if (s >= s1) return;
+    const int cell = P.cell[s];
+    const int ci = cell >> 16, cj = cell & 0xffff;
+    const float ox = P.ox[s], oy = P.oy[s];
+    float z = 0.0f;
+    for (int i = ci - 1; i < ci + 3; ++i) {
+        for (int j = cj - 1; j < cj + 3; ++j) {
+            if (i < 0 || j < 0 || i >= c->nx || j >= c->ny) continue;
+            z += tz[(size_t)i * c->ny + j] * (Nf((float)(ci - i) + ox) * Nf((float)(cj - j) + oy));
+        }
+    }
+    P.z[s] = z;
+}
+
+// --- tile scan over (slot, tile) + parking tiles ------------------------------
+
+__device__ __forceinline__ void etile_flags(const ECtl* c, const int* __restrict__ tile_count, int t, int ntiles,
+                                            int f[3]) {
+    f[0] = f[1] = f[2] = 0;
+    if (t >= ntiles) return;
+    const int n = tile_count[t];
+    f[0] = n;
+    f[1] = n > 0;
+    const int live = c->nslots * c->ntd;
+    if (t >= live) return; // parking tile: particles, never a grid tile
+    const int local = t % c->ntd, nty = c->nty;
+    const int tx = local / nty, ty = local - tx * nty;
+    int g = n > 0;
+    if (tx > 0) g |= tile_count[t - nty] > 0;
+    if (ty > 0) g |= tile_count[t - 1] > 0;
+    if (tx > 0 && ty > 0) g |= tile_count[t - nty - 1] > 0;
+    f[2] = g;
+}
+
+__device__ __forceinline__ void block_exscan3e(int v[3], int out[3], int tot[3]) {
+    __shared__ int s_w[4][3];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc[3] = {v[0], v[1], v[2]};
+    for (int o = 1; o < 64; o <<= 1) {
+        for (int k = 0; k < 3; ++k) {
+            int u = __shfl_up(inc[k], o, 64);
+            if (lane >= o) inc[k] += u;
+        }
+    }
+    if (lane == 63)
+        for (int k = 0; k < 3; ++k) s_w[wv][k] = inc[k];
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) {
+        int pre = 0, all = 0;
+        for (int w = 0; w < 4; ++w) {
+            if (w < wv) pre += s_w[w][k];
+            all += s_w[w][k];
+        }
+        out[k] = pre + inc[k] - v[k];
+        tot[k] = all;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, const int* __restrict__ tile_count, int ntiles,
+                                                int* __restrict__ bsum) {
+    if (c->stop) return;
+    int acc[3] = {0, 0, 0};
+    const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        int f[3];
+        etile_flags(c, tile_count, t0 + k, ntiles, f);
+        acc[0] += f[0];
+        acc[1] += f[1];
+        acc[2] += f[2];
+    }
+    int ex[3], tot[3];
+    block_exscan3e(acc, ex, tot);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) bsum[3 * blockIdx.x + k] = tot[k];
+}
+
+__global__ void __launch_bounds__(WG) k_escan(ECtl* c, const int* __restrict__ bsum, int nblk,
+                                              int* __restrict__ boff) {
+    if (c->stop) return;
+    int run[3] = {0, 0, 0};
+    for (int b0 = 0; b0 < nblk; b0 += WG) {
+        const int b = b0 + threadIdx.x;
+        int v[3] = {0, 0, 0};
+        if (b < nblk)
+            for (int k = 0; k < 3; ++k) v[k] = bsum[3 * b + k];
+        int ex[3], tot[3];
+        block_exscan3e(v, ex, tot);
+        if (b < nblk)
+            for (int k = 0; k < 3; ++k) boff[3 * b + k] = run[k] + ex[k];
+        for (int k = 0; k < 3; ++k) run[k] += tot[k];
+    }
+    if (threadIdx.x == 0) {
+        c->n_sorted = run[0];
+        c->n_ptiles = run[1];
+        c->n_gtiles = run[2];
+        if (run[1] > c->tile_cap) { // active-tile capacity exceeded: stop with an error
+            c->err = 7;
+            c->stop = STOP_ERROR;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const int* __restrict__ tile_count, int ntiles,
+                                                 const int* __restrict__ boff, int* __restrict__ tile_begin,
+                                                 int* __restrict__ ptiles, int* __restrict__ gtiles,
+                                                 int* __restrict__ pstamp, int* __restrict__ gstamp,
+                                                 int* __restrict__ tile_pos) {
+    if (c->stop) return;
+    const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    int f[SCAN_ITEMS][3];
+    int acc[3] = {0, 0, 0};
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        etile_flags(c, tile_count, t0 + k, ntiles, f[k]);
+        for (int q = 0; q < 3; ++q) acc[q] += f[k][q];
+    }
+    int ex[3], tot[3];
+    block_exscan3e(acc, ex, tot);
+    int run[3];
+    for (int q = 0; q < 3; ++q) run[q] = boff[3 * blockIdx.x + q] + ex[q];
+    const int ep = c->epoch;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const int t = t0 + k;
+        if (t >= ntiles) break;
+        tile_begin[t] = run[0];
+        if (f[k][1]) {
+            ptiles[run[1]] = t;
+            tile_pos[t] = run[1];
+            pstamp[t] = ep;
+        }
+        if (f[k][2]) {
+            gtiles[run[2]] = t;
+            gstamp[t] = ep;
+        }
+        for (int q = 0; q < 3; ++q) run[q] += f[k][q];
+    }
+}
+
+// --- sort ------------------------------------------------------------------
+
+__global__ void k_escatter(ECtl* c, const int* __restrict__ key, const int* __restrict__ tslot,
+                           const int* __restrict__ tile_begin, int* __restrict__ tmp) {
+    if (c->stop) return;
+    const int s = blockIdx.x * WG + threadIdx.x;
+    if (s >= c->n_store) return;
+    const int k = key[s];
+    if (k < 0) return;
+    tmp[tile_begin[k >> 8] + tslot[s]] = s;
+}
+
+// Per particle tile: counting sort by cell in LDS, then each cell's run sorted
+// by creation index: canonical order (slot, tile, cell, creation index).
+// Parking tiles keep their bucket order (their particles are only carried).
+__global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int* __restrict__ key,
+                                              const int* __restrict__ ptiles, int* __restrict__ tile_count,
+                                              const int* __restrict__ tile_begin, const int* __restrict__ tmp,
+                                              int* __restrict__ idx, int* __restrict__ cell_begin) {
+    if (c->stop) return;
+    __shared__ int s_hist[TILE_CELLS];
+    __shared__ int s_off[TILE_CELLS];
+    __shared__ int s_w[4];
+    __shared__ int s_n;
+    const int32_t* __restrict__ pid = ecur(c, A, B).pid;
+    const int nt = c->n_ptiles, live = c->nslots * c->ntd;
+    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        const int t = ptiles[w];
+        if (threadIdx.x == 0) s_n = tile_count[t];
+        s_hist[threadIdx.x] = 0;
+        __syncthreads();
+        const int n = s_n;
+        const int b = tile_begin[t];
+        int* cb = cell_begin + (size_t)w * CELL_STRIDE;
+        if (t >= live) { // parking tile
+            for (int r = threadIdx.x; r < n; r += WG) idx[b + r] = tmp[b + r];
+            cb[threadIdx.x] = threadIdx.x == 0 ? b : b + n;
+            if (threadIdx.x == 0) {
+                cb[TILE_CELLS] = b + n;
+                tile_count[t] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        for (int r = threadIdx.x; r < n; r += WG) atomicAdd(&s_hist[key[tmp[b + r]] & 255], 1);
+        __syncthreads();
+        {
+            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+            const int v = s_hist[threadIdx.x];
+            int inc = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                int u = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += u;
+            }
+            if (lane == 63) s_w[wv] = inc;
+            __syncthreads();
+            int pre = 0;
+            for (int q = 0; q < wv; ++q) pre += s_w[q];
+            s_off[threadIdx.x] = pre + inc - v;
+            cb[threadIdx.x] = b + pre + inc - v;
+            if (threadIdx.x == 0) {
+                cb[TILE_CELLS] = b + n;
+                tile_count[t] = 0;
+            }
+        }
+        __syncthreads();
+        s_hist[threadIdx.x] = s_off[threadIdx.x];
+        __syncthreads();
+        for (int r = threadIdx.x; r < n; r += WG) {
+            const int s = tmp[b + r];
+            const int pos = atomicAdd(&s_hist[key[s] & 255], 1);
+            idx[b + pos] = s;
+        }
+        __syncthreads();
+        {
+            const int lo = b + s_off[threadIdx.x];
+            const int hi = b + s_hist[threadIdx.x];
+            for (int q = lo + 1; q < hi; ++q) {
+                const int sv = idx[q];
+                const int pv = pid[sv];
+                int k = q - 1;
+                while (k >= lo && pid[idx[k]] > pv) {
+                    idx[k + 1] = idx[k];
+                    --k;
+                }
+                idx[k + 1] = sv;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// --- P2G -------------------------------------------------------------------
+
+// P2G, src/simulation.cpp:726-798, as a per-tile segmented reduction without
+// atomics: one thread per stencil-base cell sums its particles (canonical
+// order) into 9 stencil slots x 7 fields in registers; 9 phases then add slot
+// (d,e) of every cell into node cell+(d,e) of an 18x18-node LDS region (one-to
+// -one within a phase: no conflicts, fixed order); the region is this tile's
+// partial, merged per node by k_egrid.
+__global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+                                             const int* __restrict__ idx, const int* __restrict__ ptiles,
+                                             const int* __restrict__ cell_begin, const float* __restrict__ tgx,
+                                             const float* __restrict__ tgy, const float* __restrict__ tcth,
+                                             float* __restrict__ tpart) {
+    if (c->stop || c->compact) return;
+    __shared__ float s_acc[P2G_FIELDS * REGION_NODES];
+    const PB32 P = ecur(c, A, B);
+    const int nt = c->n_ptiles, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
+    const float inv = (float)c->inv_dx;
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        const int t = ptiles[w];
+        if (t >= live) continue;
+        const int slot = t / ntd, local = t - slot * ntd;
+        if (!slot_active(slots[slot].status)) continue;
+        const float g = (float)slots[slot].gravity;
+        const int tx = local / nty, ty = local - tx * nty;
+        const int ib = tx * TILE + lx, jb = ty * TILE + ly;
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
+        float acc[9][P2G_FIELDS];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
+        const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
+        const int r0 = cb[threadIdx.x], r1 = cb[threadIdx.x + 1];
+        for (int r = r0; r < r1; ++r) {
+            const int s = idx[r];
+            const int cell = P.cell[s];
+            const int ci = cell >> 16, cj = cell & 0xffff;
+            const size_t fc = (size_t)ci * ny + cj;
+            const float fx = (float)(ci - ib) + P.ox[s], fy = (float)(cj - jb) + P.oy[s];
+            const float mp = P.m[s], hp = P.h[s];
+            const float cth = tcth[fc];
+            const float coeff = -mp * g * cth * cth;
+            const float vmx = P.vx[s] * mp, vmy = P.vy[s] * mp, vzm = P.vz[s] * mp, hm = hp * mp;
+            const float mbx = coeff * tgx[fc], mby = coeff * tgy[fc];
+            // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
+            const float ps = -0.5f * mp * g * cth * cth * hp;
+            float wx[3], wy[3], dwx[3], dwy[3];
+            weights3f(fx, inv, wx, dwx);
+            weights3f(fy, inv, wy, dwy);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    const float wgt = wx[d] * wy[e];
+                    if (wgt > 3e-3f) { // simulation.cpp:780
+                        float* a = acc[d * 3 + e];
+                        a[0] += mp * wgt;
+                        a[1] += vmx * wgt;
+                        a[2] += vmy * wgt;
+                        a[3] += vzm * wgt;
+                        a[4] += hm * wgt;
+                        a[5] += mbx * wgt - ps * (dwx[d] * wy[e]);
+                        a[6] += mby * wgt - ps * (dwy[e] * wx[d]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ph = 0; ph < 9; ++ph) {
+            const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) s_acc[f * REGION_NODES + node] += acc[ph][f];
+            __syncthreads();
+        }
+        float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) dst[q] = s_acc[q];
+        __syncthreads();
+    }
+}
+
+// --- grid ------------------------------------------------------------------
+
+// computeGridVelocity (src/simulation.cpp:800-875) for one grid tile per
+// workgroup, one thread per node, after merging the node's partial sums from
+// the particle tiles of the same slot that reach it (own, left, below,
+// diagonal; fixed order).
+__global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ slots,
+                                              const int* __restrict__ gtiles, const int* __restrict__ pstamp,
+                                              const int* __restrict__ tile_pos, const float* __restrict__ tpart,
+                                              const float* __restrict__ tgx, const float* __restrict__ tgy,
+                                              const float* __restrict__ tcth, float* __restrict__ h_max,
+                                              float* __restrict__ v_max2, float4* __restrict__ vf,
+                                              float* __restrict__ out_m, float* __restrict__ out_h,
+                                              float* __restrict__ out_vz) {
+    if (c->stop || c->compact) return;
+    const int nt = c->n_gtiles, ep = c->epoch, nx = c->nx, ny = c->ny, nty = c->nty, ntd = c->ntd;
+    const size_t nn = (size_t)nx * ny;
+    const int a = threadIdx.x >> 4, b = threadIdx.x & 15;
+    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        const int t = gtiles[w];
+        const int slot = t / ntd, local = t - slot * ntd;
+        const Slot& S = slots[slot];
+        if (!slot_active(S.status)) continue;
+        const int tx = local / nty, ty = local - tx * nty;
+        const int i = tx * TILE + a, j = ty * TILE + b;
+        if (i >= nx || j >= ny) continue;
+        float acc[P2G_FIELDS];
+#pragma unroll
+        for (int f = 0; f < P2G_FIELDS; ++f) acc[f] = 0.f;
+        int src[4], rn[4];
+        int ns = 0;
+        src[ns] = t;
+        rn[ns++] = a * REGION + b;
+        if (tx > 0 && a < 2) {
+            src[ns] = t - nty;
+            rn[ns++] = (a + TILE) * REGION + b;
+        }
+        if (ty > 0 && b < 2) {
+            src[ns] = t - 1;
+            rn[ns++] = a * REGION + (b + TILE);
+        }
+        if (tx > 0 && ty > 0 && a < 2 && b < 2) {
+            src[ns] = t - nty - 1;
+            rn[ns++] = (a + TILE) * REGION + (b + TILE);
+        }
+        for (int k = 0; k < ns; ++k) {
+            if (pstamp[src[k]] != ep) continue;
+            const float* q = tpart + (size_t)tile_pos[src[k]] * (P2G_FIELDS * REGION_NODES) + rn[k];
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) acc[f] += q[f * REGION_NODES];
+        }
+        const float g = (float)S.gravity, dt = (float)S.dt;
+        const float am = acc[0];
+        const size_t kt = (size_t)i * ny + j; // terrain node
+        const size_t k = (size_t)slot * nn + kt;
+        float nvx = 0.f, nvy = 0.f, flx = 0.f, fly = 0.f, hh = 0.f, vz = 0.f;
+        if (am > 1e-6f) {
+            const float rm = 1.0f / am;
+            const float vx = acc[1] * rm, vy = acc[2] * rm;
+            vz = acc[3] * rm;
+            hh = acc[4] * rm;
+            const float vtx = vx + dt * (acc[5] * rm), vty = vy + dt * (acc[6] * rm);
+            const float vsq = vtx * vtx + vty * vty;
+            if (vsq > 1e-16f) {
+                const float Gx = tgx[kt], Gy = tgy[kt], cth = tcth[kt];
+                const float v_z = vtx * Gx + vty * Gy;
+                const float v3sq = vsq + v_z * v_z;
+                const float vn = sqrtf(v3sq);
+                float af;
+                if (S.fric == 0) {
+                    af = (float)S.mu * g * cth;
+                } else if (S.fric == 1) {
+                    af = (float)S.mu * g * cth + g * v3sq / ((float)S.xi * hh * cth);
+                } else {
+                    const float I = 5.f * (float)S.d * vn / (2.f * hh * cth * cth * sqrtf(g * hh * (float)S.phi));
+                    const float muI = (float)S.mu1 + ((float)S.mu2 - (float)S.mu1) / (1.f + (float)S.I0 / I);
+                    af = muI * g * cth;
+                }
+                const float dv = dt * af;
+                if (dv < vn) {
+                    const float f = 1.f - dv / vn;
+                    nvx = vtx * f;
+                    nvy = vty * f;
+                }
+            }
+            flx = nvx - vx;
+            fly = nvy - vy;
+            if (h_max[k] < hh) h_max[k] = hh;
+            const float v2 = nvx * nvx + nvy * nvy;
+            if (v_max2[k] < v2) v_max2[k] = v2;
+        }
+        vf[k] = make_float4(nvx, nvy, flx, fly);
+        if (out_m && S.out_step) {
+            out_m[k] = am;
+            out_h[k] = hh;
+            out_vz[k] = vz;
+        }
+    }
+}
+
+// --- G2P -------------------------------------------------------------------
+
+// G2P (src/simulation.cpp:877-940) for running slots; priming / finished
+// slots are carried unchanged.  Writes the other buffer in sorted order, the
+// next keys and tile slots, and the per-slot max |v|^2.
+__global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+                                             const int* __restrict__ idx, const float4* __restrict__ vf,
+                                             const float* __restrict__ tz, const float* __restrict__ tgx,
+                                             const float* __restrict__ tgy, int* __restrict__ key,
+                                             int* __restrict__ tslot, int* __restrict__ tile_count,
+                                             unsigned long long* __restrict__ vmx) {
+    if (c->stop) return;
+    const PB32 P = ecur(c, A, B);
+    const PB32 Q = enxt(c, A, B);
+    const int r = blockIdx.x * WG + threadIdx.x;
+    const bool in = r < c->n_sorted;
+    int t = 0, kk = -1, sl = 0;
+    float v2 = 0.f;
+    bool pub = false;
+    if (in) {
+        const int nx = c->nx, ny = c->ny;
+        const size_t nn = (size_t)nx * ny;
+        const int s = idx[r];
+        sl = P.slot[s];
+        const Slot& S = slots[sl];
+        const int st = S.status;
+        int cell = P.cell[s];
+        float ox = P.ox[s], oy = P.oy[s], vxp = P.vx[s], vyp = P.vy[s], vzp = P.vz[s], hp = P.h[s], zp = P.z[s];
+        if (st == SLOT_RUNNING && !c->compact) {
+            const float inv = (float)c->inv_dx, dt = (float)S.dt, fr = (float)S.flip_ratio;
+            const int ci = cell >> 16, cj = cell & 0xffff;
+            const size_t fc = (size_t)ci * ny + cj;
+            const float Gx = tgx[fc], Gy = tgy[fc];
+            const int ib = ebase(ci, ox, nx - 3), jb = ebase(cj, oy, ny - 3);
+            const float fx = (float)(ci - ib) + ox, fy = (float)(cj - jb) + oy;
+            float wx[3], wy[3], dwx[3], dwy[3];
+            weights3f(fx, inv, wx, dwx);
+            weights3f(fy, inv, wy, dwy);
+            float picx = 0.f, picy = 0.f, flx = 0.f, fly = 0.f, z = 0.f, tr = 0.f;
+            const float4* vfs = vf + (size_t)sl * nn;
+#pragma unroll
+            for (int di = 0; di < 3; ++di) {
+#pragma unroll
+                for (int dj = 0; dj < 3; ++dj) {
+                    const size_t k = (size_t)(ib + di) * ny + (jb + dj);
+                    const float w = wx[di] * wy[dj];
+                    const float4 q = vfs[k];
+                    picx += q.x * w;
+                    picy += q.y * w;
+                    flx += q.z * w;
+                    fly += q.w * w;
+                    z += tz[k] * w;
+                    tr += dwx[di] * wy[dj] * q.x + dwy[dj] * wx[di] * q.y;
+                }
+            }
+            tr *= dt;
+            const float nvx = fr * (vxp + flx) + (1.f - fr) * picx;
+            const float nvy = fr * (vyp + fly) + (1.f - fr) * picy;
+            float nox = ox + dt * picx * inv, noy = oy + dt * picy * inv;
+            const float fxs = floorf(nox), fys = floorf(noy);
+            int nci = ci + (int)fxs, ncj = cj + (int)fys;
+            nox -= fxs;
+            noy -= fys;
+            if (nox >= 1.f) { nox -= 1.f; nci += 1; }
+            if (noy >= 1.f) { noy -= 1.f; ncj += 1; }
+            if (!(nox >= 0.f)) nox = 0.f; // NaN guard: keeps indices in range
+            if (!(noy >= 0.f)) noy = 0.f;
+            if (nci < 0) { nci = 0; nox = 0.f; }
+            if (nci > nx - 1) { nci = nx - 1; nox = 0.f; }
+            if (ncj < 0) { ncj = 0; noy = 0.f; }
+            if (ncj > ny - 1) { ncj = ny - 1; noy = 0.f; }
+            cell = (nci << 16) | ncj;
+            ox = nox;
+            oy = noy;
+            vxp = nvx;
+            vyp = nvy;
+            vzp = nvx * Gx + nvy * Gy;
+            hp = hp / (1.f + tr);
+            zp = z;
+        }
+        Q.cell[r] = cell;
+        Q.ox[r] = ox;
+        Q.oy[r] = oy;
+        Q.vx[r] = vxp;
+        Q.vy[r] = vyp;
+        Q.vz[r] = vzp;
+        Q.h[r] = hp;
+        Q.z[r] = zp;
+        Q.m[r] = P.m[s];
+        Q.pid[r] = P.pid[s];
+        Q.slot[r] = sl;
+        kk = ekey(c, st, sl, cell, ox, oy, &t);
+        pub = slot_active(st);
+        v2 = vxp * vxp + vyp * vyp;
+    }
+    const int q = wave_slot_any(t, in && kk >= 0, tile_count);
+    if (in) {
+        key[r] = kk;
+        tslot[r] = q;
+    }
+    publish_vmax(sl, pub, v2, vmx);
+}
+
+// --- downloads / diagnostics -------------------------------------------------
+
+// Compacts one slot's particles of the current buffer into out[0..] (order
+// arbitrary; the host scatters by creation index).
+__global__ void k_ecollect(ECtl* c, PB32 A, PB32 B, int slot, int* __restrict__ counter, int32_t* __restrict__ o_cell,
+                           float* __restrict__ o_f, int32_t* __restrict__ o_pid, int cap) {
+    const PB32 P = ecur(c, A, B);
+    const int s = blockIdx.x * WG + threadIdx.x;
+    if (s >= c->n_store) return;
+    if (P.slot[s] != slot) return;
+    const int q = atomicAdd(counter, 1);
+    if (q >= cap) return;
+    o_cell[q] = P.cell[s];
+    o_pid[q] = P.pid[s];
+    float* f = o_f + (size_t)q * 8;
+    f[0] = P.ox[s];
+    f[1] = P.oy[s];
+    f[2] = P.vx[s];
+    f[3] = P.vy[s];
+    f[4] = P.vz[s];
+    f[5] = P.z[s];
+    f[6] = P.h[s];
+    f[7] = P.m[s];
+}
+
+// Bounding box of one slot's particles in the buffer its last P2G read
+// (for the reference's remesh window, src/simulation.cpp:594-606).
+__global__ void k_ebbox(ECtl* c, PB32 A, PB32 B, int slot, int n, unsigned long long* __restrict__ bb) {
+    const PB32 P = c->p2g_parity ? B : A;
+    const int s = blockIdx.x * WG + threadIdx.x;
+    if (s >= n || P.slot[s] != slot) return;
+    const int cell = P.cell[s];
+    const double x = ((double)(cell >> 16) + (double)P.ox[s]) * c->dx;
+    const double y = ((double)(cell & 0xffff) + (double)P.oy[s]) * c->dx;
+    atomicMin(&bb[0], ord_bits(x));
+    atomicMax(&bb[1], ord_bits(x));
+    atomicMin(&bb[2], ord_bits(y));
+    atomicMax(&bb[3], ord_bits(y));
+}
+
+__global__ void k_ewindow(const ECtl* c, int slot, int i0, int j0, int wnx, int wny, const int* __restrict__ gstamp,
+                          const float4* __restrict__ vf, const float* __restrict__ gm, const float* __restrict__ gh,
+                          const float* __restrict__ gvz, double* __restrict__ o) {
+    const int q = blockIdx.x * WG + threadIdx.x;
+    const size_t wn = (size_t)wnx * wny;
+    if ((size_t)q >= wn) return;
+    const int a = q / wny, b = q - a * wny;
+    const int i = i0 + a, j = j0 + b;
+    const int t = slot * c->ntd + (i >> 4) * c->nty + (j >> 4);
+    double h = 0, vx = 0, vy = 0, vz = 0, m = 0;
+    if (gstamp[t] == c->epoch) {
+        const size_t k = (size_t)slot * c->nx * c->ny + (size_t)i * c->ny + j;
+        const float4 v = vf[k];
+        vx = v.x;
+        vy = v.y;
+        if (gm) {
+            h = gh[k];
+            vz = gvz[k];
+            m = gm[k];
+        }
+    }
+    o[q] = h;
+    o[wn + q] = vx;
+    o[2 * wn + q] = vy;
+    o[3 * wn + q] = vz;
+    o[4 * wn + q] = m;
+}
+
+// FP64 sums over one slot's current grid tiles and particles.
+__global__ void __launch_bounds__(WG) k_ediag(const ECtl* c, PB32 A, PB32 B, int slot, const int* __restrict__ gtiles,
+                                              const float4* __restrict__ vf, const float* __restrict__ gm,
+                                              double rho, double* __restrict__ part) {
+    __shared__ double sh[6][WG];
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    const int nty = c->nty, ntd = c->ntd;
+    const size_t nn = (size_t)c->nx * c->ny;
+    if (gm) {
+        for (int w = blockIdx.x; w < c->n_gtiles; w += gridDim.x) {
+            const int t = gtiles[w];
+            if (t / ntd != slot) continue;
+            const int local = t - slot * ntd;
+            const int tx = local / nty, ty = local - tx * nty;
+            const int i = tx * TILE + (threadIdx.x >> 4), j = ty * TILE + (threadIdx.x & 15);
+            if (i < c->nx && j < c->ny) {
+                const size_t k = (size_t)slot * nn + (size_t)i * c->ny + j;
+                const double m = gm[k];
+                const float4 v = vf[k];
+                a[0] += m;
+                if (m > 1e-6) {
+                    a[1] += m * (double)v.x;
+                    a[2] += m * (double)v.y;
+                    a[3] += 1.0;
+                }
+            }
+        }
+    }
+    const PB32 P = ecur(c, A, B);
+    for (int r = blockIdx.x * WG + threadIdx.x; r < c->n_store; r += gridDim.x * WG) {
+        if (P.slot[r] != slot) continue;
+        const double m = P.m[r];
+        a[4] += m;
+        a[5] += m / rho;
+    }
+    for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int o = WG / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) part[6 * blockIdx.x + q] = sh[q][0];
+}
+
+// ============================================================== host code
+
+namespace {
+
+constexpr int kPersistE = 2048;
+constexpr int kNumK = 10;
+
+bool hip_ok(hipError_t e, const char* what, std::string* err) {
+    if (e == hipSuccess) return true;
+    if (err) *err = std::string(what) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return false;
+}
+
+} // namespace
+
+struct Engine32 {
+    hipStream_t stream = nullptr;
+    ECtl* d_c = nullptr;
+    ECtl* h_c = nullptr;
+    Slot* d_slots = nullptr;
+    Slot* h_slots = nullptr;
+    int nslots = 1, nx = 0, ny = 0, ntx = 0, nty = 0, ntd = 0, ntiles = 0, nblk = 0;
+    size_t nn = 0;
+    double dx = 0, inv = 0;
+    bool single = true;
+    int64_t cap = 0;
+    int tile_cap = 0;
+    float *tz = nullptr, *tgx = nullptr, *tgy = nullptr, *tcth = nullptr;
+    float *hmax = nullptr, *vmax2 = nullptr;
+    float4* vf = nullptr;
+    float *om = nullptr, *oh = nullptr, *ovz = nullptr;
+    unsigned long long* vmx = nullptr;
+    int *tile_count = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr, *tile_pos = nullptr;
+    int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr;
+    int* cell_begin = nullptr;
+    float* tpart = nullptr;
+    PB32 A{}, B{};
+    int *key = nullptr, *tslot = nullptr, *idx = nullptr, *tmp = nullptr;
+    double* dpart = nullptr;
+    std::vector<void*> allocs, pallocs;
+    std::vector<int64_t> admit_base; // storage offset of each slot's particles at admission
+    // timing / graph
+    bool timing = false;
+    struct Pend {
+        int kid, step;
+        hipEvent_t a, b;
+    };
+    std::vector<Pend> pending;
+    std::vector<hipEvent_t> ev_free;
+    double ms[kNumK] = {};
+    int64_t cnt[kNumK] = {};
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+template <typename T>
+bool ealloc(std::vector<void*>& list, T** p, size_t n, std::string* err) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (err) *err = "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed";
+        return false;
+    }
+    list.push_back(q);
+    *p = (T*)q;
+    return true;
+}
+
+void efree(std::vector<void*>& l) {
+    for (void* p : l) (void)hipFree(p);
+    l.clear();
+}
+
+bool push_all(Engine32* e, std::string* err) {
+    if (!hip_ok(hipMemcpyAsync(e->d_c, e->h_c, sizeof(ECtl), hipMemcpyHostToDevice, e->stream), "ctl upload", err))
+        return false;
+    if (!hip_ok(hipMemcpyAsync(e->d_slots, e->h_slots, sizeof(Slot) * e->nslots, hipMemcpyHostToDevice, e->stream),
+                "slot upload", err))
+        return false;
+    return hip_ok(hipStreamSynchronize(e->stream), "sync", err);
+}
+
+hipEvent_t ev_get(Engine32* e) {
+    if (!e->ev_free.empty()) {
+        hipEvent_t x = e->ev_free.back();
+        e->ev_free.pop_back();
+        return x;
+    }
+    hipEvent_t x = nullptr;
+    (void)hipEventCreate(&x);
+    return x;
+}
+
+template <typename F>
+void tk(Engine32* e, int kid, int step, F&& f) {
+    if (!e->timing) {
+        f();
+        return;
+    }
+    hipEvent_t a = ev_get(e), b = ev_get(e);
+    (void)hipEventRecord(a, e->stream);
+    f();
+    (void)hipEventRecord(b, e->stream);
+    e->pending.push_back({kid, step, a, b});
+}
+
+unsigned capblocks(const Engine32* e) { return (unsigned)std::max<int64_t>(1, (e->cap + WG - 1) / WG); }
+
+void enqueue_estep(Engine32* e, int step) {
+    hipStream_t s = e->stream;
+    ECtl* c = e->d_c;
+    const unsigned tb = (unsigned)std::max(1, std::min(e->ntiles, kPersistE));
+    const unsigned pb = (unsigned)std::max(1, std::min(e->tile_cap, kPersistE));
+    tk(e, 0, step, [&] { k_ebegin<<<1, WG, 0, s>>>(c, e->d_slots, e->vmx); });
+    tk(e, 1, step, [&] { k_ereduce<<<e->nblk, WG, 0, s>>>(c, e->tile_count, e->ntiles, e->bsum); });
+    tk(e, 2, step, [&] { k_escan<<<1, WG, 0, s>>>(c, e->bsum, e->nblk, e->boff); });
+    tk(e, 3, step, [&] {
+        k_eoffsets<<<e->nblk, WG, 0, s>>>(c, e->tile_count, e->ntiles, e->boff, e->tile_begin, e->ptiles, e->gtiles,
+                                           e->pstamp, e->gstamp, e->tile_pos);
+    });
+    tk(e, 4, step, [&] { k_escatter<<<capblocks(e), WG, 0, s>>>(c, e->key, e->tslot, e->tile_begin, e->tmp); });
+    tk(e, 5, step, [&] {
+        k_esort<<<pb, WG, 0, s>>>(c, e->A, e->B, e->key, e->ptiles, e->tile_count, e->tile_begin, e->tmp, e->idx,
+                                  e->cell_begin);
+    });
+    tk(e, 6, step, [&] {
+        k_ep2g<<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
+                                 e->tpart);
+    });
+    tk(e, 7, step, [&] {
+        k_egrid<<<tb, WG, 0, s>>>(c, e->d_slots, e->gtiles, e->pstamp, e->tile_pos, e->tpart, e->tgx, e->tgy, e->tcth,
+                                  e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz);
+    });
+    tk(e, 8, step, [&] {
+        k_eg2p<<<capblocks(e), WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->tz, e->tgx, e->tgy, e->key,
+                                           e->tslot, e->tile_count, e->vmx);
+    });
+    tk(e, 9, step, [&] { k_eend<<<1, WG, 0, s>>>(c, e->d_slots); });
+}
+
+void drop_graph(Engine32* e) {
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+}
+
+bool build_graph(Engine32* e, std::string* err) {
+    drop_graph(e);
+    if (!hip_ok(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "begin capture", err)) return false;
+    const bool tm = e->timing;
+    e->timing = false;
+    enqueue_estep(e, 0);
+    e->timing = tm;
+    if (!hip_ok(hipStreamEndCapture(e->stream, &e->graph), "end capture", err)) return false;
+    return hip_ok(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0), "graph instantiate", err);
+}
+
+void collect(Engine32* e, int valid_steps) {
+    for (auto& p : e->pending) {
+        if (p.step < valid_steps) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                e->ms[p.kid] += ms;
+                e->cnt[p.kid] += 1;
+            }
+        }
+        e->ev_free.push_back(p.a);
+        e->ev_free.push_back(p.b);
+    }
+    e->pending.clear();
+}
+
+// terrainGradients + terrainTheta (src/complex_terrain.cpp:84-111), host FP64
+// with the host libm, stored as FP32 tables {Gx, Gy, cos(theta)}.
+void terrain32(const double* z, int nx, int ny, double inv, std::vector<float>& gx, std::vector<float>& gy,
+               std::vector<float>& cth) {
+    const size_t nn = (size_t)nx * ny;
+    gx.assign(nn, 0.f);
+    gy.assign(nn, 0.f);
+    cth.assign(nn, 1.f);
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto work = [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            for (int j = 0; j < ny; ++j) {
+                const size_t k = (size_t)i * ny + j;
+                double dzx = 0.0, dzy = 0.0;
+                if (i < nx - 1 && j < ny - 1) {
+                    const double z00 = z[k], z01 = z[k + 1], z10 = z[k + ny], z11 = z[k + ny + 1];
+                    dzx = (z10 + z11 - z00 - z01) / 2. * inv;
+                    dzy = (z01 + z11 - z00 - z10) / 2. * inv;
+                }
+                gx[k] = (float)dzx;
+                gy[k] = (float)dzy;
+                cth[k] = (float)std::cos(std::atan(std::sqrt(dzx * dzx + dzy * dzy)));
+            }
+        }
+    };
+    std::vector<std::thread> ts;
+    int per = (nx + (int)nth - 1) / (int)nth;
+    for (unsigned t = 0; t < nth; ++t) {
+        int a = (int)t * per, b = std::min(nx, a + per);
+        if (a < b) ts.emplace_back(work, a, b);
+    }
+    for (auto& t : ts) t.join();
+}
+
+} // namespace
+
+int64_t engine_particle_bytes() {
+    // two state buffers (cell, ox, oy, vx, vy, vz, h, m, z, pid, slot = 44 B) + key, tslot, idx, tmp
+    return 2 * 44 + 16;
+}
+
+int64_t engine_slot_bytes(const Engine32* e) {
+    return (int64_t)e->nn * (16 + 8 + (e->single ? 12 : 0)) + VMAX_SHARDS * 8 + (int64_t)sizeof(Slot);
+}
+
+Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, int64_t max_particles, bool single,
+                        std::string* err) {
+    if (nx < 4 || ny < 4 || nx > 65535 || ny > 65535) {
+        if (err) *err = "FP32 engine: terrain must be between 4 and 65535 nodes per side";
+        return nullptr;
+    }
+    if (nslots < 1 || max_particles < 1 || max_particles > ((int64_t)1 << 30)) {
+        if (err) *err = "FP32 engine: need >= 1 slot and 1..2^30 particles of capacity";
+        return nullptr;
+    }
+    Engine32* e = new Engine32();
+    e->nslots = nslots;
+    e->single = single;
+    e->nx = nx;
+    e->ny = ny;
+    e->nn = (size_t)nx * ny;
+    e->dx = dx;
+    e->inv = 1. / dx;
+    e->ntx = (nx + TILE - 1) / TILE;
+    e->nty = (ny + TILE - 1) / TILE;
+    e->ntd = e->ntx * e->nty;
+    e->ntiles = nslots * (e->ntd + 1);
+    if ((double)e->ntiles * TILE_CELLS >= 2147483647.0) {
+        if (err) *err = "FP32 engine: slots x terrain tiles exceed the 31-bit key space";
+        delete e;
+        return nullptr;
+    }
+    e->nblk = (e->ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    e->cap = max_particles;
+    e->tile_cap = (int)std::min<int64_t>(e->ntiles, std::max<int64_t>(65536, max_particles / 32));
+    std::string why;
+    auto fail = [&](const std::string& m) -> Engine32* {
+        if (err) *err = m;
+        engine_destroy(e);
+        return nullptr;
+    };
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed");
+    if (hipHostMalloc((void**)&e->h_c, sizeof(ECtl), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&e->h_slots, sizeof(Slot) * nslots, hipHostMallocDefault) != hipSuccess)
+        return fail("hipHostMalloc failed");
+    std::memset(e->h_c, 0, sizeof(ECtl));
+    std::memset(e->h_slots, 0, sizeof(Slot) * nslots);
+    const size_t sn = (size_t)nslots * e->nn;
+    bool ok = ealloc(e->allocs, &e->d_c, 1, &why) && ealloc(e->allocs, &e->d_slots, nslots, &why) &&
+              ealloc(e->allocs, &e->tz, e->nn, &why) && ealloc(e->allocs, &e->tgx, e->nn, &why) &&
+              ealloc(e->allocs, &e->tgy, e->nn, &why) && ealloc(e->allocs, &e->tcth, e->nn, &why) &&
+              ealloc(e->allocs, &e->hmax, sn, &why) && ealloc(e->allocs, &e->vmax2, sn, &why) &&
+              ealloc(e->allocs, &e->vf, sn, &why) && ealloc(e->allocs, &e->vmx, (size_t)nslots * VMAX_SHARDS, &why) &&
+              ealloc(e->allocs, &e->tile_count, e->ntiles, &why) && ealloc(e->allocs, &e->tile_begin, e->ntiles, &why) &&
+              ealloc(e->allocs, &e->pstamp, e->ntiles, &why) && ealloc(e->allocs, &e->gstamp, e->ntiles, &why) &&
+              ealloc(e->allocs, &e->tile_pos, e->ntiles, &why) && ealloc(e->allocs, &e->ptiles, e->ntiles, &why) &&
+              ealloc(e->allocs, &e->gtiles, e->ntiles, &why) && ealloc(e->allocs, &e->bsum, 3 * (size_t)e->nblk, &why) &&
+              ealloc(e->allocs, &e->boff, 3 * (size_t)e->nblk, &why) &&
+              ealloc(e->allocs, &e->cell_begin, (size_t)e->tile_cap * CELL_STRIDE, &why) &&
+              ealloc(e->allocs, &e->tpart, (size_t)e->tile_cap * P2G_FIELDS * REGION_NODES, &why) &&
+              ealloc(e->allocs, &e->dpart, 6 * 256, &why);
+    if (ok && single)
+        ok = ealloc(e->allocs, &e->om, sn, &why) && ealloc(e->allocs, &e->oh, sn, &why) &&
+             ealloc(e->allocs, &e->ovz, sn, &why);
+    for (PB32* b : {&e->A, &e->B}) {
+        ok = ok && ealloc(e->pallocs, &b->cell, max_particles, &why);
+        for (float** q : {&b->ox, &b->oy, &b->vx, &b->vy, &b->vz, &b->h, &b->m, &b->z})
+            ok = ok && ealloc(e->pallocs, q, max_particles, &why);
+        ok = ok && ealloc(e->pallocs, &b->pid, max_particles, &why) && ealloc(e->pallocs, &b->slot, max_particles, &why);
+    }
+    ok = ok && ealloc(e->pallocs, &e->key, max_particles, &why) && ealloc(e->pallocs, &e->tslot, max_particles, &why) &&
+         ealloc(e->pallocs, &e->idx, max_particles, &why) && ealloc(e->pallocs, &e->tmp, max_particles, &why);
+    if (!ok) return fail(why);
+    std::vector<float> gx, gy, cth, zf(e->nn);
+    terrain32(z, nx, ny, e->inv, gx, gy, cth);
+    for (size_t k = 0; k < e->nn; ++k) zf[k] = (float)z[k];
+    hipStream_t s = e->stream;
+    ok = hipMemcpyAsync(e->tz, zf.data(), e->nn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(e->tgx, gx.data(), e->nn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(e->tgy, gy.data(), e->nn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(e->tcth, cth.data(), e->nn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemsetAsync(e->hmax, 0, sn * 4, s) == hipSuccess && hipMemsetAsync(e->vmax2, 0, sn * 4, s) == hipSuccess &&
+         hipMemsetAsync(e->vf, 0, sn * sizeof(float4), s) == hipSuccess &&
+         hipMemsetAsync(e->vmx, 0, (size_t)nslots * VMAX_SHARDS * 8, s) == hipSuccess &&
+         hipMemsetAsync(e->tile_count, 0, (size_t)e->ntiles * 4, s) == hipSuccess &&
+         hipMemsetAsync(e->pstamp, 0xff, (size_t)e->ntiles * 4, s) == hipSuccess &&
+         hipMemsetAsync(e->gstamp, 0xff, (size_t)e->ntiles * 4, s) == hipSuccess;
+    if (ok && single)
+        ok = hipMemsetAsync(e->om, 0, sn * 4, s) == hipSuccess && hipMemsetAsync(e->oh, 0, sn * 4, s) == hipSuccess &&
+             hipMemsetAsync(e->ovz, 0, sn * 4, s) == hipSuccess;
+    ok = ok && hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) return fail("terrain upload failed");
+    ECtl& c = *e->h_c;
+    c.nx = nx;
+    c.ny = ny;
+    c.ntx = e->ntx;
+    c.nty = e->nty;
+    c.ntd = e->ntd;
+    c.nslots = nslots;
+    c.single = single ? 1 : 0;
+    c.has_out = single ? 1 : 0;
+    c.tile_cap = e->tile_cap;
+    c.dx = dx;
+    c.inv_dx = e->inv;
+    e->admit_base.assign(nslots, -1);
+    if (!push_all(e, err)) {
+        engine_destroy(e);
+        return nullptr;
+    }
+    return e;
+}
+
+void engine_destroy(Engine32* e) {
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    for (auto& p : e->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto x : e->ev_free) (void)hipEventDestroy(x);
+    efree(e->pallocs);
+    efree(e->allocs);
+    if (e->h_c) (void)hipHostFree(e->h_c);
+    if (e->h_slots) (void)hipHostFree(e->h_slots);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+bool engine_pull(Engine32* e, std::string* err) {
+    if (!hip_ok(hipMemcpyAsync(e->h_c, e->d_c, sizeof(ECtl), hipMemcpyDeviceToHost, e->stream), "ctl download", err))
+        return false;
+    if (!hip_ok(hipMemcpyAsync(e->h_slots, e->d_slots, sizeof(Slot) * e->nslots, hipMemcpyDeviceToHost, e->stream),
+                "slot download", err))
+        return false;
+    return hip_ok(hipStreamSynchronize(e->stream), "sync", err);
+}
+
+const ECtl& engine_ctl(const Engine32* e) { return *e->h_c; }
+const Slot& engine_slot(const Engine32* e, int s) { return e->h_slots[s]; }
+
+int64_t engine_free_particles(const Engine32* e) { return e->cap - e->h_c->n_store; }
+
+int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, const double* y, const double* vx,
+                 const double* vy, const double* vz, const double* h, const double* A, const double* m,
+                 std::string* err) {
+    if (!engine_pull(e, err)) return -1;
+    int slot = -1;
+    for (int s = 0; s < e->nslots; ++s)
+        if (e->h_slots[s].status == SLOT_FREE) {
+            slot = s;
+            break;
+        }
+    if (slot < 0) {
+        if (err) *err = "no free slot";
+        return -1;
+    }
+    const int64_t base = e->h_c->n_store;
+    if (np < 1 || base + np > e->cap) {
+        if (err) *err = "particle capacity exhausted";
+        return -1;
+    }
+    const size_t n = (size_t)np;
+    std::vector<int32_t> cell(n), pid(n), sl(n, slot);
+    std::vector<float> ox(n), oy(n), fvx(n), fvy(n), fvz(n), fh(n), fm(n), zz(n, 0.f);
+    for (size_t p = 0; p < n; ++p) {
+        const double fx = x[p] * e->inv, fy = y[p] * e->inv;
+        double ci = std::floor(fx), cj = std::floor(fy);
+        if (!std::isfinite(ci)) ci = 0;
+        if (!std::isfinite(cj)) cj = 0;
+        ci = std::min(std::max(ci, 0.0), (double)(e->nx - 1));
+        cj = std::min(std::max(cj, 0.0), (double)(e->ny - 1));
+        float a = (float)(fx - ci), b = (float)(fy - cj);
+        int ii = (int)ci, jj = (int)cj;
+        if (a >= 1.f) {
+            a = 0.f;
+            ii = std::min(ii + 1, e->nx - 1);
+        }
+        if (b >= 1.f) {
+            b = 0.f;
+            jj = std::min(jj + 1, e->ny - 1);
+        }
+        if (!(a >= 0.f)) a = 0.f;
+        if (!(b >= 0.f)) b = 0.f;
+        cell[p] = (ii << 16) | jj;
+        ox[p] = a;
+        oy[p] = b;
+        fvx[p] = vx ? (float)vx[p] : 0.f;
+        fvy[p] = vy ? (float)vy[p] : 0.f;
+        fvz[p] = vz ? (float)vz[p] : 0.f;
+        fh[p] = (float)h[p];
+        fm[p] = (float)(m ? m[p] : prm.rho * h[p] * A[p]); // simulation.cpp:452
+        pid[p] = (int32_t)p;
+    }
+    const PB32& P = e->h_c->parity ? e->B : e->A;
+    hipStream_t s = e->stream;
+    bool ok = true;
+    auto up = [&](void* dst, const void* src) {
+        ok = ok && hipMemcpyAsync((char*)dst + base * 4, src, n * 4, hipMemcpyHostToDevice, s) == hipSuccess;
+    };
+    up(P.cell, cell.data());
+    up(P.ox, ox.data());
+    up(P.oy, oy.data());
+    up(P.vx, fvx.data());
+    up(P.vy, fvy.data());
+    up(P.vz, fvz.data());
+    up(P.h, fh.data());
+    up(P.m, fm.data());
+    up(P.z, zz.data());
+    up(P.pid, pid.data());
+    up(P.slot, sl.data());
+    // per-slot grid state starts empty
+    ok = ok && hipMemsetAsync(e->hmax + (size_t)slot * e->nn, 0, e->nn * 4, s) == hipSuccess &&
+         hipMemsetAsync(e->vmax2 + (size_t)slot * e->nn, 0, e->nn * 4, s) == hipSuccess;
+    if (!ok || !hip_ok(hipStreamSynchronize(s), "admit upload", err)) {
+        if (err && err->empty()) *err = "admit upload failed";
+        return -1;
+    }
+    Slot S = prm;
+    S.status = SLOT_PRIMING;
+    S.time = 0.0;
+    S.dt = 0.0;
+    S.frame = 0;
+    S.step = 0;
+    S.frame_hit = 0;
+    S.np = (int32_t)np;
+    S.err = 0;
+    e->h_slots[slot] = S;
+    e->h_c->n_store = (int32_t)(base + np);
+    if (e->h_c->stop == STOP_DONE) e->h_c->stop = RUNNING;
+    if (!push_all(e, err)) return -1;
+    const unsigned nb = (unsigned)((n + WG - 1) / WG);
+    k_ez<<<nb, WG, 0, s>>>(e->d_c, e->A, e->B, (int)base, (int)(base + np), e->tz);
+    k_ekeys<<<nb, WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, (int)base, (int)(base + np), e->key, e->tslot,
+                              e->tile_count);
+    if (!hip_ok(hipGetLastError(), "admit kernels", err) || !hip_ok(hipStreamSynchronize(s), "admit", err)) return -1;
+    e->admit_base[slot] = base;
+    return slot;
+}
+
+bool engine_step(Engine32* e, int max_steps, std::string* err) {
+    hipStream_t s = e->stream;
+    k_eclear<<<1, WG, 0, s>>>(e->d_c, e->d_slots);
+    if (e->use_graph && !e->timing && !e->gexec && !build_graph(e, err)) return false;
+    const int chunk_max = 16;
+    int done = 0;
+    while (done < max_steps) {
+        const int chunk = std::min(chunk_max, max_steps - done);
+        for (int k = 0; k < chunk; ++k) {
+            if (e->use_graph && !e->timing) {
+                if (!hip_ok(hipGraphLaunch(e->gexec, s), "graph launch", err)) return false;
+            } else {
+                enqueue_estep(e, done + k);
+            }
+        }
+        if (!hip_ok(hipGetLastError(), "step launch", err)) return false;
+        if (!engine_pull(e, err)) return false;
+        if (e->timing) collect(e, e->h_c->steps_in_call);
+        done += chunk;
+        if (e->h_c->stop != RUNNING) break;
+    }
+    for (int q = 0; q < e->nslots; ++q) e->admit_base[q] = -1; // storage was re-sorted
+    if (e->h_c->stop == STOP_ERROR) {
+        if (err) *err = e->h_c->err == 7 ? "active-tile capacity exceeded" : "device error";
+        return false;
+    }
+    return true;
+}
+
+bool engine_release(Engine32* e, int slot, std::string* err) {
+    if (!engine_pull(e, err)) return false;
+    if (slot < 0 || slot >= e->nslots) {
+        if (err) *err = "bad slot";
+        return false;
+    }
+    e->h_slots[slot].status = SLOT_FREE;
+    e->h_slots[slot].np = 0;
+    if (!push_all(e, err)) return false;
+    // Re-key every stored particle with the new statuses (the released slot's
+    // particles get no key), then run one re-sort pass that only permutes:
+    // storage is compact again and no slot advances.
+    hipStream_t s = e->stream;
+    const int n = e->h_c->n_store;
+    if (n > 0) {
+        if (!hip_ok(hipMemsetAsync(e->tile_count, 0, (size_t)e->ntiles * 4, s), "release", err)) return false;
+        k_ekeys<<<(unsigned)((n + WG - 1) / WG), WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, 0, n, e->key, e->tslot,
+                                                             e->tile_count);
+        e->h_c->compact = 1;
+        e->h_c->stop = RUNNING;
+        if (!hip_ok(hipMemcpyAsync(e->d_c, e->h_c, sizeof(ECtl), hipMemcpyHostToDevice, s), "release", err))
+            return false;
+        const bool tm = e->timing;
+        e->timing = false;
+        enqueue_estep(e, 0);
+        e->timing = tm;
+        if (!hip_ok(hipGetLastError(), "release pass", err)) return false;
+    }
+    return engine_pull(e, err);
+}
+
+bool engine_particles(Engine32* e, int slot, double* const dst[9], std::string* err) {
+    if (!engine_pull(e, err)) return false;
+    const int n = e->h_slots[slot].np;
+    if (n <= 0) return true;
+    std::vector<void*> tl;
+    int* cnt = nullptr;
+    int32_t *oc = nullptr, *op = nullptr;
+    float* of = nullptr;
+    if (!ealloc(tl, &cnt, 1, err) || !ealloc(tl, &oc, n, err) || !ealloc(tl, &op, n, err) ||
+        !ealloc(tl, &of, (size_t)n * 8, err)) {
+        efree(tl);
+        return false;
+    }
+    hipStream_t s = e->stream;
+    bool ok = hipMemsetAsync(cnt, 0, 4, s) == hipSuccess;
+    const unsigned nb = (unsigned)std::max(1, (e->h_c->n_store + WG - 1) / WG);
+    k_ecollect<<<nb, WG, 0, s>>>(e->d_c, e->A, e->B, slot, cnt, oc, of, op, n);
+    std::vector<int32_t> c(n), p(n);
+    std::vector<float> f((size_t)n * 8);
+    int got = 0;
+    ok = ok && hipMemcpyAsync(&got, cnt, 4, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipMemcpyAsync(c.data(), oc, n * 4, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipMemcpyAsync(p.data(), op, n * 4, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipMemcpyAsync(f.data(), of, (size_t)n * 32, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipStreamSynchronize(s) == hipSuccess;
+    efree(tl);
+    if (!ok || got != n) {
+        if (err) *err = ok ? "particle count mismatch while collecting a slot" : "collect failed";
+        return false;
+    }
+    const double rho = e->h_slots[slot].rho;
+    for (int k = 0; k < n; ++k) {
+        const int q = p[k];
+        if (q < 0 || q >= n) continue;
+        const float* g = &f[(size_t)k * 8];
+        if (dst[0]) dst[0][q] = ((double)(c[k] >> 16) + (double)g[0]) * e->dx;
+        if (dst[1]) dst[1][q] = ((double)(c[k] & 0xffff) + (double)g[1]) * e->dx;
+        if (dst[2]) dst[2][q] = g[2];
+        if (dst[3]) dst[3][q] = g[3];
+        if (dst[4]) dst[4][q] = g[4];
+        if (dst[5]) dst[5][q] = g[5];
+        if (dst[6]) dst[6][q] = g[6];
+        if (dst[7]) dst[7][q] = (double)g[7] / ((double)g[6] * rho);
+        if (dst[8]) dst[8][q] = g[7];
+    }
+    return true;
+}
+
+bool engine_max_maps(Engine32* e, int slot, double* h_max, double* v_max2, std::string* err) {
+    std::vector<float> f(e->nn);
+    double* dst[] = {h_max, v_max2};
+    float* src[] = {e->hmax + (size_t)slot * e->nn, e->vmax2 + (size_t)slot * e->nn};
+    for (int k = 0; k < 2; ++k) {
+        if (!dst[k]) continue;
+        if (!hip_ok(hipMemcpyAsync(f.data(), src[k], e->nn * 4, hipMemcpyDeviceToHost, e->stream), "max maps", err) ||
+            !hip_ok(hipStreamSynchronize(e->stream), "sync", err))
+            return false;
+        for (size_t q = 0; q < e->nn; ++q) dst[k][q] = f[q];
+    }
+    return true;
+}
+
+bool engine_window(Engine32* e, int slot, int* i0, int* j0, int* wnx, int* wny, double* const dst[5],
+                   std::string* err) {
+    if (!engine_pull(e, err)) return false;
+    const Slot& S = e->h_slots[slot];
+    std::vector<void*> tl;
+    unsigned long long* bb = nullptr;
+    if (!ealloc(tl, &bb, 4, err)) return false;
+    unsigned long long init[4] = {~0ull, 0ull, ~0ull, 0ull};
+    hipStream_t s = e->stream;
+    const int n = e->h_c->n_store;
+    bool ok = hipMemcpyAsync(bb, init, sizeof(init), hipMemcpyHostToDevice, s) == hipSuccess;
+    k_ebbox<<<(unsigned)std::max(1, (n + WG - 1) / WG), WG, 0, s>>>(e->d_c, e->A, e->B, slot, n, bb);
+    unsigned long long hb[4];
+    ok = ok && hipMemcpyAsync(hb, bb, sizeof(hb), hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipStreamSynchronize(s) == hipSuccess;
+    efree(tl);
+    if (!ok) {
+        if (err) *err = "bbox failed";
+        return false;
+    }
+    // remesh window, src/simulation.cpp:594-606
+    const double eps = 1e-8 * e->dx;
+    auto cl = [](double v) { return (int)std::max(-1e9, std::min(1e9, v)); };
+    int iL = std::max(0, cl(ord_value(hb[0]) * e->inv - 1.0 - eps));
+    int iR = std::min(e->nx - 1, cl(ord_value(hb[1]) * e->inv + 2.0 + eps));
+    int jB = std::max(0, cl(ord_value(hb[2]) * e->inv - 1.0 - eps));
+    int jT = std::min(e->ny - 1, cl(ord_value(hb[3]) * e->inv + 2.0 + eps));
+    if (iR < iL || jT < jB) {
+        if (err) *err = "particle window outside the terrain";
+        return false;
+    }
+    *i0 = iL;
+    *j0 = jB;
+    *wnx = iR - iL + 1;
+    *wny = jT - jB + 1;
+    bool any = false;
+    for (int k = 0; k < 5; ++k) any = any || dst[k];
+    if (!any) return true;
+    if (S.last_out_epoch != e->h_c->epoch) {
+        if (err) *err = "last step kept no grid outputs";
+        return false;
+    }
+    const size_t wn = (size_t)(*wnx) * (*wny);
+    double* o = nullptr;
+    if (!ealloc(tl, &o, 5 * wn, err)) return false;
+    k_ewindow<<<(unsigned)((wn + WG - 1) / WG), WG, 0, s>>>(e->d_c, slot, iL, jB, *wnx, *wny, e->gstamp, e->vf, e->om,
+                                                            e->oh, e->ovz, o);
+    ok = hipStreamSynchronize(s) == hipSuccess;
+    for (int k = 0; k < 5 && ok; ++k)
+        if (dst[k]) ok = hipMemcpy(dst[k], o + k * wn, wn * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    efree(tl);
+    if (!ok && err) *err = "window download failed";
+    return ok;
+}
+
+bool engine_diag(Engine32* e, int slot, double out[8], std::string* err) {
+    if (!engine_pull(e, err)) return false;
+    const Slot& S = e->h_slots[slot];
+    hipStream_t s = e->stream;
+    const bool gridok = e->om && S.last_out_epoch == e->h_c->epoch;
+    k_ediag<<<256, WG, 0, s>>>(e->d_c, e->A, e->B, slot, e->gtiles, e->vf, gridok ? e->om : nullptr, S.rho, e->dpart);
+    std::vector<double> h(6 * 256);
+    if (!hip_ok(hipMemcpyAsync(h.data(), e->dpart, h.size() * 8, hipMemcpyDeviceToHost, s), "diag", err) ||
+        !hip_ok(hipStreamSynchronize(s), "sync", err))
+        return false;
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < 256; ++b)
+        for (int q = 0; q < 6; ++q) a[q] += h[6 * b + q];
+    const double nan = std::nan("");
+    out[0] = gridok ? a[0] : nan;
+    out[1] = gridok ? a[1] : nan;
+    out[2] = gridok ? a[2] : nan;
+    out[3] = a[4];
+    out[4] = a[5];
+    out[5] = S.dt;
+    out[6] = nan; // max |v_p| is reduced on the device per step; not kept
+    out[7] = gridok ? a[3] : nan;
+    return true;
+}
+
+bool engine_set_velocity(Engine32* e, int slot, const double* vx, const double* vy, std::string* err) {
+    if (!engine_pull(e, err)) return false;
+    const int64_t base = e->admit_base[slot];
+    const Slot& S = e->h_slots[slot];
+    if (base < 0 || S.status != SLOT_PRIMING) {
+        if (err) *err = "set the velocity before the first step";
+        return false;
+    }
+    const size_t n = (size_t)S.np;
+    std::vector<float> a(n), b(n);
+    for (size_t k = 0; k < n; ++k) {
+        a[k] = (float)vx[k];
+        b[k] = (float)vy[k];
+    }
+    const PB32& P = e->h_c->parity ? e->B : e->A;
+    return hip_ok(hipMemcpy(P.vx + base, a.data(), n * 4, hipMemcpyHostToDevice), "velocity", err) &&
+           hip_ok(hipMemcpy(P.vy + base, b.data(), n * 4, hipMemcpyHostToDevice), "velocity", err);
+}
+
+bool engine_set_time(Engine32* e, int slot, double time, int frame, std::string* err) {
+    if (!engine_pull(e, err)) return false;
+    e->h_slots[slot].time = time;
+    e->h_slots[slot].frame = frame;
+    return push_all(e, err);
+}
+
+bool engine_keep_grid(Engine32* e, int slot, int on, std::string* err) {
+    if (!engine_pull(e, err)) return false;
+    e->h_slots[slot].keep_grid = on ? 1 : 0;
+    return push_all(e, err);
+}
+
+void engine_timing(Engine32* e, int enable, double* ms, int64_t* counts, int n) {
+    for (int k = 0; k < n && k < kNumK; ++k) {
+        if (ms) ms[k] = e->ms[k];
+        if (counts) counts[k] = e->cnt[k];
+    }
+    if (enable && !e->timing) {
+        std::fill(e->ms, e->ms + kNumK, 0.0);
+        std::fill(e->cnt, e->cnt + kNumK, 0);
+    }
+    e->timing = enable != 0;
+}
+
+int engine_use_graph(Engine32* e, int on) {
+    e->use_graph = on != 0;
+    if (!on) drop_graph(e);
+    return 0;
+}
+
+} // namespace slab

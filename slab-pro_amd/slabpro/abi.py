@@ -45,6 +45,15 @@ class Terrain(ctypes.Structure):
                 ("z", ctypes.POINTER(ctypes.c_double))]
 
 
+class SlotInfo(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double), ("status", ctypes.c_int32),
+                ("frame", ctypes.c_int32), ("step", ctypes.c_int32), ("np", ctypes.c_int32),
+                ("frame_hit", ctypes.c_int32), ("err", ctypes.c_int32)]
+
+
+SLOT_FREE, SLOT_PRIMING, SLOT_RUNNING, SLOT_DONE, SLOT_FAILED = range(5)
+
+
 class StepInfo(ctypes.Structure):
     _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double), ("frame", ctypes.c_int32),
                 ("step", ctypes.c_int32), ("frame_hit", ctypes.c_int32), ("finished", ctypes.c_int32),
@@ -74,6 +83,20 @@ SIGNATURES = {
     "slab_last_error": (ctypes.c_char_p, []),
     "slab_run_destroy": (None, [_VP]),
     "slab_device_info": (ctypes.c_int, [_IP]),
+    "slab_batch_create": (ctypes.c_int, [ctypes.POINTER(Terrain), ctypes.c_int32, ctypes.c_int64,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "slab_batch_admit": (ctypes.c_int, [_VP, ctypes.POINTER(Params), ctypes.POINTER(Friction), ctypes.c_int64]
+                         + [_DP] * 7 + [_IP]),
+    "slab_batch_step": (ctypes.c_int, [_VP, ctypes.c_int32, _IP]),
+    "slab_batch_slot_info": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(SlotInfo)]),
+    "slab_batch_statuses": (ctypes.c_int, [_VP, _IP, _DP]),
+    "slab_batch_download_max": (ctypes.c_int, [_VP, ctypes.c_int32, _DP, _DP]),
+    "slab_batch_download_particles": (ctypes.c_int, [_VP, ctypes.c_int32] + [_DP] * 9),
+    "slab_batch_release": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "slab_batch_footprint": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int64)]),
+    "slab_batch_use_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "slab_batch_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _DP, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
+    "slab_batch_destroy": (None, [_VP]),
 }
 
 _lib = None
@@ -234,6 +257,92 @@ class Run:
 
     def set_time(self, t, frame):
         _check(lib().slab_run_set_time(self.h, float(t), int(frame)))
+
+
+class Batch:
+    """Many scenarios on one shared DEM (FP32 form), advanced together."""
+
+    def __init__(self, z, dx, max_slots, max_particles):
+        z = _f64(z)
+        self.nx, self.ny = z.shape
+        self.dx = float(dx)
+        self.max_slots = int(max_slots)
+        t = Terrain(self.nx, self.ny, self.dx, _p(z))
+        h = ctypes.c_void_p()
+        _check(lib().slab_batch_create(ctypes.byref(t), self.max_slots, int(max_particles), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().slab_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, params, friction, x, y, h, A, m=None, vx=None, vy=None):
+        """Returns the slot id, or None when no slot / particle capacity is free."""
+        x, y, h, A, m, vx, vy = map(_f64, (x, y, h, A, m, vx, vy))
+        slot = ctypes.c_int32(-1)
+        st = lib().slab_batch_admit(self.h, ctypes.byref(params), ctypes.byref(friction), len(x), _p(x), _p(y),
+                                    _p(vx), _p(vy), _p(h), _p(A), _p(m), ctypes.byref(slot))
+        if st == ENOMEM:
+            return None
+        _check(st)
+        return slot.value
+
+    def step(self, max_steps=1):
+        n = ctypes.c_int32(0)
+        _check(lib().slab_batch_step(self.h, int(max_steps), ctypes.byref(n)))
+        return n.value
+
+    def statuses(self):
+        st = np.zeros(self.max_slots, dtype=np.int32)
+        tm = np.zeros(self.max_slots)
+        _check(lib().slab_batch_statuses(self.h, st.ctypes.data_as(_IP), _p(tm)))
+        return st, tm
+
+    def slot_info(self, slot):
+        i = SlotInfo()
+        _check(lib().slab_batch_slot_info(self.h, int(slot), ctypes.byref(i)))
+        return {"time": i.time, "dt": i.dt, "status": i.status, "frame": i.frame, "step": i.step, "np": i.np,
+                "frame_hit": bool(i.frame_hit), "err": i.err}
+
+    def max_maps(self, slot):
+        hm = np.zeros((self.nx, self.ny))
+        vm = np.zeros((self.nx, self.ny))
+        _check(lib().slab_batch_download_max(self.h, int(slot), _p(hm), _p(vm)))
+        return hm, vm
+
+    def particles(self, slot):
+        n = self.slot_info(slot)["np"]
+        names = ["x", "y", "vx", "vy", "vz", "z", "h", "A", "m"]
+        arrs = {k: np.zeros(n) for k in names}
+        _check(lib().slab_batch_download_particles(self.h, int(slot), *[_p(arrs[k]) for k in names]))
+        return arrs
+
+    def release(self, slot):
+        _check(lib().slab_batch_release(self.h, int(slot)))
+
+    def footprint(self):
+        out = (ctypes.c_int64 * 4)()
+        _check(lib().slab_batch_footprint(self.h, out))
+        return {"bytes_per_particle": out[0], "bytes_per_slot": out[1], "free_particles": out[2],
+                "free_slots": out[3]}
+
+    def use_graph(self, on=True):
+        _check(lib().slab_batch_use_graph(self.h, 1 if on else 0))
+
+    def timing(self, enable):
+        n = lib().slab_kernel_count()
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
+        _check(lib().slab_batch_timing(self.h, 1 if enable else 0, _p(ms),
+                                       cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), n))
+        return {name: (ms[k], int(cnt[k])) for k, name in enumerate(kernel_names())}
 
 
 def params(gravity=9.81, rho=300.0, C_cfl=0.5, dt_max=0.01, flip_ratio=0.95, output_dt=1.0, total_time=60.0,

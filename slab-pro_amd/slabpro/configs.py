@@ -306,3 +306,79 @@ def tiny(nx=48, ny=40, dx=2.0, theta_deg=30.0, fric=None, seed=7, jitter=True, r
     fr = fric or {"type": 0, "mu": 0.3}
     return {"name": "tiny", "z": np.ascontiguousarray(z), "dx": dx,
             "params": _params(rho=rho, total_time=5.0, output_dt=0.5), "friction": fr, **part}
+
+
+# ---------------------------------------------------------------- ensemble (c3) / requests (c4)
+
+def _scenario_release(g, dx, n_target, x_band=(600.0, 3000.0), y_band=(1500.0, 8500.0)):
+    """1-10 ellipses clustered around a random centre, semi-axes scaled so the
+    UNIFORM sample count is within 5% of n_target.  Overlaps take the larger
+    height (as a raster release would); each particle covers dx^2/4."""
+    k = int(g.integers(1, 11))
+    cx, cy = g.uniform(*x_band), g.uniform(*y_band)
+    shapes = []
+    for _ in range(k):
+        ex, ey = cx + g.uniform(-300, 300), cy + g.uniform(-300, 300)
+        rx, ry = g.uniform(0.5, 1.5), g.uniform(0.5, 1.5)
+        shapes.append((ex, ey, rx, ry, g.uniform(0.0, math.pi), g.uniform(0.5, 3.0)))
+    unit_area = sum(math.pi * s[2] * s[3] for s in shapes)
+    scale = math.sqrt(n_target * dx * dx / 4.0 / unit_area)
+    for _ in range(8):
+        xs, ys, hs = _ellipse_union(dx, shapes, scale)
+        if len(xs) == 0:
+            scale *= 1.5
+            continue
+        ratio = len(xs) / n_target
+        if abs(ratio - 1.0) <= 0.05:
+            break
+        scale /= math.sqrt(ratio)
+    return xs, ys, hs
+
+
+def _ellipse_union(dx, shapes, scale):
+    xmin = min(s[0] - max(s[2], s[3]) * scale for s in shapes)
+    xmax = max(s[0] + max(s[2], s[3]) * scale for s in shapes)
+    ymin = min(s[1] - max(s[2], s[3]) * scale for s in shapes)
+    ymax = max(s[1] + max(s[2], s[3]) * scale for s in shapes)
+    px, py = uniform_samples(dx, xmin, xmax, ymin, ymax)
+    h = np.zeros(len(px))
+    for (ex, ey, rx, ry, rot, hgt) in shapes:
+        inside = ellipse_keep(px, py, ex, ey, rx * scale, ry * scale, rot)
+        np.maximum(h, np.where(inside, hgt, 0.0), out=h)
+    keep = h > 0
+    return px[keep], py[keep], h[keep]
+
+
+def c3_scenario(i, dem_n=2000, dx=5.0):
+    """Scenario i of the 256-scenario ensemble on the c1 DEM: log-uniform particle
+    count in [2k, 1M], friction mix 50% Voellmy / 30% Coulomb / 20% mu(I)."""
+    g = rng(3, i)
+    n_target = int(round(math.exp(g.uniform(math.log(2e3), math.log(1e6)))))
+    u = g.uniform()
+    if u < 0.5:
+        fr = {"type": 1, "mu": float(g.uniform(0.1, 0.3)), "xi": float(g.uniform(500.0, 4000.0))}
+        rho = 300.0
+    elif u < 0.8:
+        fr = {"type": 0, "mu": float(g.uniform(0.3, 0.6))}
+        rho = 300.0
+    else:
+        fr = {"type": 2, "mu1": 0.38, "mu2": 0.64, "I0": 0.279, "d": 0.5, "phi": 0.6}
+        rho = 2000.0
+    px, py, h = _scenario_release(g, dx, n_target)
+    A = np.full(len(px), dx * dx / 4.0)
+    part = {"x": px, "y": py, "h": h, "A": A, "m": rho * h * A}
+    return {"name": f"c3_{i}", "dx": dx, "params": _params(rho=rho, total_time=120.0), "friction": fr,
+            "n_target": n_target, **part}
+
+
+def c1_dem():
+    return synthetic_dem(2000, 5.0, valley_y=5000.0, seed=subseed(1, 0))
+
+
+def c4_requests(n=2000, lam=5.0, seed_index=0):
+    """Open-loop Poisson arrival times (s) of n requests at rate lam, and the
+    scenario index each draws from the c3 distribution (request k uses c3
+    scenario generator stream 1000 + k)."""
+    g = rng(4, seed_index)
+    gaps = g.exponential(1.0 / lam, n)
+    return np.cumsum(gaps), [1000 + k for k in range(n)]

@@ -1,0 +1,176 @@
+"""Batched FP32 engine (many scenarios on one DEM) -- needs an MI355X.
+
+The engine orders particles by (slot, tile, cell, creation index) and every
+per-slot quantity (dt, friction, grid, max maps) depends only on that slot's
+own particles, so a scenario's results are bitwise the same alone and in any
+mix (tolerance: exact).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenario(seed, fric, n_extra=0):
+    from slabpro import configs
+    g = np.random.default_rng(seed)
+    dx = 2.0
+    cx = 60.0 + 20.0 * g.uniform()
+    cy = 60.0 + 20.0 * g.uniform()
+    px, py, area = configs.ellipse_release(dx, cx, cy, 14.0 + n_extra, 10.0 + g.uniform() * 4, g.uniform(0, 180))
+    h = np.full(len(px), 1.0 + g.uniform())
+    part = configs.finish_particles(px, py, h, area * h[0], 300.0)
+    return part, fric
+
+
+def _terrain():
+    nx, ny, dx = 96, 80, 2.0
+    x = np.arange(nx) * dx
+    y = np.arange(ny) * dx
+    z = 500.0 - 0.45 * x[:, None] + 0.002 * (y[None, :] - 80.0) ** 2
+    return np.ascontiguousarray(z), dx
+
+
+FRICS = [{"type": 0, "mu": 0.3}, {"type": 1, "mu": 0.155, "xi": 2000.0},
+         {"type": 2, "mu1": 0.38, "mu2": 0.64, "I0": 0.279, "d": 0.5, "phi": 0.6}]
+
+
+def _params(total=3.0):
+    from slabpro import abi
+    return abi.params(rho=300.0, total_time=total, output_dt=0.5, precision=abi.FP32)
+
+
+def _run_batch(scens, max_slots=8, cap=200000, steps=100000):
+    from slabpro import abi
+    z, dx = _terrain()
+    b = abi.Batch(z, dx, max_slots, cap)
+    slots = []
+    for part, fr in scens:
+        s = b.admit(_params(), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
+        assert s is not None
+        slots.append(s)
+    n = 0
+    while b.step(64) > 0:
+        n += 1
+        assert n < steps
+    out = [(b.max_maps(s), b.particles(s), b.slot_info(s)) for s in slots]
+    b.close()
+    return out
+
+
+def test_alone_and_in_mix_are_bitwise_equal():
+    target = _scenario(11, FRICS[1])
+    others = [_scenario(20 + k, FRICS[k % 3], n_extra=3 * k) for k in range(5)]
+    (alone,) = _run_batch([target])
+    mix = _run_batch(others[:2] + [target] + others[2:])
+    (hm_a, vm_a), p_a, i_a = alone
+    (hm_m, vm_m), p_m, i_m = mix[2]
+    assert i_a["status"] == i_m["status"] and i_a["step"] == i_m["step"] and i_a["time"] == i_m["time"]
+    assert np.array_equal(hm_a, hm_m) and np.array_equal(vm_a, vm_m)
+    for k in p_a:
+        assert np.array_equal(p_a[k], p_m[k]), k
+
+
+def test_batch_slot_equals_single_fp32_run():
+    import slabpro
+    from slabpro import abi
+    part, fr = _scenario(5, FRICS[2])
+    (res,) = _run_batch([(part, fr)])
+    z, dx = _terrain()
+    cfg = {"z": z, "dx": dx, "params": dict(rho=300.0, total_time=3.0, output_dt=0.5), "friction": fr, **part}
+    run = slabpro.make_run(cfg, precision=abi.FP32)
+    run.prime()
+    run.run()
+    hm, vm = run.max_maps()
+    p = run.particles()
+    assert np.array_equal(hm, res[0][0]) and np.array_equal(vm, res[0][1])
+    for k in ("x", "y", "vx", "vy", "h"):
+        assert np.array_equal(p[k], res[1][k]), k
+
+
+def test_non_finite_request_rejected_and_device_failure_isolated():
+    from slabpro import abi
+    z, dx = _terrain()
+    good = [_scenario(31, FRICS[0]), _scenario(32, FRICS[1])]
+    bad_part, bad_fr = _scenario(33, FRICS[2])
+    # a NaN input is refused at admission; nothing is admitted
+    b = abi.Batch(z, dx, 4, 100000)
+    h = bad_part["h"].copy()
+    h[7] = np.nan
+    with pytest.raises(abi.SlabError) as ei:
+        b.admit(_params(), abi.friction(**bad_fr), bad_part["x"], bad_part["y"], h, bad_part["A"])
+    assert ei.value.status == abi.EINVAL
+    assert b.footprint()["free_slots"] == 4
+    b.close()
+    # a finite input that overflows on the device fails only its own slot
+    ref = _run_batch(good)
+    b = abi.Batch(z, dx, 8, 200000)
+    s0 = b.admit(_params(), abi.friction(**good[0][1]), *[good[0][0][k] for k in ("x", "y", "h", "A", "m")])
+    vx = np.zeros(len(bad_part["x"]))
+    vx[7] = 1e30  # |v|^2 overflows FP32 on the device
+    s1 = b.admit(_params(), abi.friction(**bad_fr), bad_part["x"], bad_part["y"], bad_part["h"], bad_part["A"],
+                 vx=vx)
+    s2 = b.admit(_params(), abi.friction(**good[1][1]), *[good[1][0][k] for k in ("x", "y", "h", "A", "m")])
+    while b.step(64) > 0:
+        pass
+    assert b.slot_info(s1)["status"] == abi.SLOT_FAILED
+    for r, s in ((ref[0], s0), (ref[1], s2)):
+        assert b.slot_info(s)["status"] == abi.SLOT_DONE
+        assert np.array_equal(r[0][0], b.max_maps(s)[0])
+        assert np.array_equal(r[1]["x"], b.particles(s)["x"])
+    b.close()
+
+
+def test_admission_capacity_release_and_reuse():
+    from slabpro import abi
+    z, dx = _terrain()
+    parts = [_scenario(40 + k, FRICS[k % 3]) for k in range(4)]
+    npart = [len(p["x"]) for p, _ in parts]
+    cap = npart[0] + npart[1] + 10
+    b = abi.Batch(z, dx, 3, cap)
+    fp = b.footprint()
+    assert fp["free_slots"] == 3 and fp["free_particles"] == cap and fp["bytes_per_particle"] > 0
+    s0 = b.admit(_params(1.0), abi.friction(**parts[0][1]), **{k: parts[0][0][k] for k in ("x", "y", "h", "A", "m")})
+    s1 = b.admit(_params(1.0), abi.friction(**parts[1][1]), **{k: parts[1][0][k] for k in ("x", "y", "h", "A", "m")})
+    assert (s0, s1) == (0, 1)
+    # particle capacity exhausted -> refused, nothing changes
+    assert b.admit(_params(1.0), abi.friction(**parts[2][1]),
+                   **{k: parts[2][0][k] for k in ("x", "y", "h", "A", "m")}) is None
+    while b.step(64) > 0:
+        pass
+    st, tm = b.statuses()
+    assert st[0] == abi.SLOT_DONE and st[1] == abi.SLOT_DONE and tm[0] >= 1.0
+    p0 = b.particles(0)
+    assert len(p0["x"]) == npart[0] and np.all(np.isfinite(p0["x"]))
+    b.release(0)
+    b.release(1)
+    b.step(1)  # released particles drop out of the state
+    assert b.footprint()["free_particles"] == cap
+    s2 = b.admit(_params(1.0), abi.friction(**parts[2][1]), **{k: parts[2][0][k] for k in ("x", "y", "h", "A", "m")})
+    assert s2 == 0
+    while b.step(64) > 0:
+        pass
+    assert b.slot_info(s2)["status"] == abi.SLOT_DONE
+    b.close()
+
+
+def test_graph_replay_matches_eager():
+    from slabpro import abi
+    z, dx = _terrain()
+    scens = [_scenario(50 + k, FRICS[k % 3]) for k in range(3)]
+    outs = []
+    for graph in (False, True):
+        b = abi.Batch(z, dx, 4, 100000)
+        b.use_graph(graph)
+        slots = [b.admit(_params(2.0), abi.friction(**fr), p["x"], p["y"], p["h"], p["A"], p["m"]) for p, fr in scens]
+        for _ in range(5):  # several replays with host work in between
+            b.step(17)
+            b.statuses()
+        while b.step(64) > 0:
+            pass
+        outs.append([(b.max_maps(s)[0], b.particles(s)["x"]) for s in slots])
+        b.close()
+    for (ha, xa), (hb, xb) in zip(*outs):
+        assert np.array_equal(ha, hb) and np.array_equal(xa, xb)

@@ -85,6 +85,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", default="fp64", choices=["fp32", "fp64"])
+    ap.add_argument("--ensemble", action="store_true",
+                    help="256-scenario ensemble: FP32 batched vs FP64 one-at-a-time, scenarios/h")
+    ap.add_argument("--scenarios", type=int, default=256)
+    ap.add_argument("--fp64-sample", type=int, default=16)
+    ap.add_argument("--serve", action="store_true", help="continuous-batching server, p50/p95/p99 latency")
+    ap.add_argument("--rates", default="2,5,10")
+    ap.add_argument("--requests", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build's step runs on a single MI355X "
@@ -94,6 +101,11 @@ def main():
     import numpy as np
     import slabpro
     from slabpro import abi, configs
+
+    if args.ensemble:
+        return ensemble_main(args)
+    if args.serve:
+        return serve_main(args)
 
     t0 = time.time()
     cfg = getattr(configs, args.workload)()
@@ -179,6 +191,71 @@ def main():
         "cpu_baseline": cpu,
     }
     run.close()
+    print(json.dumps(out), flush=True)
+
+
+def ensemble_main(args):
+    """C3: 256 scenarios on the 2000x2000 DEM at 5 m; FP32 batched (all) vs the
+    plain FP64 one-at-a-time GPU path (a size-stratified sample, stated)."""
+    import numpy as np
+    from slabpro import abi, configs, ensemble
+    t0 = time.time()
+    dem = configs.c1_dem()
+    scen = [configs.c3_scenario(i) for i in range(args.scenarios)]
+    sizes = np.array([len(s["x"]) for s in scen])
+    log(f"ensemble: {len(scen)} scenarios, {sizes.sum()} particles (min {sizes.min()}, max {sizes.max()}), "
+        f"generated in {time.time() - t0:.1f}s")
+    fr = [s["friction"]["type"] for s in scen]
+    bat = ensemble.run_batched(dem, 5.0, scen, max_slots=len(scen), progress=log)
+    log(f"fp32 batched: {bat['wall_s']:.1f}s, {bat['scenarios_per_hour']:.0f} scen/h, "
+        f"{bat['particle_updates_per_s']:.3g} pu/s, failed {bat['failed']}")
+    k = max(1, min(args.fp64_sample, len(scen)))
+    rank = np.argsort(sizes, kind="stable")
+    sample = [int(rank[int((q + 0.5) * len(scen) / k)]) for q in range(k)]
+    one64 = ensemble.run_one_at_a_time(dem, 5.0, [scen[i] for i in sample], abi.FP64, progress=log)
+    log(f"fp64 one-at-a-time on {k} scenarios: {one64['wall_s']:.1f}s, {one64['scenarios_per_hour']:.0f} scen/h")
+    one32 = ensemble.run_one_at_a_time(dem, 5.0, [scen[i] for i in sample], abi.FP32, progress=log)
+    log(f"fp32 one-at-a-time on the same {k}: {one32['wall_s']:.1f}s, {one32['scenarios_per_hour']:.0f} scen/h")
+    out = {
+        "metric": "scenarios/h on the 256-scenario ensemble",
+        "value": bat["scenarios_per_hour"], "unit": "scenarios/h", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic (seeded DEM + ellipse releases)",
+        "config": {"workload": f"c3: {len(scen)} scenarios on the 2000x2000 DEM at 5 m, 1-10 ellipses each, "
+                               f"log-uniform 2k-1M particles, 50% Voellmy / 30% Coulomb / 20% mu(I), 120 s each",
+                   "particles_total": int(sizes.sum()),
+                   "friction_counts": {"coulomb": fr.count(0), "voellmy": fr.count(1), "mu_I": fr.count(2)}},
+        "fp32_batched": {k2: v for k2, v in bat.items() if k2 != "results"},
+        "fp64_one_at_a_time": {"sample": f"{k} scenarios, one per size stratum (ranks {sample})",
+                               **{k2: v for k2, v in one64.items() if k2 != "per_scenario_s"}},
+        "fp32_one_at_a_time": {k2: v for k2, v in one32.items() if k2 != "per_scenario_s"},
+        "speedup_vs_fp64_one_at_a_time": bat["scenarios_per_hour"] / one64["scenarios_per_hour"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serve_main(args):
+    """C4: open-loop Poisson arrivals drawn from the ensemble distribution on the
+    2000x2000 DEM; continuous batching with admission by device footprint."""
+    import numpy as np
+    from slabpro import configs, ensemble
+    dem = configs.c1_dem()
+    res = {}
+    n = args.requests
+    t0 = time.time()
+    reqs = [configs.c3_scenario(1000 + k) for k in range(n)]
+    log(f"serve: {n} requests generated in {time.time() - t0:.1f}s, {sum(len(r['x']) for r in reqs)} particles")
+    for lam in [float(v) for v in args.rates.split(",")]:
+        arr, _ = configs.c4_requests(n, lam, seed_index=int(lam * 10))
+        r = ensemble.serve(dem, 5.0, reqs, arr, max_slots=64, capacity=12_000_000, progress=log)
+        res[f"lambda_{lam:g}"] = r
+        log(f"lambda {lam:g}/s: p50 {r['p50_s']:.2f}s p95 {r['p95_s']:.2f}s p99 {r['p99_s']:.2f}s, "
+            f"{r['scenarios_per_hour']:.0f} scen/h over {r['wall_s']:.1f}s")
+    first = res[next(iter(res))]
+    out = {"metric": "p99 completion latency (serving)", "value": first["p99_s"], "unit": "s", "n_gpus": 1,
+           "higher_is_better": False, "dtype": "f32", "data": "synthetic (seeded requests, Poisson arrivals)",
+           "config": {"workload": f"c4: {n} of the 2,000 seeded requests per rate, rates {args.rates}/s, "
+                                  "c3 scenario distribution on the 2000x2000 DEM, 64 slots, 12M-particle capacity"},
+           "per_rate": res}
     print(json.dumps(out), flush=True)
 
 

@@ -5,7 +5,7 @@
  * Written from the equations of the reference; every block cites the
  * reference lines (src/) whose arithmetic it restates.
  */
-#define _POSIX_C_SOURCE 199309L
+#define _GNU_SOURCE /* sincos */
 #include "slab_oracle.h"
 
 #include <math.h>
@@ -568,9 +568,13 @@ void orc_set_time(orc_sim* s, double time, int32_t frame) {
 
 /* ---------------- front-end restatements ---------------- */
 
-/* simulation.cpp:478-482 */
+/* simulation.cpp:478-482.  cos and sin of one angle come from one sincos()
+ * call, as a GCC build of the reference computes them (GCC merges the pair
+ * into sincos at -O1 and above); glibc's sincos and cos/sin can differ in
+ * the last bit. */
 static void rotate(double px, double py, double ang, double* ox, double* oy) {
-    double c = cos(ang), sn = sin(ang);
+    double c, sn;
+    sincos(ang, &sn, &c);
     *ox = c * px - sn * py;
     *oy = sn * px + c * py;
 }

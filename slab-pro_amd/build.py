@@ -1,7 +1,10 @@
-"""Builds libslabpro.so (HIP, gfx950) in-tree: slab-pro_amd/lib/libslabpro.so.
+"""Builds libslabpro.so (HIP, gfx950) and the slab-pro command in-tree:
+slab-pro_amd/lib/libslabpro.so, slab-pro_amd/bin/slab-pro.
 
-Each .hip translation unit is compiled by hipcc for gfx950 only, then linked
-into one shared library exporting the C ABI of include/slab_abi.h.
+Each .hip translation unit is compiled by hipcc for gfx950 only; the host
+front end (host/*.cpp: JSON, readers, writers, samplers) by g++; all are
+linked into one shared library exporting the C ABI of include/slab_abi.h and
+include/slab_frontend.h.  The command links against that library.
 """
 import os
 import subprocess
@@ -10,9 +13,19 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
 LIBDIR = os.path.join(HERE, "lib")
+BINDIR = os.path.join(HERE, "bin")
 LIB = os.path.join(LIBDIR, "libslabpro.so")
+CLI = os.path.join(BINDIR, "slab-pro")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+
+HOST_SOURCES = ["io.cpp", "poisson.cpp", "simulation.cpp", "frontend_abi.cpp"]
+# host arithmetic without FMA contraction, so initial states are the
+# reference's operation for operation on any x86-64 target
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
+              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-I" + HOST]
 
 # Per-TU floating-point contraction: the FP64 form is built without FMA
 # contraction so its arithmetic is the reference's operation for operation.
@@ -48,9 +61,20 @@ def build(verbose=False):
         if verbose and out:
             print(out)
         objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(LIBDIR, "host_" + src.replace(".cpp", ".o"))
+        out = _run([CXX] + HOST_FLAGS + ["-c", os.path.join(HOST, src), "-o", obj])
+        if verbose and out:
+            print(out)
+        objs.append(obj)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)
+    os.makedirs(BINDIR, exist_ok=True)
+    tmp = CLI + ".tmp"
+    _run([CXX] + HOST_FLAGS + [os.path.join(HOST, "main.cpp"), "-o", tmp, "-L" + LIBDIR, "-lslabpro",
+                               "-Wl,-rpath,$ORIGIN/../lib"])
+    os.replace(tmp, CLI)
     return LIB
 
 

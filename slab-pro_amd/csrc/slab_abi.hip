@@ -3,6 +3,7 @@
 // FP32 runs and batches drive the FP32 engine (slab_engine32.hip).
 #include "slab_abi.h"
 #include "slab_engine32.hpp"
+#include "slab_error.hpp"
 #include "slab_kernels.hpp"
 
 #include <algorithm>
@@ -307,6 +308,8 @@ slab_status check_params(const slab_params* p, const slab_friction* f) {
 }
 
 } // namespace
+
+slab_status slab::set_error(slab_status s, const std::string& msg) { return fail(s, msg); }
 
 extern "C" {
 

@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG, "lib", "libslabpro.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "slab_abi.h")
+FRONTEND_HEADER = os.path.join(os.path.dirname(PKG), "include", "slab_frontend.h")
 
 OK, EINVAL, ENOMEM, EDEVICE, EDOMAIN, ENAN, ESTATE = range(7)
 STATUS_NAMES = ["SLAB_OK", "SLAB_EINVAL", "SLAB_ENOMEM", "SLAB_EDEVICE", "SLAB_EDOMAIN", "SLAB_ENAN",
@@ -97,6 +98,30 @@ SIGNATURES = {
     "slab_batch_use_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "slab_batch_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _DP, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
     "slab_batch_destroy": (None, [_VP]),
+    # include/slab_frontend.h
+    "slab_sim_load_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "slab_sim_initialize": (ctypes.c_int, [_VP]),
+    "slab_sim_initialize_terrain": (ctypes.c_int, [_VP]),
+    "slab_sim_initialize_particles": (ctypes.c_int, [_VP]),
+    "slab_sim_simulate": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int64)]),
+    "slab_sim_set_verbose": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32]),
+    "slab_sim_set_precision": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "slab_sim_terrain": (ctypes.c_int, [_VP, _IP, _IP, _DP, _DP, _DP, _DP]),
+    "slab_sim_num_particles": (ctypes.c_int64, [_VP]),
+    "slab_sim_particles": (ctypes.c_int, [_VP] + [_DP] * 5),
+    "slab_sim_release": (ctypes.c_int, [_VP, _DP, _DP]),
+    "slab_sim_output": (ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int32, _IP, _IP]),
+    "slab_sim_destroy": (None, [_VP]),
+    "slab_poisson_disk": (ctypes.c_int64, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                           _DP, _DP, ctypes.c_int64]),
+    "slab_polygon_area": (ctypes.c_double, [_DP, _DP, ctypes.c_int32]),
+    "slab_point_in_polygon": (ctypes.c_int32, [ctypes.c_double, ctypes.c_double, _DP, _DP, ctypes.c_int32]),
+    "slab_read_shapefile_polygon": (ctypes.c_int, [ctypes.c_char_p, _DP, _DP, ctypes.c_int32, _IP]),
+    "slab_write_ply": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64] + [_DP] * 8),
+    "slab_write_geotiff": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_char_p),
+                                          ctypes.POINTER(ctypes.POINTER(ctypes.c_float)), ctypes.c_float]),
 }
 
 _lib = None

@@ -1,0 +1,158 @@
+"""Ensemble and serving drivers over the C ABI.
+
+run_batched       -- all scenarios of an ensemble in one FP32 batch (admitted
+                     as slots free up), advanced together
+run_one_at_a_time -- the plain FP64 path: one scenario per run, sequentially
+serve             -- continuous batching: open-loop arrivals, admission by
+                     device footprint (free slots + particle capacity), batch
+                     reformed every step, p50/p95/p99 completion latency
+"""
+import collections
+import time
+
+import numpy as np
+
+from . import abi
+
+
+def _params(s, precision):
+    return abi.params(precision=precision, **s["params"])
+
+
+def _fric(s):
+    return abi.friction(**s["friction"])
+
+
+def _footprint(s):
+    return len(s["x"])
+
+
+def run_batched(dem, dx, scenarios, max_slots=256, capacity=None, steps_per_call=32, graph=True, results=False,
+                progress=None, every_s=15.0):
+    """Advance every scenario in one FP32 batch.  Scenarios are admitted in
+    order whenever a slot and particle capacity are free; finished slots are
+    collected (max maps) and released.  Returns wall time and counts.
+    progress(msg) is called about every every_s seconds with the state of the
+    batch (finished scenarios, active slots, simulated-time range)."""
+    n = len(scenarios)
+    cap = capacity or sum(_footprint(s) for s in scenarios)
+    t0 = time.perf_counter()
+    b = abi.Batch(dem, dx, min(max_slots, n), cap)
+    b.use_graph(graph)
+    t_setup = time.perf_counter() - t0
+    todo = collections.deque(range(n))
+    slot_of = {}
+    done = {}
+    pupd = 0
+    calls = 0
+    t0 = time.perf_counter()
+    last = t0
+    while todo or slot_of:
+        while todo:
+            s = scenarios[todo[0]]
+            slot = b.admit(_params(s, abi.FP32), _fric(s), s["x"], s["y"], s["h"], s["A"], s["m"])
+            if slot is None:
+                break
+            slot_of[slot] = todo.popleft()
+        if not slot_of:
+            raise RuntimeError(f"scenario {todo[0]} does not fit an empty batch")
+        b.step(steps_per_call)
+        calls += 1
+        st, tm = b.statuses()
+        if progress and time.perf_counter() - last > every_s:
+            last = time.perf_counter()
+            act = [tm[k] for k in slot_of]
+            progress(f"batched: {len(done)}/{n} done, {len(slot_of)} active, {calls * steps_per_call} steps, "
+                     f"t in [{min(act):.2f}, {max(act):.2f}] s, {last - t0:.0f}s wall")
+        for slot, idx in list(slot_of.items()):
+            if st[slot] in (abi.SLOT_DONE, abi.SLOT_FAILED):
+                info = b.slot_info(slot)
+                pupd += info["np"] * info["step"]
+                done[idx] = (info, b.max_maps(slot) if results else None)
+                b.release(slot)
+                del slot_of[slot]
+    wall = time.perf_counter() - t0
+    b.close()
+    failed = sum(1 for v in done.values() if v[0]["status"] == abi.SLOT_FAILED)
+    return {"wall_s": wall, "setup_s": t_setup, "scenarios": n, "failed": failed,
+            "scenarios_per_hour": n / wall * 3600.0, "particle_updates": pupd,
+            "particle_updates_per_s": pupd / wall, "results": done}
+
+
+def run_one_at_a_time(dem, dx, scenarios, precision=abi.FP64, progress=None):
+    """The plain path: each scenario its own run (create, upload, prime, run to
+    total_time, download max maps), one after another.  progress(msg) is
+    called after each scenario."""
+    t0 = time.perf_counter()
+    per = []
+    pupd = 0
+    for s in scenarios:
+        t1 = time.perf_counter()
+        run = abi.Run(dem, dx, _params(s, precision), _fric(s))
+        run.upload(s["x"], s["y"], s["h"], s["A"], s["m"])
+        run.prime()
+        _, info = run.run()
+        run.max_maps()
+        run.close()
+        pupd += len(s["x"]) * info["step"]
+        per.append(time.perf_counter() - t1)
+        if progress:
+            progress(f"one-at-a-time: {len(per)}/{len(scenarios)} ({len(s['x'])} particles, {info['step']} steps) "
+                     f"in {per[-1]:.1f}s")
+    wall = time.perf_counter() - t0
+    return {"wall_s": wall, "scenarios": len(scenarios), "per_scenario_s": per,
+            "scenarios_per_hour": len(scenarios) / wall * 3600.0, "particle_updates": pupd,
+            "particle_updates_per_s": pupd / wall}
+
+
+def serve(dem, dx, requests, arrivals, max_slots=64, capacity=8_000_000, steps_per_call=8, graph=True,
+          progress=None):
+    """Continuous batching.  requests[k] (a scenario dict) arrives at wall
+    time arrivals[k] (s, open loop).  Each loop: admit arrived requests in FIFO
+    order while a slot and particle capacity are free; run a few batch steps;
+    collect and release finished slots.  Latency = completion - arrival."""
+    b = abi.Batch(dem, dx, max_slots, capacity)
+    b.use_graph(graph)
+    n = len(requests)
+    order = np.argsort(arrivals, kind="stable")
+    nxt = 0
+    queue = collections.deque()
+    slot_of = {}
+    lat = np.full(n, np.nan)
+    busy_particles = []
+    t0 = time.perf_counter()
+    last_report = t0
+    while nxt < n or queue or slot_of:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[order[nxt]] <= now:
+            queue.append(order[nxt])
+            nxt += 1
+        while queue:
+            k = queue[0]
+            s = requests[k]
+            slot = b.admit(_params(s, abi.FP32), _fric(s), s["x"], s["y"], s["h"], s["A"], s["m"])
+            if slot is None:
+                break
+            slot_of[slot] = k
+            queue.popleft()
+        if slot_of:
+            b.step(steps_per_call)
+            st, _ = b.statuses()
+            fin = time.perf_counter() - t0
+            for slot, k in list(slot_of.items()):
+                if st[slot] in (abi.SLOT_DONE, abi.SLOT_FAILED):
+                    lat[k] = fin - arrivals[k]
+                    b.release(slot)
+                    del slot_of[slot]
+            busy_particles.append(sum(len(requests[k]["x"]) for k in slot_of.values()))
+        elif nxt < n:
+            time.sleep(max(0.0, min(0.05, arrivals[order[nxt]] - (time.perf_counter() - t0))))
+        if progress and time.perf_counter() - last_report > 30:
+            last_report = time.perf_counter()
+            progress(f"serve: {int(np.sum(np.isfinite(lat)))}/{n} done, queue {len(queue)}, active {len(slot_of)}")
+    wall = time.perf_counter() - t0
+    b.close()
+    return {"wall_s": wall, "requests": n, "p50_s": float(np.nanpercentile(lat, 50)),
+            "p95_s": float(np.nanpercentile(lat, 95)), "p99_s": float(np.nanpercentile(lat, 99)),
+            "mean_s": float(np.nanmean(lat)), "scenarios_per_hour": n / wall * 3600.0,
+            "mean_active_particles": float(np.mean(busy_particles)) if busy_particles else 0.0}

@@ -1,5 +1,6 @@
 """C-ABI library checks that need no GPU: it loads, exports every function
-include/slab_abi.h declares, and fails cleanly (no CPU fallback) without a device."""
+include/slab_abi.h and include/slab_frontend.h declare, and fails cleanly (no
+CPU fallback) without a device."""
 import os
 import re
 import subprocess
@@ -10,9 +11,10 @@ from slabpro import abi
 
 
 def _declared_functions():
-    text = open(abi.HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = set(re.findall(r"\b(slab_[a-z0-9_]+)\s*\(", text))
+    names = set()
+    for h in (abi.HEADER, abi.FRONTEND_HEADER):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(slab_[a-z0-9_]+)\s*\(", text))
     return sorted(names)
 
 

@@ -136,6 +136,21 @@ void Simulation::loadFromJSON(const std::string& path) {
         c.terrain_Lx = a.at("Lx").num();
         c.terrain_Ly = a.at("Ly").num();
         c.dx = a.at("dx").num();
+        // optional extension: "profile" selects another analytical terrain
+        if (a.contains("profile")) {
+            const std::string pr = a.at("profile").str();
+            if (pr == "PLANE") {
+                c.profile = 0;
+            } else if (pr == "PLANE_ARC_FLAT") {
+                c.profile = 1;
+                if (a.contains("theta_deg")) c.prof_theta_deg = a.at("theta_deg").num();
+                if (a.contains("x_t")) c.prof_x_t = a.at("x_t").num();
+                if (a.contains("R")) c.prof_R = a.at("R").num();
+                if (a.contains("z_f")) c.prof_z_f = a.at("z_f").num();
+            } else {
+                throw std::runtime_error("Unknown analytical profile: " + pr);
+            }
+        }
     }
     const Json& re = j.at("release");
     c.release = parse_release(re.at("type").str());
@@ -215,11 +230,29 @@ void Simulation::initializeTerrain() {
         t.nx = (int)t.x.size();
         t.ny = (int)t.y.size();
         t.z.resize((size_t)t.nx * t.ny);
-        // the one analytical terrain: a plane falling 5 degrees along x and
-        // along y (src/simulation.cpp:181-185)
-        const double th = 5.0 * M_PI / 180.0;
-        for (int i = 0; i < t.nx; ++i)
-            for (int j = 0; j < t.ny; ++j) t.z[(size_t)i * t.ny + j] = -t.x[i] * std::tan(th) - t.y[j] * std::tan(th);
+        if (cfg.profile == 0) {
+            // the reference's analytical terrain: a plane falling 5 degrees
+            // along x and along y (src/simulation.cpp:181-185)
+            const double th = 5.0 * M_PI / 180.0;
+            for (int i = 0; i < t.nx; ++i)
+                for (int j = 0; j < t.ny; ++j)
+                    t.z[(size_t)i * t.ny + j] = -t.x[i] * std::tan(th) - t.y[j] * std::tan(th);
+        } else {
+            // plane / circular arc / flat runout along x, constant along y
+            const double th = cfg.prof_theta_deg * M_PI / 180.0, R = cfg.prof_R, xt = cfg.prof_x_t;
+            const double xc = xt + R * std::sin(th);
+            for (int i = 0; i < t.nx; ++i) {
+                const double x = t.x[i];
+                double z;
+                if (x <= xt)
+                    z = cfg.prof_z_f + R * (1.0 - std::cos(th)) + (xt - x) * std::tan(th);
+                else if (x < xc)
+                    z = cfg.prof_z_f + R - std::sqrt(R * R - (x - xc) * (x - xc));
+                else
+                    z = cfg.prof_z_f;
+                for (int j = 0; j < t.ny; ++j) t.z[(size_t)i * t.ny + j] = z;
+            }
+        }
     }
     t.dx = cfg.dx;
     terrain = std::move(t);

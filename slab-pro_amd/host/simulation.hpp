@@ -31,6 +31,11 @@ struct SimConfig {
     TerrainType terrain = TerrainType::DEM;
     std::string dem_path;
     double terrain_Lx = 0, terrain_Ly = 0, dx = 1.0;
+    // analytical profile: 0 = the reference's 5-degree plane in x and y;
+    // 1 = plane of slope theta for x <= x_t, circular arc of radius R, flat
+    // runout at z_f (z and slope continuous)
+    int profile = 0;
+    double prof_theta_deg = 35.0, prof_x_t = 600.0, prof_R = 300.0, prof_z_f = 0.0;
     ReleaseShapeType release = ReleaseShapeType::Rectangle;
     double h0 = 0;
     double rect_cx = 0, rect_cy = 0, rect_Lx = 0, rect_Ly = 0, rect_rot = 0; // rot in radians

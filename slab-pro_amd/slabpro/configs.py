@@ -53,7 +53,7 @@ def generate_grid(length, dx):
 def c0_profile(x, theta_deg=35.0, x_t=600.0, R=300.0, z_f=0.0):
     """35 deg plane for x <= 600 m, circular arc of radius 300 m, flat runout.
     z and slope are continuous at x_t and at x_c = x_t + R sin(theta)."""
-    th = math.radians(theta_deg)
+    th = theta_deg * math.pi / 180.0
     x_c = x_t + R * math.sin(th)
     x = np.asarray(x, dtype=np.float64)
     z = np.empty_like(x)

@@ -186,6 +186,33 @@ def test_analytical_terrain_closed_form(tmp_path):
     assert (t["xllcorner"], t["yllcorner"]) == (0.0, 0.0)
 
 
+def test_c0_from_json_equals_the_c0_workload(tmp_path):
+    """The 4,800-particle C0 case (35-degree plane, arc, flat runout; 60x80 m
+    rectangle) written as a JSON file gives the same terrain and particles as
+    the workload generator used by the tests and the benchmark."""
+    from slabpro import configs
+    edits = {"terrain.type": "ANALYTICAL", "terrain.analytical": {
+                 "Lx": 1200.0, "Ly": 300.0, "dx": 2.0, "profile": "PLANE_ARC_FLAT",
+                 "theta_deg": 35.0, "x_t": 600.0, "R": 300.0, "z_f": 0.0},
+             "release.type": "RECTANGLE", "release.h0": 2.0,
+             "release.rectangle": {"center": [100.0, 150.0], "Lx": 60.0, "Ly": 80.0, "rotation_deg": 0.0},
+             "numerical_params.sampling_method": "UNIFORM", "physical_params.rho": 300.0}
+    _, p = _example("brienz_polygon", tmp_path, **edits)
+    s = _sim(p)
+    s.initialize_terrain()
+    s.initialize_particles()
+    c0 = configs.c0()
+    assert np.array_equal(s.terrain()["z"], c0["z"])
+    q = s.particles()
+    assert len(q["x"]) == 4800
+    for k in ("x", "y", "h", "A", "m"):
+        assert np.array_equal(q[k], c0[k]), k
+    _, p2 = _example("brienz_polygon", tmp_path, **{"terrain.type": "ANALYTICAL",
+                                                      "terrain.analytical.profile": "CONE"})
+    with pytest.raises(abi.SlabError, match="Unknown analytical profile: CONE"):
+        frontend.Simulation.load_from_json(p2)
+
+
 def _compare_init(s, ref):
     p = s.particles()
     assert len(p["x"]) == len(ref["x"]) > 0

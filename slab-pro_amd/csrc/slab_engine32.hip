@@ -54,17 +54,24 @@ __device__ __forceinline__ int ebase(int ci, float o, int nmax) {
 
 __device__ __forceinline__ bool slot_active(int st) { return st == SLOT_PRIMING || st == SLOT_RUNNING; }
 
-// Key of a particle: live (slot, tile, cell), parking tile of a finished slot,
-// or -1 (released slot: the particle drops out of the next sort).
-__device__ __forceinline__ int ekey(const ECtl* c, int st, int slot, int cell, float ox, float oy, int* tile) {
-    if (slot_active(st)) {
+// True when a cell lies outside [1, N-3] on either axis (outflow).
+__device__ __forceinline__ bool cell_out(const ECtl* c, int ci, int cj) {
+    return ci < 1 || ci > c->nx - 3 || cj < 1 || cj > c->ny - 3;
+}
+
+// Key of a particle: live (slot, tile, cell), parking tile of its slot (the
+// slot finished, or the particle flowed out), or -1 (released slot: the
+// particle drops out of the next sort).
+__device__ __forceinline__ int ekey(const ECtl* c, int st, int slot, bool out, int cell, float ox, float oy,
+                                    int* tile) {
+    if (slot_active(st) && !out) {
         const int ci = cell >> 16, cj = cell & 0xffff;
         const int ib = ebase(ci, ox, c->nx - 3), jb = ebase(cj, oy, c->ny - 3);
         const int t = slot * c->ntd + (ib >> 4) * c->nty + (jb >> 4);
         *tile = t;
         return t * TILE_CELLS + (((ib & 15) << 4) | (jb & 15));
     }
-    if (st == SLOT_DONE || st == SLOT_FAILED) {
+    if (slot_active(st) || st == SLOT_DONE || st == SLOT_FAILED) {
         const int t = c->nslots * c->ntd + slot;
         *tile = t;
         return t * TILE_CELLS;
@@ -271,8 +278,8 @@ __global__ void __launch_bounds__(WG) k_ekeys(ECtl* c, const Slot* __restrict__ 
     const bool in = s < s1;
     int t = 0, k = -1;
     if (in) {
-        const int sl = P.slot[s];
-        k = ekey(c, slots[sl].status, sl, P.cell[s], P.ox[s], P.oy[s], &t);
+        const int raw = P.slot[s], sl = raw & SLOT_MASK;
+        k = ekey(c, slots[sl].status, sl, (raw & SLOT_OUT) != 0, P.cell[s], P.ox[s], P.oy[s], &t);
     }
     const bool valid = in && k >= 0;
     const int q = wave_slot_any(t, valid, tile_count);
@@ -722,12 +729,14 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
         const int nx = c->nx, ny = c->ny;
         const size_t nn = (size_t)nx * ny;
         const int s = idx[r];
-        sl = P.slot[s];
+        const int raw = P.slot[s];
+        sl = raw & SLOT_MASK;
+        bool out = (raw & SLOT_OUT) != 0;
         const Slot& S = slots[sl];
         const int st = S.status;
         int cell = P.cell[s];
         float ox = P.ox[s], oy = P.oy[s], vxp = P.vx[s], vyp = P.vy[s], vzp = P.vz[s], hp = P.h[s], zp = P.z[s];
-        if (st == SLOT_RUNNING && !c->compact) {
+        if (st == SLOT_RUNNING && !c->compact && !out) {
             const float inv = (float)c->inv_dx, dt = (float)S.dt, fr = (float)S.flip_ratio;
             const int ci = cell >> 16, cj = cell & 0xffff;
             const size_t fc = (size_t)ci * ny + cj;
@@ -778,6 +787,10 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
             vzp = nvx * Gx + nvy * Gy;
             hp = hp / (1.f + tr);
             zp = z;
+            if (cell_out(c, nci, ncj)) { // outflow: frozen from here on
+                out = true;
+                vxp = vyp = vzp = 0.f;
+            }
         }
         Q.cell[r] = cell;
         Q.ox[r] = ox;
@@ -789,9 +802,9 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
         Q.z[r] = zp;
         Q.m[r] = P.m[s];
         Q.pid[r] = P.pid[s];
-        Q.slot[r] = sl;
-        kk = ekey(c, st, sl, cell, ox, oy, &t);
-        pub = slot_active(st);
+        Q.slot[r] = sl | (out ? SLOT_OUT : 0);
+        kk = ekey(c, st, sl, out, cell, ox, oy, &t);
+        pub = slot_active(st) && !out;
         v2 = vxp * vxp + vyp * vyp;
     }
     const int q = wave_slot_any(t, in && kk >= 0, tile_count);
@@ -811,7 +824,7 @@ __global__ void k_ecollect(ECtl* c, PB32 A, PB32 B, int slot, int* __restrict__ 
     const PB32 P = ecur(c, A, B);
     const int s = blockIdx.x * WG + threadIdx.x;
     if (s >= c->n_store) return;
-    if (P.slot[s] != slot) return;
+    if ((P.slot[s] & SLOT_MASK) != slot) return;
     const int q = atomicAdd(counter, 1);
     if (q >= cap) return;
     o_cell[q] = P.cell[s];
@@ -832,7 +845,7 @@ __global__ void k_ecollect(ECtl* c, PB32 A, PB32 B, int slot, int* __restrict__ 
 __global__ void k_ebbox(ECtl* c, PB32 A, PB32 B, int slot, int n, unsigned long long* __restrict__ bb) {
     const PB32 P = c->p2g_parity ? B : A;
     const int s = blockIdx.x * WG + threadIdx.x;
-    if (s >= n || P.slot[s] != slot) return;
+    if (s >= n || (P.slot[s] & SLOT_MASK) != slot) return;
     const int cell = P.cell[s];
     const double x = ((double)(cell >> 16) + (double)P.ox[s]) * c->dx;
     const double y = ((double)(cell & 0xffff) + (double)P.oy[s]) * c->dx;
@@ -900,7 +913,7 @@ __global__ void __launch_bounds__(WG) k_ediag(const ECtl* c, PB32 A, PB32 B, int
     }
     const PB32 P = ecur(c, A, B);
     for (int r = blockIdx.x * WG + threadIdx.x; r < c->n_store; r += gridDim.x * WG) {
-        if (P.slot[r] != slot) continue;
+        if ((P.slot[r] & SLOT_MASK) != slot) continue;
         const double m = P.m[r];
         a[4] += m;
         a[5] += m / rho;
@@ -1318,6 +1331,7 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
         if (!(a >= 0.f)) a = 0.f;
         if (!(b >= 0.f)) b = 0.f;
         cell[p] = (ii << 16) | jj;
+        if (ii < 1 || ii > e->nx - 3 || jj < 1 || jj > e->ny - 3) sl[p] = slot | SLOT_OUT; // outside: parked
         ox[p] = a;
         oy[p] = b;
         fvx[p] = vx ? (float)vx[p] : 0.f;

@@ -36,6 +36,13 @@ struct PB32 {
     int32_t *pid, *slot;
 };
 
+// Outflow: a particle whose position leaves the cells [1, N-3] of either axis
+// (where its 3x3 stencil would leave the terrain) is parked for the rest of
+// its run: its state is frozen (velocity zero), it no longer enters P2G / G2P
+// or the CFL dt.  Marked by this bit in the particle's slot field.
+constexpr int32_t SLOT_OUT = 1 << 30;
+constexpr int32_t SLOT_MASK = SLOT_OUT - 1;
+
 constexpr int REGION = TILE + 2;              // 18x18 P2G partial region per particle tile
 constexpr int REGION_NODES = REGION * REGION;
 constexpr int P2G_FIELDS = 7;                 // m, P_x, P_y, P_z, H, F_x, F_y

@@ -310,10 +310,13 @@ def tiny(nx=48, ny=40, dx=2.0, theta_deg=30.0, fric=None, seed=7, jitter=True, r
 
 # ---------------------------------------------------------------- ensemble (c3) / requests (c4)
 
-def _scenario_release(g, dx, n_target, x_band=(600.0, 3000.0), y_band=(1500.0, 8500.0)):
+def _scenario_release(g, dx, n_target, x_band=(600.0, 3000.0), y_band=(1500.0, 8500.0), extent=10000.0,
+                      margin=250.0):
     """1-10 ellipses clustered around a random centre, semi-axes scaled so the
     UNIFORM sample count is within 5% of n_target.  Overlaps take the larger
-    height (as a raster release would); each particle covers dx^2/4."""
+    height (as a raster release would); each particle covers dx^2/4.  The
+    shapes are then translated, if needed, so every particle lies at least
+    `margin` inside the [0, extent]^2 DEM."""
     k = int(g.integers(1, 11))
     cx, cy = g.uniform(*x_band), g.uniform(*y_band)
     shapes = []
@@ -332,6 +335,15 @@ def _scenario_release(g, dx, n_target, x_band=(600.0, 3000.0), y_band=(1500.0, 8
         if abs(ratio - 1.0) <= 0.05:
             break
         scale /= math.sqrt(ratio)
+    sx = sy = 0.0
+    if len(xs):
+        sx = max(0.0, margin - xs.min()) - max(0.0, xs.max() - (extent - margin))
+        sy = max(0.0, margin - ys.min()) - max(0.0, ys.max() - (extent - margin))
+    if sx or sy:
+        # whole cells, so the UNIFORM lattice and the sample count are kept
+        sx, sy = math.ceil(abs(sx) / dx) * dx * math.copysign(1, sx), math.ceil(abs(sy) / dx) * dx * math.copysign(1, sy)
+        shapes = [(ex + sx, ey + sy, rx, ry, rot, hgt) for (ex, ey, rx, ry, rot, hgt) in shapes]
+        xs, ys, hs = _ellipse_union(dx, shapes, scale)
     return xs, ys, hs
 
 

@@ -42,13 +42,13 @@ def _params(total=3.0):
     return abi.params(rho=300.0, total_time=total, output_dt=0.5, precision=abi.FP32)
 
 
-def _run_batch(scens, max_slots=8, cap=200000, steps=100000):
+def _run_batch(scens, max_slots=8, cap=200000, steps=100000, total=3.0):
     from slabpro import abi
     z, dx = _terrain()
     b = abi.Batch(z, dx, max_slots, cap)
     slots = []
     for part, fr in scens:
-        s = b.admit(_params(), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
+        s = b.admit(_params(total), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
         assert s is not None
         slots.append(s)
     n = 0
@@ -174,3 +174,29 @@ def test_graph_replay_matches_eager():
         b.close()
     for (ha, xa), (hb, xb) in zip(*outs):
         assert np.array_equal(ha, hb) and np.array_equal(xa, xb)
+
+
+def test_outflow_particles_are_parked_and_isolated():
+    """Particles that flow off the terrain (cell outside [1, N-3]) are frozen
+    in place with zero velocity and leave P2G/G2P; a neighbour slot's results
+    are unchanged by them (bitwise, alone vs in the mix)."""
+    from slabpro import abi, configs
+    dx = 2.0
+    px, py, area = configs.ellipse_release(dx, 172.0, 80.0, 10.0, 8.0, 0.0)   # 18 m from the +x edge
+    h = np.full(len(px), 1.5)
+    edge = (configs.finish_particles(px, py, h, area * 1.5, 300.0), {"type": 0, "mu": 0.05})
+    inner = _scenario(11, FRICS[1])
+    (hm_e, pe, ie), (hm_i, pi_, ii) = _run_batch([edge, inner], total=6.0)
+    (hm_a, pa, ia), = _run_batch([inner], total=6.0)
+    assert ie["status"] == abi.SLOT_DONE and ii["status"] == abi.SLOT_DONE
+    nx, ny = _terrain()[0].shape
+    x_out = (nx - 2) * dx
+    out = pe["x"] >= x_out
+    print(f"outflow: {int(out.sum())} of {len(out)} particles parked", flush=True)
+    assert out.sum() > 0.9 * len(out)
+    assert np.all(pe["vx"][out] == 0) and np.all(pe["vy"][out] == 0)
+    assert np.all(pe["x"] <= (nx - 1) * dx) and np.all(np.isfinite(pe["h"]))
+    assert np.allclose(pe["m"].sum(), edge[0]["m"].sum(), rtol=1e-6)
+    for k in ("x", "y", "vx", "vy", "h"):
+        assert np.array_equal(pi_[k], pa[k]), k
+    assert np.array_equal(hm_i[0], hm_a[0])

@@ -7,9 +7,12 @@ pass of the step loop over all particles (sort, P2G + grid update, G2P, dt).
 Inputs are resident in device memory before the timed region starts.
 
 Prints ONE JSON line.  roofline: dominant kernel's algorithmic bytes per launch
-/ its mean HIP-event duration vs the 8 TB/s HBM peak.  cpu_baseline: the FP64
-CPU oracle (single-threaded restatement of the reference step) timed on this
-host on a bounded number of steps of the same workload.
+/ its mean HIP-event duration vs the 8 TB/s HBM peak; roofline.traffic: that
+kernel's HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary of the same workload (profiles/step_<form>_<workload>_*.md),
+null when there is none.  cpu_baseline: the FP64 CPU oracle (single-threaded
+restatement of the reference step) timed on this host on a bounded number of
+steps of the same workload.
 """
 import argparse
 import json
@@ -38,8 +41,9 @@ ALG_BYTES_FP64 = {
 }
 STEP_BYTES_FP64 = {"particle": 104, "node": 120}
 # FP32 form: particle = cell int32 + offsets + vx vy vz h m (32 B read, 28 B
-# written by G2P); "p2g" = per-tile LDS reduction of the particle state,
-# "grid" = partial merge + grid update (terrain 12 B, max maps 16 B, vf 16 B).
+# written by G2P); "p2g" = the fused per-tile sort + LDS reduction of the
+# particle state (sort traffic excluded), "grid" = partial merge + grid update
+# (terrain 12 B, max maps 16 B, vf 16 B).
 ALG_BYTES_FP32 = {
     "p2g": {"particle": 32, "node": 0},
     "grid": {"particle": 0, "node": 12 + 16 + 16},
@@ -48,6 +52,30 @@ ALG_BYTES_FP32 = {
     "scatter": {"particle": 12, "node": 0},
 }
 STEP_BYTES_FP32 = {"particle": 60, "node": 60}
+
+# kernel names of the event timers -> rocprof kernel names, per form
+ROCPROF_NAMES = {
+    "fp32": {"g2p": "k_eg2p", "p2g": "k_esortp2g", "grid": "k_egrid", "scatter": "k_escatter"},
+    "fp64": {"g2p": "k_g2p", "p2g": "k_payload", "grid": "k_p2g_grid", "tile_sort": "k_tile_sort",
+             "scatter": "k_scatter"},
+}
+
+
+def measured_traffic(form, workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprof summary
+    (column 'HBM MB/launch (scaled)'), or (None, None)."""
+    import glob
+    want = "`" + ROCPROF_NAMES.get(form, {}).get(kernel, "?") + "`"
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"step_{form}_{workload}_*.md"))):
+        with open(path) as f:
+            for line in f:
+                cols = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cols) >= 8 and cols[0] == want:
+                    try:
+                        return float(cols[6].replace(",", "")) * 1e6, os.path.relpath(path, ROOT)
+                    except ValueError:
+                        pass
+    return None, None
 
 
 def log(*a):
@@ -84,12 +112,15 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2"])
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", default="fp64", choices=["fp32", "fp64"])
+    ap.add_argument("--form", default="fp32", choices=["fp32", "fp64"],
+                    help="fp32: the FP32 form (int cell + FP32 offsets, FP32 bulk, FP64 time); fp64: plain form")
     ap.add_argument("--ensemble", action="store_true",
                     help="256-scenario ensemble: FP32 batched vs FP64 one-at-a-time, scenarios/h")
     ap.add_argument("--scenarios", type=int, default=256)
     ap.add_argument("--fp64-sample", type=int, default=16)
     ap.add_argument("--serve", action="store_true", help="continuous-batching server, p50/p95/p99 latency")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="full run of --workload in both forms: FP32 error vs FP64 (max maps, runout, deposit)")
     ap.add_argument("--rates", default="2,5,10")
     ap.add_argument("--requests", type=int, default=300)
     args = ap.parse_args()
@@ -106,6 +137,8 @@ def main():
         return ensemble_main(args)
     if args.serve:
         return serve_main(args)
+    if args.accuracy:
+        return accuracy_main(args)
 
     t0 = time.time()
     cfg = getattr(configs, args.workload)()
@@ -155,6 +188,7 @@ def main():
     alg = ab["particle"] * npart + ab["node"] * n_act
     achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
     step_alg = step_b["particle"] * npart + step_b["node"] * n_act
+    traffic, traffic_src = measured_traffic(args.form, args.workload, dom)
     log("kernel times (ms total, launches): " + ", ".join(f"{k} {v[0]:.2f}/{v[1]}" for k, v in tim.items()))
 
     cpu = None
@@ -183,8 +217,9 @@ def main():
             "c2": "8000x8000 DEM at 1 m, 32-vertex star polygon r=800 m h0=4 m, mu(I)"}[args.workload],
             "particles": npart, "grid": f"{nx}x{ny}", "form": args.form, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                     "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3,
+                     "traffic_source": traffic_src},
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBs": step_alg / (ms_step * 1e-3) / 1e9,
                           "frac": step_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "n_active_nodes": n_act},
         "kernels_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in tim.items()},
@@ -230,6 +265,23 @@ def ensemble_main(args):
         "fp32_one_at_a_time": {k2: v for k2, v in one32.items() if k2 != "per_scenario_s"},
         "speedup_vs_fp64_one_at_a_time": bat["scenarios_per_hour"] / one64["scenarios_per_hour"],
     }
+    print(json.dumps(out), flush=True)
+
+
+def accuracy_main(args):
+    """Full run of one workload in the FP64 and the FP32 form; FP32 error."""
+    from slabpro import accuracy, configs
+    t0 = time.time()
+    cfg = getattr(configs, args.workload)()
+    log(f"accuracy {args.workload}: {len(cfg['x'])} particles, generated in {time.time() - t0:.1f}s, "
+        f"total_time {cfg['params']['total_time']} s")
+    rep, tm = accuracy.compare_forms(cfg)
+    log(f"fp64 {tm['fp64_s']:.1f}s, fp32 {tm['fp32_s']:.1f}s: {rep}")
+    out = {"metric": "FP32-form error vs the FP64 form (full run)", "value": rep["hmax_L1"],
+           "unit": "relative L1 of the max-height map", "n_gpus": 1, "higher_is_better": False,
+           "dtype": "f32 vs f64", "data": "synthetic", "config": {"workload": args.workload,
+                                                                  "particles": len(cfg["x"])},
+           "report": rep, "wall_s": tm}
     print(json.dumps(out), flush=True)
 
 

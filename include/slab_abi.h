@@ -198,6 +198,15 @@ slab_status slab_batch_release(slab_batch* b, int32_t slot);
 slab_status slab_batch_footprint(slab_batch* b, int64_t out[4]);
 /* Replay the step's launch sequence from a captured hipGraph (on = 1). */
 slab_status slab_batch_use_graph(slab_batch* b, int32_t on);
+/* Storage re-sort period of the FP32 form (default 16): every `steps`-th step
+ * G2P writes the particles to the other buffer in canonical order (dropping
+ * released ones); the steps between update them in place.  Results do not
+ * depend on it (every sum runs in canonical order); speed does.  No reference
+ * counterpart beyond the every-50-steps block sort of src/simulation.cpp:540. */
+slab_status slab_batch_set_resort(slab_batch* b, int32_t steps);
+/* The same for a single FP32 run (after slab_run_upload_particles); a no-op
+ * for the FP64 form, which re-sorts its storage every step. */
+slab_status slab_run_set_resort(slab_run* run, int32_t steps);
 slab_status slab_batch_timing(slab_batch* b, int32_t enable, double* ms, int64_t* counts, int32_t n);
 void slab_batch_destroy(slab_batch* b);
 

@@ -944,6 +944,21 @@ slab_status slab_batch_use_graph(slab_batch* b, int32_t on) {
     return SLAB_OK;
 }
 
+slab_status slab_batch_set_resort(slab_batch* b, int32_t steps) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    std::string why;
+    return engine_set_reorder(b->eng, steps, &why) ? SLAB_OK : fail(SLAB_EINVAL, why);
+}
+
+slab_status slab_run_set_resort(slab_run* r, int32_t steps) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    if (steps < 1) return fail(SLAB_EINVAL, "re-sort period must be >= 1");
+    if (!r->f32) return SLAB_OK; // the FP64 form re-sorts its storage every step
+    if (!r->eng) return fail(SLAB_ESTATE, "no particles uploaded");
+    std::string why;
+    return engine_set_reorder(r->eng, steps, &why) ? SLAB_OK : fail(SLAB_EINVAL, why);
+}
+
 slab_status slab_batch_timing(slab_batch* b, int32_t enable, double* ms, int64_t* counts, int32_t n) {
     if (!b) return fail(SLAB_EINVAL, "null batch");
     engine_timing(b->eng, enable, ms, counts, n);

@@ -3,8 +3,8 @@
 //
 // Step launch sequence (fixed grid sizes; every kernel reads its counts from
 // device memory, so a step can be replayed from a captured hipGraph):
-//   begin -> tile_reduce -> tile_scan -> tile_offsets -> scatter -> tile_sort
-//   -> p2g -> grid -> g2p -> end
+//   begin -> tile_reduce -> tile_scan -> tile_offsets -> scatter
+//   -> sort + p2g (fused) -> grid -> g2p -> end
 // Positions are a packed floor cell (ci << 16 | cj) plus FP32 offsets in cell
 // units; h*A = m / rho (A is not stored); time and dt are FP64 per slot.
 // Multiply-add contraction is allowed in this translation unit.
@@ -123,6 +123,7 @@ __device__ __forceinline__ void publish_vmax(int slot, bool valid, float v2, uns
                 if (s_slot[k] >= 0 && s_uni[k]) atomicMax(&vmx[(size_t)s_slot[k] * VMAX_SHARDS + sh], s_max[k]);
         }
     }
+    __syncthreads(); // the LDS slots are reused by the next call
 }
 
 // Tile slots for arbitrary valid masks: one atomic per run of equal tiles
@@ -168,11 +169,12 @@ __global__ void __launch_bounds__(WG) k_ebegin(ECtl* c, Slot* __restrict__ slots
         if (threadIdx.x == 0) {
             c->epoch += 1;
             c->p2g_parity = c->parity;
+            c->reorder = 1;
         }
         return;
     }
-    __shared__ int s_act;
-    if (threadIdx.x == 0) s_act = 0;
+    __shared__ int s_act, s_out;
+    if (threadIdx.x == 0) s_act = s_out = 0;
     __syncthreads();
     const int ep = c->epoch + 1;
     int act = 0;
@@ -217,7 +219,10 @@ __global__ void __launch_bounds__(WG) k_ebegin(ECtl* c, Slot* __restrict__ slots
         const double tnew = S.time + dt;
         const int hit = (S.output_dt * (double)(S.frame + 1) - tnew < 1e-13);
         S.out_step = hit || S.keep_grid;
-        if (S.out_step) S.last_out_epoch = ep;
+        if (S.out_step) {
+            S.last_out_epoch = ep;
+            s_out = 1;
+        }
         ++act;
     }
     if (act) atomicAdd(&s_act, act);
@@ -226,6 +231,9 @@ __global__ void __launch_bounds__(WG) k_ebegin(ECtl* c, Slot* __restrict__ slots
         c->epoch = ep;
         c->p2g_parity = c->parity;
         c->n_active = s_act;
+        // a single run keeps the pre-G2P positions of output steps (its remesh
+        // window is read from them), so those steps write the other buffer
+        c->reorder = c->reorder_every <= 1 || ep % c->reorder_every == 0 || (c->single && s_out);
         if (s_act == 0) c->stop = STOP_DONE; // nothing to advance
     }
 }
@@ -238,6 +246,7 @@ __global__ void __launch_bounds__(WG) k_eend(ECtl* c, Slot* __restrict__ slots) 
             c->parity ^= 1;
             c->n_store = c->n_sorted;
             c->compact = 0;
+            c->reorder = 0;
         }
         return;
     }
@@ -261,8 +270,10 @@ __global__ void __launch_bounds__(WG) k_eend(ECtl* c, Slot* __restrict__ slots) 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        c->parity ^= 1;
-        c->n_store = c->n_sorted;
+        if (c->reorder) {
+            c->parity ^= 1;
+            c->n_store = c->n_sorted;
+        }
         if (c->n_active > 0) c->steps_in_call += 1;
         if (c->single && s_hit) c->stop = STOP_FRAME;
     }
@@ -443,165 +454,244 @@ __global__ void k_escatter(ECtl* c, const int* __restrict__ key, const int* __re
     tmp[tile_begin[k >> 8] + tslot[s]] = s;
 }
 
-// Per particle tile: counting sort by cell in LDS, then each cell's run sorted
-// by creation index: canonical order (slot, tile, cell, creation index).
-// Parking tiles keep their bucket order (their particles are only carried).
-__global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int* __restrict__ key,
-                                              const int* __restrict__ ptiles, int* __restrict__ tile_count,
-                                              const int* __restrict__ tile_begin, const int* __restrict__ tmp,
-                                              int* __restrict__ idx, int* __restrict__ cell_begin) {
-    if (c->stop) return;
-    __shared__ int s_hist[TILE_CELLS];
-    __shared__ int s_off[TILE_CELLS];
-    __shared__ int s_w[4];
-    __shared__ int s_n;
-    const int32_t* __restrict__ pid = ecur(c, A, B).pid;
-    const int nt = c->n_ptiles, live = c->nslots * c->ntd;
-    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
-        const int t = ptiles[w];
-        if (threadIdx.x == 0) s_n = tile_count[t];
-        s_hist[threadIdx.x] = 0;
-        __syncthreads();
-        const int n = s_n;
-        const int b = tile_begin[t];
-        int* cb = cell_begin + (size_t)w * CELL_STRIDE;
-        if (t >= live) { // parking tile
-            for (int r = threadIdx.x; r < n; r += WG) idx[b + r] = tmp[b + r];
-            cb[threadIdx.x] = threadIdx.x == 0 ? b : b + n;
-            if (threadIdx.x == 0) {
-                cb[TILE_CELLS] = b + n;
-                tile_count[t] = 0;
+// Sort + P2G, one workgroup per listed particle tile.
+//
+// Sort: the tile's particles (bucketed by tile in arrival order) are put in
+// canonical order (cell, then creation index) -- a counting sort by cell and
+// an insertion sort of each cell's run by creation index -- and written to
+// idx, the order G2P walks.  Parking tiles keep their bucket order (their
+// particles are only carried).
+//
+// P2G (src/simulation.cpp:726-798) as a per-tile segmented reduction without
+// atomics: one thread per stencil-base cell sums its particles (canonical
+// order) into 9 stencil slots x 7 fields in registers; 9 phases then add slot
+// (d,e) of every cell into node cell+(d,e) of an 18x18-node LDS region
+// (one-to-one within a phase: no conflicts, fixed order); the region is this
+// tile's partial, merged per node by k_egrid.
+//
+// Tiles of at most SP_CAP particles sort in LDS: one coalesced pass stages
+// (cell, creation index) of every particle and the sort permutes LDS indices;
+// P2G then stages the particle fields in canonical order, SP_CHUNK particles
+// at a time (coalesced loads), and every cell thread reads its particles of
+// the chunk from LDS.  Larger tiles sort through global memory and gather the
+// particle fields per cell.
+constexpr int SP_CAP = 2048;   // particles per tile sorted in LDS
+constexpr int SP_CHUNK = 512;  // particles staged per P2G pass
+constexpr int SP_FIELDS = 7;   // ox, oy, vx, vy, vz, h, m
+
+struct SortP2GShared {
+    int hist[TILE_CELLS];
+    int off[TILE_CELLS];
+    int wsum[4];
+    float acc[P2G_FIELDS * REGION_NODES];
+    float f[SP_FIELDS][SP_CHUNK];
+    int pid[SP_CAP];
+    unsigned short ord[SP_CAP];
+    unsigned char cell[SP_CAP];
+};
+
+// Exclusive scan of hist[] (one entry per thread) into off[]; returns the
+// thread's exclusive offset.
+__device__ __forceinline__ int cell_scan(SortP2GShared& sh) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int v = sh.hist[threadIdx.x];
+    int inc = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+    }
+    if (lane == 63) sh.wsum[wv] = inc;
+    __syncthreads();
+    int pre = 0;
+    for (int q = 0; q < wv; ++q) pre += sh.wsum[q];
+    const int ex = pre + inc - v;
+    sh.off[threadIdx.x] = ex;
+    __syncthreads();
+    return ex;
+}
+
+// P2G contribution of one particle to the 3x3 x 7 register accumulator.
+// fx, fy: position relative to the stencil base node in cell units.
+__device__ __forceinline__ void p2g_add(float (&acc)[9][P2G_FIELDS], float fx, float fy, float inv, float mp,
+                                        float hp, float vx, float vy, float vz, float g, float cth, float gx,
+                                        float gy) {
+    const float coeff = -mp * g * cth * cth;
+    const float vmx = vx * mp, vmy = vy * mp, vzm = vz * mp, hm = hp * mp;
+    const float mbx = coeff * gx, mby = coeff * gy;
+    // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
+    const float ps = -0.5f * mp * g * cth * cth * hp;
+    float wx[3], wy[3], dwx[3], dwy[3];
+    weights3f(fx, inv, wx, dwx);
+    weights3f(fy, inv, wy, dwy);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const float wgt = wx[d] * wy[e];
+            if (wgt > 3e-3f) { // simulation.cpp:780
+                float* a = acc[d * 3 + e];
+                a[0] += mp * wgt;
+                a[1] += vmx * wgt;
+                a[2] += vmy * wgt;
+                a[3] += vzm * wgt;
+                a[4] += hm * wgt;
+                a[5] += mbx * wgt - ps * (dwx[d] * wy[e]);
+                a[6] += mby * wgt - ps * (dwy[e] * wx[d]);
             }
-            __syncthreads();
-            continue;
         }
-        for (int r = threadIdx.x; r < n; r += WG) atomicAdd(&s_hist[key[tmp[b + r]] & 255], 1);
-        __syncthreads();
-        {
-            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-            const int v = s_hist[threadIdx.x];
-            int inc = v;
-            for (int o = 1; o < 64; o <<= 1) {
-                int u = __shfl_up(inc, o, 64);
-                if (lane >= o) inc += u;
-            }
-            if (lane == 63) s_w[wv] = inc;
-            __syncthreads();
-            int pre = 0;
-            for (int q = 0; q < wv; ++q) pre += s_w[q];
-            s_off[threadIdx.x] = pre + inc - v;
-            cb[threadIdx.x] = b + pre + inc - v;
-            if (threadIdx.x == 0) {
-                cb[TILE_CELLS] = b + n;
-                tile_count[t] = 0;
-            }
-        }
-        __syncthreads();
-        s_hist[threadIdx.x] = s_off[threadIdx.x];
-        __syncthreads();
-        for (int r = threadIdx.x; r < n; r += WG) {
-            const int s = tmp[b + r];
-            const int pos = atomicAdd(&s_hist[key[s] & 255], 1);
-            idx[b + pos] = s;
-        }
-        __syncthreads();
-        {
-            const int lo = b + s_off[threadIdx.x];
-            const int hi = b + s_hist[threadIdx.x];
-            for (int q = lo + 1; q < hi; ++q) {
-                const int sv = idx[q];
-                const int pv = pid[sv];
-                int k = q - 1;
-                while (k >= lo && pid[idx[k]] > pv) {
-                    idx[k + 1] = idx[k];
-                    --k;
-                }
-                idx[k + 1] = sv;
-            }
-        }
-        __syncthreads();
     }
 }
 
-// --- P2G -------------------------------------------------------------------
+// Adds every cell's 9 register slots into the 18x18 region in 9 conflict-free
+// phases and writes the region to the tile's partial.
+__device__ __forceinline__ void p2g_flush(SortP2GShared& sh, float (&acc)[9][P2G_FIELDS], float* __restrict__ dst) {
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+#pragma unroll
+    for (int ph = 0; ph < 9; ++ph) {
+        const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
+#pragma unroll
+        for (int f = 0; f < P2G_FIELDS; ++f) sh.acc[f * REGION_NODES + node] += acc[ph][f];
+        __syncthreads();
+    }
+    for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) dst[q] = sh.acc[q];
+}
 
-// P2G, src/simulation.cpp:726-798, as a per-tile segmented reduction without
-// atomics: one thread per stencil-base cell sums its particles (canonical
-// order) into 9 stencil slots x 7 fields in registers; 9 phases then add slot
-// (d,e) of every cell into node cell+(d,e) of an 18x18-node LDS region (one-to
-// -one within a phase: no conflicts, fixed order); the region is this tile's
-// partial, merged per node by k_egrid.
-__global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
-                                             const int* __restrict__ idx, const int* __restrict__ ptiles,
-                                             const int* __restrict__ cell_begin, const float* __restrict__ tgx,
-                                             const float* __restrict__ tgy, const float* __restrict__ tcth,
-                                             float* __restrict__ tpart) {
-    if (c->stop || c->compact) return;
-    __shared__ float s_acc[P2G_FIELDS * REGION_NODES];
+__global__ void __launch_bounds__(WG) k_esortp2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+                                                 const int* __restrict__ key, const int* __restrict__ ptiles,
+                                                 int* __restrict__ tile_count, const int* __restrict__ tile_begin,
+                                                 const int* __restrict__ tmp, int* __restrict__ idx,
+                                                 int* __restrict__ cell_begin, const float* __restrict__ tgx,
+                                                 const float* __restrict__ tgy, const float* __restrict__ tcth,
+                                                 float* __restrict__ tpart) {
+    if (c->stop) return;
+    __shared__ SortP2GShared sh;
     const PB32 P = ecur(c, A, B);
     const int nt = c->n_ptiles, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
     const float inv = (float)c->inv_dx;
-    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const int tid = threadIdx.x, lx = tid >> 4, ly = tid & 15;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         const int t = ptiles[w];
-        if (t >= live) continue;
+        const int n = tile_count[t];
+        const int b = tile_begin[t];
+        __syncthreads(); // every thread has read tile_count[t] before it is reset
+        if (tid == 0) tile_count[t] = 0;
+        if (t >= live) { // parking tile
+            for (int r = tid; r < n; r += WG) idx[b + r] = tmp[b + r];
+            continue;
+        }
         const int slot = t / ntd, local = t - slot * ntd;
-        if (!slot_active(slots[slot].status)) continue;
+        const bool do_p2g = slot_active(slots[slot].status) && !c->compact;
         const float g = (float)slots[slot].gravity;
         const int tx = local / nty, ty = local - tx * nty;
         const int ib = tx * TILE + lx, jb = ty * TILE + ly;
-        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
         float acc[9][P2G_FIELDS];
 #pragma unroll
         for (int k = 0; k < 9; ++k)
 #pragma unroll
             for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
-        const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
-        const int r0 = cb[threadIdx.x], r1 = cb[threadIdx.x + 1];
-        for (int r = r0; r < r1; ++r) {
-            const int s = idx[r];
-            const int cell = P.cell[s];
-            const int ci = cell >> 16, cj = cell & 0xffff;
-            const size_t fc = (size_t)ci * ny + cj;
-            const float fx = (float)(ci - ib) + P.ox[s], fy = (float)(cj - jb) + P.oy[s];
-            const float mp = P.m[s], hp = P.h[s];
-            const float cth = tcth[fc];
-            const float coeff = -mp * g * cth * cth;
-            const float vmx = P.vx[s] * mp, vmy = P.vy[s] * mp, vzm = P.vz[s] * mp, hm = hp * mp;
-            const float mbx = coeff * tgx[fc], mby = coeff * tgy[fc];
-            // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
-            const float ps = -0.5f * mp * g * cth * cth * hp;
-            float wx[3], wy[3], dwx[3], dwy[3];
-            weights3f(fx, inv, wx, dwx);
-            weights3f(fy, inv, wy, dwy);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-#pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    const float wgt = wx[d] * wy[e];
-                    if (wgt > 3e-3f) { // simulation.cpp:780
-                        float* a = acc[d * 3 + e];
-                        a[0] += mp * wgt;
-                        a[1] += vmx * wgt;
-                        a[2] += vmy * wgt;
-                        a[3] += vzm * wgt;
-                        a[4] += hm * wgt;
-                        a[5] += mbx * wgt - ps * (dwx[d] * wy[e]);
-                        a[6] += mby * wgt - ps * (dwy[e] * wx[d]);
+        sh.hist[tid] = 0;
+        for (int q = tid; q < P2G_FIELDS * REGION_NODES; q += WG) sh.acc[q] = 0.f;
+        __syncthreads();
+        if (n <= SP_CAP) {
+            // stage (cell, creation index): one coalesced pass over the bucket
+            for (int r = tid; r < n; r += WG) {
+                const int s = tmp[b + r];
+                const int cc = key[s] & 255;
+                sh.cell[r] = (unsigned char)cc;
+                sh.pid[r] = P.pid[s];
+                atomicAdd(&sh.hist[cc], 1);
+            }
+            __syncthreads();
+            const int cnt = sh.hist[tid];
+            const int off = cell_scan(sh);
+            sh.hist[tid] = off; // cursor
+            __syncthreads();
+            for (int r = tid; r < n; r += WG) sh.ord[atomicAdd(&sh.hist[sh.cell[r]], 1)] = (unsigned short)r;
+            __syncthreads();
+            // canonical order inside the cell: creation index
+            for (int q = off + 1; q < off + cnt; ++q) {
+                const unsigned short rv = sh.ord[q];
+                const int pv = sh.pid[rv];
+                int k = q - 1;
+                while (k >= off && sh.pid[sh.ord[k]] > pv) {
+                    sh.ord[k + 1] = sh.ord[k];
+                    --k;
+                }
+                sh.ord[k + 1] = rv;
+            }
+            __syncthreads();
+            if (!do_p2g) {
+                for (int q = tid; q < n; q += WG) idx[b + q] = tmp[b + sh.ord[q]];
+            } else {
+                for (int q0 = 0; q0 < n; q0 += SP_CHUNK) {
+                    const int qn = min(SP_CHUNK, n - q0);
+                    for (int k = tid; k < qn; k += WG) {
+                        const int s = tmp[b + sh.ord[q0 + k]];
+                        idx[b + q0 + k] = s;
+                        sh.f[0][k] = P.ox[s];
+                        sh.f[1][k] = P.oy[s];
+                        sh.f[2][k] = P.vx[s];
+                        sh.f[3][k] = P.vy[s];
+                        sh.f[4][k] = P.vz[s];
+                        sh.f[5][k] = P.h[s];
+                        sh.f[6][k] = P.m[s];
                     }
+                    __syncthreads();
+                    const int lo = max(off, q0), hi = min(off + cnt, q0 + qn);
+                    for (int q = lo; q < hi; ++q) {
+                        const int k = q - q0;
+                        const float ox = sh.f[0][k], oy = sh.f[1][k];
+                        // live particles sit in cells [1, N-3], so the floor cell
+                        // is the stencil base plus (offset < 1/2)
+                        const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
+                        const size_t fc = (size_t)(ib + ax) * ny + (jb + ay);
+                        p2g_add(acc, (float)ax + ox, (float)ay + oy, inv, sh.f[6][k], sh.f[5][k], sh.f[2][k],
+                                sh.f[3][k], sh.f[4][k], g, tcth[fc], tgx[fc], tgy[fc]);
+                    }
+                    __syncthreads();
+                }
+            }
+        } else {
+            // large tile: counting sort through global memory
+            int* cb = cell_begin + (size_t)w * CELL_STRIDE;
+            for (int r = tid; r < n; r += WG) atomicAdd(&sh.hist[key[tmp[b + r]] & 255], 1);
+            __syncthreads();
+            const int cnt = sh.hist[tid];
+            const int off = cell_scan(sh);
+            cb[tid] = b + off;
+            if (tid == 0) cb[TILE_CELLS] = b + n;
+            sh.hist[tid] = off;
+            __syncthreads();
+            for (int r = tid; r < n; r += WG) {
+                const int s = tmp[b + r];
+                idx[b + atomicAdd(&sh.hist[key[s] & 255], 1)] = s;
+            }
+            __syncthreads();
+            const int lo = b + off, hi = b + off + cnt;
+            for (int q = lo + 1; q < hi; ++q) {
+                const int sv = idx[q];
+                const int pv = P.pid[sv];
+                int k = q - 1;
+                while (k >= lo && P.pid[idx[k]] > pv) {
+                    idx[k + 1] = idx[k];
+                    --k;
+                }
+                idx[k + 1] = sv;
+            }
+            __syncthreads();
+            if (do_p2g) {
+                for (int q = lo; q < hi; ++q) {
+                    const int s = idx[q];
+                    const float ox = P.ox[s], oy = P.oy[s];
+                    const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
+                    const size_t fc = (size_t)(ib + ax) * ny + (jb + ay);
+                    p2g_add(acc, (float)ax + ox, (float)ay + oy, inv, P.m[s], P.h[s], P.vx[s], P.vy[s], P.vz[s], g,
+                            tcth[fc], tgx[fc], tgy[fc]);
                 }
             }
         }
         __syncthreads();
-#pragma unroll
-        for (int ph = 0; ph < 9; ++ph) {
-            const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
-#pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) s_acc[f * REGION_NODES + node] += acc[ph][f];
-            __syncthreads();
-        }
-        float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
-        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) dst[q] = s_acc[q];
+        if (do_p2g) p2g_flush(sh, acc, tpart + (size_t)w * (P2G_FIELDS * REGION_NODES));
         __syncthreads();
     }
 }
@@ -709,8 +799,11 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
 // --- G2P -------------------------------------------------------------------
 
 // G2P (src/simulation.cpp:877-940) for running slots; priming / finished
-// slots are carried unchanged.  Writes the other buffer in sorted order, the
-// next keys and tile slots, and the per-slot max |v|^2.
+// slots are carried unchanged.  Walks the particles in canonical order; on a
+// re-sort step writes them to the other buffer in that order, otherwise
+// updates them in place (the constant mass and creation index are not
+// rewritten).  Also writes the next keys and tile slots (indexed like the
+// storage) and the per-slot max |v|^2.
 __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                              const int* __restrict__ idx, const float4* __restrict__ vf,
                                              const float* __restrict__ tz, const float* __restrict__ tgx,
@@ -719,23 +812,27 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
                                              unsigned long long* __restrict__ vmx) {
     if (c->stop) return;
     const PB32 P = ecur(c, A, B);
-    const PB32 Q = enxt(c, A, B);
+    const bool reo = c->reorder != 0;
+    const PB32 Q = reo ? enxt(c, A, B) : P;
     const int r = blockIdx.x * WG + threadIdx.x;
     const bool in = r < c->n_sorted;
-    int t = 0, kk = -1, sl = 0;
+    int t = 0, kk = -1, sl = 0, o = 0;
     float v2 = 0.f;
     bool pub = false;
     if (in) {
         const int nx = c->nx, ny = c->ny;
         const size_t nn = (size_t)nx * ny;
         const int s = idx[r];
+        o = reo ? r : s; // storage index written
         const int raw = P.slot[s];
         sl = raw & SLOT_MASK;
         bool out = (raw & SLOT_OUT) != 0;
+        const bool was_out = out;
         const Slot& S = slots[sl];
         const int st = S.status;
         int cell = P.cell[s];
-        float ox = P.ox[s], oy = P.oy[s], vxp = P.vx[s], vyp = P.vy[s], vzp = P.vz[s], hp = P.h[s], zp = P.z[s];
+        float ox = P.ox[s], oy = P.oy[s], vxp = P.vx[s], vyp = P.vy[s], hp = P.h[s];
+        float vzp, zp; // recomputed by a running particle, carried otherwise
         if (st == SLOT_RUNNING && !c->compact && !out) {
             const float inv = (float)c->inv_dx, dt = (float)S.dt, fr = (float)S.flip_ratio;
             const int ci = cell >> 16, cj = cell & 0xffff;
@@ -791,26 +888,31 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
                 out = true;
                 vxp = vyp = vzp = 0.f;
             }
+        } else {
+            vzp = P.vz[s];
+            zp = P.z[s];
         }
-        Q.cell[r] = cell;
-        Q.ox[r] = ox;
-        Q.oy[r] = oy;
-        Q.vx[r] = vxp;
-        Q.vy[r] = vyp;
-        Q.vz[r] = vzp;
-        Q.h[r] = hp;
-        Q.z[r] = zp;
-        Q.m[r] = P.m[s];
-        Q.pid[r] = P.pid[s];
-        Q.slot[r] = sl | (out ? SLOT_OUT : 0);
+        Q.cell[o] = cell;
+        Q.ox[o] = ox;
+        Q.oy[o] = oy;
+        Q.vx[o] = vxp;
+        Q.vy[o] = vyp;
+        Q.vz[o] = vzp;
+        Q.h[o] = hp;
+        Q.z[o] = zp;
+        if (reo) {
+            Q.m[o] = P.m[s];
+            Q.pid[o] = P.pid[s];
+        }
+        if (reo || out != was_out) Q.slot[o] = sl | (out ? SLOT_OUT : 0);
         kk = ekey(c, st, sl, out, cell, ox, oy, &t);
         pub = slot_active(st) && !out;
         v2 = vxp * vxp + vyp * vyp;
     }
     const int q = wave_slot_any(t, in && kk >= 0, tile_count);
     if (in) {
-        key[r] = kk;
-        tslot[r] = q;
+        key[o] = kk;
+        tslot[o] = q;
     }
     publish_vmax(sl, pub, v2, vmx);
 }
@@ -934,6 +1036,7 @@ __global__ void __launch_bounds__(WG) k_ediag(const ECtl* c, PB32 A, PB32 B, int
 namespace {
 
 constexpr int kPersistE = 2048;
+constexpr int kReorderEvery = 16; // default storage re-sort period (steps)
 constexpr int kNumK = 10;
 
 bool hip_ok(hipError_t e, const char* what, std::string* err) {
@@ -1054,13 +1157,10 @@ void enqueue_estep(Engine32* e, int step) {
                                            e->pstamp, e->gstamp, e->tile_pos);
     });
     tk(e, 4, step, [&] { k_escatter<<<capblocks(e), WG, 0, s>>>(c, e->key, e->tslot, e->tile_begin, e->tmp); });
-    tk(e, 5, step, [&] {
-        k_esort<<<pb, WG, 0, s>>>(c, e->A, e->B, e->key, e->ptiles, e->tile_count, e->tile_begin, e->tmp, e->idx,
-                                  e->cell_begin);
-    });
+    // sort and P2G run fused (timed under "p2g")
     tk(e, 6, step, [&] {
-        k_ep2g<<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
-                                 e->tpart);
+        k_esortp2g<<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->key, e->ptiles, e->tile_count, e->tile_begin,
+                                     e->tmp, e->idx, e->cell_begin, e->tgx, e->tgy, e->tcth, e->tpart);
     });
     tk(e, 7, step, [&] {
         k_egrid<<<tb, WG, 0, s>>>(c, e->d_slots, e->gtiles, e->pstamp, e->tile_pos, e->tpart, e->tgx, e->tgy, e->tcth,
@@ -1248,6 +1348,7 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     c.single = single ? 1 : 0;
     c.has_out = single ? 1 : 0;
     c.tile_cap = e->tile_cap;
+    c.reorder_every = kReorderEvery;
     c.dx = dx;
     c.inv_dx = e->inv;
     e->admit_base.assign(nslots, -1);
@@ -1637,6 +1738,16 @@ int engine_use_graph(Engine32* e, int on) {
     e->use_graph = on != 0;
     if (!on) drop_graph(e);
     return 0;
+}
+
+bool engine_set_reorder(Engine32* e, int k, std::string* err) {
+    if (k < 1 || k > (1 << 20)) {
+        if (err) *err = "re-sort period must be in [1, 2^20]";
+        return false;
+    }
+    if (!engine_pull(e, err)) return false;
+    e->h_c->reorder_every = k;
+    return push_all(e, err);
 }
 
 } // namespace slab

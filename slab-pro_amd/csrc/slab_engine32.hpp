@@ -53,7 +53,10 @@ struct ECtl {
     int32_t epoch, parity, p2g_parity, stop;
     int32_t err, n_store, n_sorted, n_ptiles;
     int32_t n_gtiles, tile_cap, steps_in_call, n_active;
-    int32_t compact, pad0, pad1, pad2; // compact: this pass only re-sorts (drops released particles)
+    int32_t compact;       // this pass only re-sorts (drops released particles)
+    int32_t reorder;       // this step's G2P writes the other buffer in sorted order
+    int32_t reorder_every; // storage re-sort period in steps (other steps update in place)
+    int32_t pad0;
     double dx, inv_dx;
 };
 
@@ -91,5 +94,9 @@ int64_t engine_particle_bytes();                      // device bytes per partic
 int64_t engine_slot_bytes(const Engine32* e);         // device bytes per slot (grid + max maps)
 int64_t engine_free_particles(const Engine32* e);     // capacity left for admissions
 int engine_use_graph(Engine32* e, int on);             // capture the step sequence in a hipGraph
+// Storage re-sort period: every k-th step G2P writes the particles in sorted
+// order to the other buffer (dropping released ones); other steps update them
+// in place.  Results do not depend on k (the computation order is canonical).
+bool engine_set_reorder(Engine32* e, int k, std::string* err);
 
 } // namespace slab

@@ -98,6 +98,8 @@ SIGNATURES = {
     "slab_batch_use_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "slab_batch_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _DP, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
     "slab_batch_destroy": (None, [_VP]),
+    "slab_batch_set_resort": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "slab_run_set_resort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     # include/slab_frontend.h
     "slab_sim_load_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
     "slab_sim_initialize": (ctypes.c_int, [_VP]),
@@ -280,6 +282,9 @@ class Run:
     def set_velocity(self, vx, vy):
         _check(lib().slab_run_set_velocity(self.h, _p(_f64(vx)), _p(_f64(vy))))
 
+    def set_resort(self, steps):
+        _check(lib().slab_run_set_resort(self.h, int(steps)))
+
     def set_time(self, t, frame):
         _check(lib().slab_run_set_time(self.h, float(t), int(frame)))
 
@@ -360,6 +365,9 @@ class Batch:
 
     def use_graph(self, on=True):
         _check(lib().slab_batch_use_graph(self.h, 1 if on else 0))
+
+    def set_resort(self, steps):
+        _check(lib().slab_batch_set_resort(self.h, int(steps)))
 
     def timing(self, enable):
         n = lib().slab_kernel_count()

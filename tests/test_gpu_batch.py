@@ -42,10 +42,12 @@ def _params(total=3.0):
     return abi.params(rho=300.0, total_time=total, output_dt=0.5, precision=abi.FP32)
 
 
-def _run_batch(scens, max_slots=8, cap=200000, steps=100000, total=3.0):
+def _run_batch(scens, max_slots=8, cap=200000, steps=100000, total=3.0, resort=None):
     from slabpro import abi
     z, dx = _terrain()
     b = abi.Batch(z, dx, max_slots, cap)
+    if resort is not None:
+        b.set_resort(resort)
     slots = []
     for part, fr in scens:
         s = b.admit(_params(total), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
@@ -200,3 +202,17 @@ def test_outflow_particles_are_parked_and_isolated():
     for k in ("x", "y", "vx", "vy", "h"):
         assert np.array_equal(pi_[k], pa[k]), k
     assert np.array_equal(hm_i[0], hm_a[0])
+
+
+@pytest.mark.parametrize("period", [1, 5])
+def test_resort_period_does_not_change_batch_results(period):
+    """Every scenario of a mix is bitwise the same with storage re-sorted every
+    step and every `period` steps (default 16)."""
+    scens = [_scenario(3, FRICS[0]), _scenario(4, FRICS[1]), _scenario(5, FRICS[2])]
+    a = _run_batch(scens, resort=period)
+    b = _run_batch(scens)
+    for (ma, pa, ia), (mb, pb, ib) in zip(a, b):
+        assert ia["step"] == ib["step"]
+        for k in ("x", "y", "vx", "vy", "h", "z"):
+            assert np.array_equal(pa[k], pb[k]), k
+        assert np.array_equal(ma[0], mb[0]) and np.array_equal(ma[1], mb[1])

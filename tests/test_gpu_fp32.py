@@ -91,23 +91,33 @@ def test_fp32_bitwise_reproducible():
     assert np.array_equal(outs[0][1][0], outs[1][1][0])
 
 
-def error_report(h32, v32, p32, h64, v64, p64, dx, x_foot=None):
-    """Max-height map L1 / L-inf, max-velocity map L1, runout, deposited mass."""
-    w = np.abs(h64).sum()
-    rep = {
-        "hmax_L1": float(np.abs(h32 - h64).sum() / w),
-        "hmax_Linf": float(np.abs(h32 - h64).max() / np.abs(h64).max()),
-        "vmax_L1": float(np.abs(np.sqrt(v32) - np.sqrt(v64)).sum() / np.sqrt(v64).sum()),
-        "runout_x_fp32": float(p32["x"].max()),
-        "runout_x_fp64": float(p64["x"].max()),
-        "mass_fp32": float(np.sum(p32["m"])),
-        "mass_fp64": float(np.sum(p64["m"])),
-    }
-    rep["runout_diff_cells"] = abs(rep["runout_x_fp32"] - rep["runout_x_fp64"]) / dx
-    if x_foot is not None:
-        rep["deposit_frac_fp32"] = float(np.sum(p32["m"][p32["x"] > x_foot]) / np.sum(p32["m"]))
-        rep["deposit_frac_fp64"] = float(np.sum(p64["m"][p64["x"] > x_foot]) / np.sum(p64["m"]))
-    return rep
+@pytest.mark.parametrize("period", [1, 3, 16])
+def test_fp32_storage_resort_period_does_not_change_results(period):
+    """In-place steps and re-sort steps give the same numbers (canonical order),
+    also across output frames where the grid window is read."""
+    from slabpro import configs
+    cfg = configs.c0()
+    res = []
+    for k in (1, period):
+        r = _mk(cfg, 32, keep=False)
+        r.set_resort(k)
+        r.prime()
+        frames = []
+        for _ in range(3):                      # three output frames (1 s apart)
+            info = r.step(1 << 20)
+            assert info["frame_hit"]
+            frames.append(r.grid())
+        res.append((r.particles(), r.max_maps(), frames, info["step"]))
+        r.close()
+    (p0, m0, f0, s0), (p1, m1, f1, s1) = res
+    assert s0 == s1
+    for k in p0:
+        assert np.array_equal(p0[k], p1[k]), k
+    assert np.array_equal(m0[0], m1[0]) and np.array_equal(m0[1], m1[1])
+    for a, b in zip(f0, f1):
+        assert a["window"] == b["window"]
+        for k in ("mass", "h", "vx", "vy", "vz"):
+            assert np.array_equal(a[k], b[k]), k
 
 
 def test_fp32_c0_full_run_error_report():
@@ -124,6 +134,7 @@ def test_fp32_c0_full_run_error_report():
         r.close()
     assert out[32][3]["step"] == out[64][3]["step"]
     x_foot = 600.0 + 300.0 * math.sin(math.radians(35.0))
+    from slabpro.accuracy import error_report
     rep = error_report(out[32][0], out[32][1], out[32][2], out[64][0], out[64][1], out[64][2], cfg["dx"], x_foot)
     print("c0 fp32-vs-fp64 error report", rep, flush=True)
     assert rep["hmax_L1"] <= 0.01

@@ -2,9 +2,12 @@
 
 Workload (default): 8000x8000-node synthetic DEM at 1 m, 32-vertex star polygon
 release (mean radius 800 m, h0 = 4 m, ~8M UNIFORM particles), mu(I) friction
-(mu1 0.38, mu2 0.64, I0 0.279, d 0.5 m, phi 0.6), rho 2000.  A "step" is one
-pass of the step loop over all particles (sort, P2G + grid update, G2P, dt).
-Inputs are resident in device memory before the timed region starts.
+(mu1 0.38, mu2 0.64, I0 0.279, d 0.5 m, phi 0.6), rho 2000, 120 s.  A "step"
+is one pass of the step loop over all particles (sort, P2G + grid update,
+G2P, dt).  Inputs are resident in device memory before the timed region
+starts; the default timed region is 10,000 consecutive steps (most of the
+run: the cost per step grows as the flow piles up), per-kernel event times
+come from the 100 steps after it.
 
 Prints ONE JSON line.  roofline: dominant kernel's algorithmic bytes per launch
 / its mean HIP-event duration vs the 8 TB/s HBM peak; roofline.traffic: that
@@ -41,9 +44,8 @@ ALG_BYTES_FP64 = {
 }
 STEP_BYTES_FP64 = {"particle": 104, "node": 120}
 # FP32 form: particle = cell int32 + offsets + vx vy vz h m (32 B read, 28 B
-# written by G2P); "p2g" = the fused per-tile sort + LDS reduction of the
-# particle state (sort traffic excluded), "grid" = partial merge + grid update
-# (terrain 12 B, max maps 16 B, vf 16 B).
+# written by G2P); "p2g" = per-tile LDS reduction of the particle state,
+# "grid" = partial merge + grid update (terrain 12 B, max maps 16 B, vf 16 B).
 ALG_BYTES_FP32 = {
     "p2g": {"particle": 32, "node": 0},
     "grid": {"particle": 0, "node": 12 + 16 + 16},
@@ -55,7 +57,7 @@ STEP_BYTES_FP32 = {"particle": 60, "node": 60}
 
 # kernel names of the event timers -> rocprof kernel names, per form
 ROCPROF_NAMES = {
-    "fp32": {"g2p": "k_eg2p", "p2g": "k_esortp2g", "grid": "k_egrid", "scatter": "k_escatter"},
+    "fp32": {"g2p": "k_eg2p", "p2g": "k_ep2g", "grid": "k_egrid", "tile_sort": "k_esort", "scatter": "k_escatter"},
     "fp64": {"g2p": "k_g2p", "p2g": "k_payload", "grid": "k_p2g_grid", "tile_sort": "k_tile_sort",
              "scatter": "k_scatter"},
 }
@@ -107,11 +109,13 @@ def cpu_baseline(cfg, steps, warm):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=10000,
+                    help="timed steps (default: 10,000 of the C2 run's ~12,000, i.e. most of its 120 s)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2"])
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (FP32 form)")
     ap.add_argument("--form", default="fp32", choices=["fp32", "fp64"],
                     help="fp32: the FP32 form (int cell + FP32 offsets, FP32 bulk, FP64 time); fp64: plain form")
     ap.add_argument("--ensemble", action="store_true",
@@ -152,6 +156,12 @@ def main():
     log(f"device setup + prime {time.time() - t0:.1f}s")
     for _ in range(args.warmup):
         run.step(1)
+    tuned = None
+    if args.form == "fp32" and not args.no_tune:
+        # launch-shape autotuner: its trial steps are ordinary, untimed steps
+        t0 = time.time()
+        tuned = run.autotune(8)
+        log(f"autotune: {tuned} in {time.time() - t0:.1f}s")
 
     # timed region: K steps, inputs resident, no event instrumentation
     done = 0
@@ -165,8 +175,9 @@ def main():
     # per-kernel HIP-event durations on the run's stream, separate short region
     run.timing(True)
     ksteps = 0
-    while ksteps < min(args.steps, 20):
-        info = run.step(min(args.steps, 20) - ksteps)
+    kwant = min(args.steps, 100)
+    while ksteps < kwant:
+        info = run.step(kwant - ksteps)
         ksteps += info["steps_done"]
         if info["finished"]:
             break
@@ -215,7 +226,8 @@ def main():
             "c0": "601x151 nodes at 2 m, analytical 35deg plane+arc+flat, 4,800 particles, Coulomb",
             "c1": "2000x2000 DEM at 5 m, raster release of 40 ellipses, Voellmy",
             "c2": "8000x8000 DEM at 1 m, 32-vertex star polygon r=800 m h0=4 m, mu(I)"}[args.workload],
-            "particles": npart, "grid": f"{nx}x{ny}", "form": args.form, "parallelism": "single GPU"},
+            "particles": npart, "grid": f"{nx}x{ny}", "form": args.form, "parallelism": "single GPU",
+            "autotuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": avg_s * 1e3,

@@ -207,6 +207,14 @@ slab_status slab_batch_set_resort(slab_batch* b, int32_t steps);
 /* The same for a single FP32 run (after slab_run_upload_particles); a no-op
  * for the FP64 form, which re-sorts its storage every step. */
 slab_status slab_run_set_resort(slab_run* run, int32_t steps);
+/* Autotuner of the FP32 form: times candidate launch shapes -- storage
+ * re-sort period {1, 4, 16, 64}, then workgroups of the tile-list kernels
+ * {1024, 2048, 4096, 8192} -- on the next steps of the live run (each trial
+ * runs max(trial_steps, 2 x period) ordinary steps; results do not depend on
+ * the shape) and keeps the fastest.  out[0] = period, out[1] = workgroups.  The run variant
+ * needs slab_run_prime first and is a no-op for the FP64 form. */
+slab_status slab_batch_autotune(slab_batch* b, int32_t trial_steps, int32_t out[2]);
+slab_status slab_run_autotune(slab_run* run, int32_t trial_steps, int32_t out[2]);
 slab_status slab_batch_timing(slab_batch* b, int32_t enable, double* ms, int64_t* counts, int32_t n);
 void slab_batch_destroy(slab_batch* b);
 

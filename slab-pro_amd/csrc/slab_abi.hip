@@ -959,6 +959,32 @@ slab_status slab_run_set_resort(slab_run* r, int32_t steps) {
     return engine_set_reorder(r->eng, steps, &why) ? SLAB_OK : fail(SLAB_EINVAL, why);
 }
 
+slab_status slab_batch_autotune(slab_batch* b, int32_t trial_steps, int32_t out[2]) {
+    if (!b || !out) return fail(SLAB_EINVAL, "null argument");
+    std::string why;
+    int o[2] = {0, 0};
+    if (!engine_autotune(b->eng, trial_steps, o, &why)) return fail(SLAB_EDEVICE, why);
+    out[0] = o[0];
+    out[1] = o[1];
+    return SLAB_OK;
+}
+
+slab_status slab_run_autotune(slab_run* r, int32_t trial_steps, int32_t out[2]) {
+    if (!r || !out) return fail(SLAB_EINVAL, "null argument");
+    if (!r->primed) return fail(SLAB_ESTATE, "call slab_run_prime first");
+    if (!r->f32) { // the FP64 form has one launch shape
+        out[0] = 1;
+        out[1] = 0;
+        return SLAB_OK;
+    }
+    std::string why;
+    int o[2] = {0, 0};
+    if (!engine_autotune(r->eng, trial_steps, o, &why)) return eng_fail(why);
+    out[0] = o[0];
+    out[1] = o[1];
+    return SLAB_OK;
+}
+
 slab_status slab_batch_timing(slab_batch* b, int32_t enable, double* ms, int64_t* counts, int32_t n) {
     if (!b) return fail(SLAB_EINVAL, "null batch");
     engine_timing(b->eng, enable, ms, counts, n);

@@ -3,8 +3,8 @@
 //
 // Step launch sequence (fixed grid sizes; every kernel reads its counts from
 // device memory, so a step can be replayed from a captured hipGraph):
-//   begin -> tile_reduce -> tile_scan -> tile_offsets -> scatter
-//   -> sort + p2g (fused) -> grid -> g2p -> end
+//   begin -> tile_reduce -> tile_scan -> tile_offsets -> scatter -> tile_sort
+//   -> p2g -> grid -> g2p -> end
 // Positions are a packed floor cell (ci << 16 | cj) plus FP32 offsets in cell
 // units; h*A = m / rho (A is not stored); time and dt are FP64 per slot.
 // Multiply-add contraction is allowed in this translation unit.
@@ -454,244 +454,165 @@ __global__ void k_escatter(ECtl* c, const int* __restrict__ key, const int* __re
     tmp[tile_begin[k >> 8] + tslot[s]] = s;
 }
 
-// Sort + P2G, one workgroup per listed particle tile.
-//
-// Sort: the tile's particles (bucketed by tile in arrival order) are put in
-// canonical order (cell, then creation index) -- a counting sort by cell and
-// an insertion sort of each cell's run by creation index -- and written to
-// idx, the order G2P walks.  Parking tiles keep their bucket order (their
-// particles are only carried).
-//
-// P2G (src/simulation.cpp:726-798) as a per-tile segmented reduction without
-// atomics: one thread per stencil-base cell sums its particles (canonical
-// order) into 9 stencil slots x 7 fields in registers; 9 phases then add slot
-// (d,e) of every cell into node cell+(d,e) of an 18x18-node LDS region
-// (one-to-one within a phase: no conflicts, fixed order); the region is this
-// tile's partial, merged per node by k_egrid.
-//
-// Tiles of at most SP_CAP particles sort in LDS: one coalesced pass stages
-// (cell, creation index) of every particle and the sort permutes LDS indices;
-// P2G then stages the particle fields in canonical order, SP_CHUNK particles
-// at a time (coalesced loads), and every cell thread reads its particles of
-// the chunk from LDS.  Larger tiles sort through global memory and gather the
-// particle fields per cell.
-constexpr int SP_CAP = 2048;   // particles per tile sorted in LDS
-constexpr int SP_CHUNK = 512;  // particles staged per P2G pass
-constexpr int SP_FIELDS = 7;   // ox, oy, vx, vy, vz, h, m
-
-struct SortP2GShared {
-    int hist[TILE_CELLS];
-    int off[TILE_CELLS];
-    int wsum[4];
-    float acc[P2G_FIELDS * REGION_NODES];
-    float f[SP_FIELDS][SP_CHUNK];
-    int pid[SP_CAP];
-    unsigned short ord[SP_CAP];
-    unsigned char cell[SP_CAP];
-};
-
-// Exclusive scan of hist[] (one entry per thread) into off[]; returns the
-// thread's exclusive offset.
-__device__ __forceinline__ int cell_scan(SortP2GShared& sh) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int v = sh.hist[threadIdx.x];
-    int inc = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int u = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += u;
-    }
-    if (lane == 63) sh.wsum[wv] = inc;
-    __syncthreads();
-    int pre = 0;
-    for (int q = 0; q < wv; ++q) pre += sh.wsum[q];
-    const int ex = pre + inc - v;
-    sh.off[threadIdx.x] = ex;
-    __syncthreads();
-    return ex;
-}
-
-// P2G contribution of one particle to the 3x3 x 7 register accumulator.
-// fx, fy: position relative to the stencil base node in cell units.
-__device__ __forceinline__ void p2g_add(float (&acc)[9][P2G_FIELDS], float fx, float fy, float inv, float mp,
-                                        float hp, float vx, float vy, float vz, float g, float cth, float gx,
-                                        float gy) {
-    const float coeff = -mp * g * cth * cth;
-    const float vmx = vx * mp, vmy = vy * mp, vzm = vz * mp, hm = hp * mp;
-    const float mbx = coeff * gx, mby = coeff * gy;
-    // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
-    const float ps = -0.5f * mp * g * cth * cth * hp;
-    float wx[3], wy[3], dwx[3], dwy[3];
-    weights3f(fx, inv, wx, dwx);
-    weights3f(fy, inv, wy, dwy);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-            const float wgt = wx[d] * wy[e];
-            if (wgt > 3e-3f) { // simulation.cpp:780
-                float* a = acc[d * 3 + e];
-                a[0] += mp * wgt;
-                a[1] += vmx * wgt;
-                a[2] += vmy * wgt;
-                a[3] += vzm * wgt;
-                a[4] += hm * wgt;
-                a[5] += mbx * wgt - ps * (dwx[d] * wy[e]);
-                a[6] += mby * wgt - ps * (dwy[e] * wx[d]);
-            }
-        }
-    }
-}
-
-// Adds every cell's 9 register slots into the 18x18 region in 9 conflict-free
-// phases and writes the region to the tile's partial.
-__device__ __forceinline__ void p2g_flush(SortP2GShared& sh, float (&acc)[9][P2G_FIELDS], float* __restrict__ dst) {
-    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
-#pragma unroll
-    for (int ph = 0; ph < 9; ++ph) {
-        const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
-#pragma unroll
-        for (int f = 0; f < P2G_FIELDS; ++f) sh.acc[f * REGION_NODES + node] += acc[ph][f];
-        __syncthreads();
-    }
-    for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) dst[q] = sh.acc[q];
-}
-
-__global__ void __launch_bounds__(WG) k_esortp2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
-                                                 const int* __restrict__ key, const int* __restrict__ ptiles,
-                                                 int* __restrict__ tile_count, const int* __restrict__ tile_begin,
-                                                 const int* __restrict__ tmp, int* __restrict__ idx,
-                                                 int* __restrict__ cell_begin, const float* __restrict__ tgx,
-                                                 const float* __restrict__ tgy, const float* __restrict__ tcth,
-                                                 float* __restrict__ tpart) {
+// Per particle tile: counting sort by cell in LDS, then each cell's run sorted
+// by creation index: canonical order (slot, tile, cell, creation index).
+// Parking tiles keep their bucket order (their particles are only carried).
+__global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int* __restrict__ key,
+                                              const int* __restrict__ ptiles, int* __restrict__ tile_count,
+                                              const int* __restrict__ tile_begin, const int* __restrict__ tmp,
+                                              int* __restrict__ idx, int* __restrict__ cell_begin) {
     if (c->stop) return;
-    __shared__ SortP2GShared sh;
-    const PB32 P = ecur(c, A, B);
-    const int nt = c->n_ptiles, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
-    const float inv = (float)c->inv_dx;
-    const int tid = threadIdx.x, lx = tid >> 4, ly = tid & 15;
+    __shared__ int s_hist[TILE_CELLS];
+    __shared__ int s_off[TILE_CELLS];
+    __shared__ int s_w[4];
+    __shared__ int s_n;
+    const int32_t* __restrict__ pid = ecur(c, A, B).pid;
+    const int nt = c->n_ptiles, live = c->nslots * c->ntd;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         const int t = ptiles[w];
-        const int n = tile_count[t];
+        if (threadIdx.x == 0) s_n = tile_count[t];
+        s_hist[threadIdx.x] = 0;
+        __syncthreads();
+        const int n = s_n;
         const int b = tile_begin[t];
-        __syncthreads(); // every thread has read tile_count[t] before it is reset
-        if (tid == 0) tile_count[t] = 0;
+        int* cb = cell_begin + (size_t)w * CELL_STRIDE;
         if (t >= live) { // parking tile
-            for (int r = tid; r < n; r += WG) idx[b + r] = tmp[b + r];
+            for (int r = threadIdx.x; r < n; r += WG) idx[b + r] = tmp[b + r];
+            cb[threadIdx.x] = threadIdx.x == 0 ? b : b + n;
+            if (threadIdx.x == 0) {
+                cb[TILE_CELLS] = b + n;
+                tile_count[t] = 0;
+            }
+            __syncthreads();
             continue;
         }
-        const int slot = t / ntd, local = t - slot * ntd;
-        const bool do_p2g = slot_active(slots[slot].status) && !c->compact;
-        const float g = (float)slots[slot].gravity;
-        const int tx = local / nty, ty = local - tx * nty;
-        const int ib = tx * TILE + lx, jb = ty * TILE + ly;
-        float acc[9][P2G_FIELDS];
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-#pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
-        sh.hist[tid] = 0;
-        for (int q = tid; q < P2G_FIELDS * REGION_NODES; q += WG) sh.acc[q] = 0.f;
+        for (int r = threadIdx.x; r < n; r += WG) atomicAdd(&s_hist[key[tmp[b + r]] & 255], 1);
         __syncthreads();
-        if (n <= SP_CAP) {
-            // stage (cell, creation index): one coalesced pass over the bucket
-            for (int r = tid; r < n; r += WG) {
-                const int s = tmp[b + r];
-                const int cc = key[s] & 255;
-                sh.cell[r] = (unsigned char)cc;
-                sh.pid[r] = P.pid[s];
-                atomicAdd(&sh.hist[cc], 1);
+        {
+            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+            const int v = s_hist[threadIdx.x];
+            int inc = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                int u = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += u;
             }
+            if (lane == 63) s_w[wv] = inc;
             __syncthreads();
-            const int cnt = sh.hist[tid];
-            const int off = cell_scan(sh);
-            sh.hist[tid] = off; // cursor
-            __syncthreads();
-            for (int r = tid; r < n; r += WG) sh.ord[atomicAdd(&sh.hist[sh.cell[r]], 1)] = (unsigned short)r;
-            __syncthreads();
-            // canonical order inside the cell: creation index
-            for (int q = off + 1; q < off + cnt; ++q) {
-                const unsigned short rv = sh.ord[q];
-                const int pv = sh.pid[rv];
-                int k = q - 1;
-                while (k >= off && sh.pid[sh.ord[k]] > pv) {
-                    sh.ord[k + 1] = sh.ord[k];
-                    --k;
-                }
-                sh.ord[k + 1] = rv;
+            int pre = 0;
+            for (int q = 0; q < wv; ++q) pre += s_w[q];
+            s_off[threadIdx.x] = pre + inc - v;
+            cb[threadIdx.x] = b + pre + inc - v;
+            if (threadIdx.x == 0) {
+                cb[TILE_CELLS] = b + n;
+                tile_count[t] = 0;
             }
-            __syncthreads();
-            if (!do_p2g) {
-                for (int q = tid; q < n; q += WG) idx[b + q] = tmp[b + sh.ord[q]];
-            } else {
-                for (int q0 = 0; q0 < n; q0 += SP_CHUNK) {
-                    const int qn = min(SP_CHUNK, n - q0);
-                    for (int k = tid; k < qn; k += WG) {
-                        const int s = tmp[b + sh.ord[q0 + k]];
-                        idx[b + q0 + k] = s;
-                        sh.f[0][k] = P.ox[s];
-                        sh.f[1][k] = P.oy[s];
-                        sh.f[2][k] = P.vx[s];
-                        sh.f[3][k] = P.vy[s];
-                        sh.f[4][k] = P.vz[s];
-                        sh.f[5][k] = P.h[s];
-                        sh.f[6][k] = P.m[s];
-                    }
-                    __syncthreads();
-                    const int lo = max(off, q0), hi = min(off + cnt, q0 + qn);
-                    for (int q = lo; q < hi; ++q) {
-                        const int k = q - q0;
-                        const float ox = sh.f[0][k], oy = sh.f[1][k];
-                        // live particles sit in cells [1, N-3], so the floor cell
-                        // is the stencil base plus (offset < 1/2)
-                        const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
-                        const size_t fc = (size_t)(ib + ax) * ny + (jb + ay);
-                        p2g_add(acc, (float)ax + ox, (float)ay + oy, inv, sh.f[6][k], sh.f[5][k], sh.f[2][k],
-                                sh.f[3][k], sh.f[4][k], g, tcth[fc], tgx[fc], tgy[fc]);
-                    }
-                    __syncthreads();
-                }
-            }
-        } else {
-            // large tile: counting sort through global memory
-            int* cb = cell_begin + (size_t)w * CELL_STRIDE;
-            for (int r = tid; r < n; r += WG) atomicAdd(&sh.hist[key[tmp[b + r]] & 255], 1);
-            __syncthreads();
-            const int cnt = sh.hist[tid];
-            const int off = cell_scan(sh);
-            cb[tid] = b + off;
-            if (tid == 0) cb[TILE_CELLS] = b + n;
-            sh.hist[tid] = off;
-            __syncthreads();
-            for (int r = tid; r < n; r += WG) {
-                const int s = tmp[b + r];
-                idx[b + atomicAdd(&sh.hist[key[s] & 255], 1)] = s;
-            }
-            __syncthreads();
-            const int lo = b + off, hi = b + off + cnt;
+        }
+        __syncthreads();
+        s_hist[threadIdx.x] = s_off[threadIdx.x];
+        __syncthreads();
+        for (int r = threadIdx.x; r < n; r += WG) {
+            const int s = tmp[b + r];
+            const int pos = atomicAdd(&s_hist[key[s] & 255], 1);
+            idx[b + pos] = s;
+        }
+        __syncthreads();
+        {
+            const int lo = b + s_off[threadIdx.x];
+            const int hi = b + s_hist[threadIdx.x];
             for (int q = lo + 1; q < hi; ++q) {
                 const int sv = idx[q];
-                const int pv = P.pid[sv];
+                const int pv = pid[sv];
                 int k = q - 1;
-                while (k >= lo && P.pid[idx[k]] > pv) {
+                while (k >= lo && pid[idx[k]] > pv) {
                     idx[k + 1] = idx[k];
                     --k;
                 }
                 idx[k + 1] = sv;
             }
-            __syncthreads();
-            if (do_p2g) {
-                for (int q = lo; q < hi; ++q) {
-                    const int s = idx[q];
-                    const float ox = P.ox[s], oy = P.oy[s];
-                    const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
-                    const size_t fc = (size_t)(ib + ax) * ny + (jb + ay);
-                    p2g_add(acc, (float)ax + ox, (float)ay + oy, inv, P.m[s], P.h[s], P.vx[s], P.vy[s], P.vz[s], g,
-                            tcth[fc], tgx[fc], tgy[fc]);
+        }
+        __syncthreads();
+    }
+}
+
+// --- P2G -------------------------------------------------------------------
+
+// P2G, src/simulation.cpp:726-798, as a per-tile segmented reduction without
+// atomics: one thread per stencil-base cell sums its particles (canonical
+// order) into 9 stencil slots x 7 fields in registers; 9 phases then add slot
+// (d,e) of every cell into node cell+(d,e) of an 18x18-node LDS region (one-to
+// -one within a phase: no conflicts, fixed order); the region is this tile's
+// partial, merged per node by k_egrid.
+__global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+                                             const int* __restrict__ idx, const int* __restrict__ ptiles,
+                                             const int* __restrict__ cell_begin, const float* __restrict__ tgx,
+                                             const float* __restrict__ tgy, const float* __restrict__ tcth,
+                                             float* __restrict__ tpart) {
+    if (c->stop || c->compact) return;
+    __shared__ float s_acc[P2G_FIELDS * REGION_NODES];
+    const PB32 P = ecur(c, A, B);
+    const int nt = c->n_ptiles, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
+    const float inv = (float)c->inv_dx;
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        const int t = ptiles[w];
+        if (t >= live) continue;
+        const int slot = t / ntd, local = t - slot * ntd;
+        if (!slot_active(slots[slot].status)) continue;
+        const float g = (float)slots[slot].gravity;
+        const int tx = local / nty, ty = local - tx * nty;
+        const int ib = tx * TILE + lx, jb = ty * TILE + ly;
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
+        float acc[9][P2G_FIELDS];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
+        const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
+        const int r0 = cb[threadIdx.x], r1 = cb[threadIdx.x + 1];
+        for (int r = r0; r < r1; ++r) {
+            const int s = idx[r];
+            const int cell = P.cell[s];
+            const int ci = cell >> 16, cj = cell & 0xffff;
+            const size_t fc = (size_t)ci * ny + cj;
+            const float fx = (float)(ci - ib) + P.ox[s], fy = (float)(cj - jb) + P.oy[s];
+            const float mp = P.m[s], hp = P.h[s];
+            const float cth = tcth[fc];
+            const float coeff = -mp * g * cth * cth;
+            const float vmx = P.vx[s] * mp, vmy = P.vy[s] * mp, vzm = P.vz[s] * mp, hm = hp * mp;
+            const float mbx = coeff * tgx[fc], mby = coeff * tgy[fc];
+            // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
+            const float ps = -0.5f * mp * g * cth * cth * hp;
+            float wx[3], wy[3], dwx[3], dwy[3];
+            weights3f(fx, inv, wx, dwx);
+            weights3f(fy, inv, wy, dwy);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    const float wgt = wx[d] * wy[e];
+                    if (wgt > 3e-3f) { // simulation.cpp:780
+                        float* a = acc[d * 3 + e];
+                        a[0] += mp * wgt;
+                        a[1] += vmx * wgt;
+                        a[2] += vmy * wgt;
+                        a[3] += vzm * wgt;
+                        a[4] += hm * wgt;
+                        a[5] += mbx * wgt - ps * (dwx[d] * wy[e]);
+                        a[6] += mby * wgt - ps * (dwy[e] * wx[d]);
+                    }
                 }
             }
         }
         __syncthreads();
-        if (do_p2g) p2g_flush(sh, acc, tpart + (size_t)w * (P2G_FIELDS * REGION_NODES));
+#pragma unroll
+        for (int ph = 0; ph < 9; ++ph) {
+            const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) s_acc[f * REGION_NODES + node] += acc[ph][f];
+            __syncthreads();
+        }
+        float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) dst[q] = s_acc[q];
         __syncthreads();
     }
 }
@@ -1085,6 +1006,7 @@ struct Engine32 {
     double ms[kNumK] = {};
     int64_t cnt[kNumK] = {};
     bool use_graph = false;
+    int persist = kPersistE; // workgroups of the tile-list kernels (sort + P2G, grid)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 };
@@ -1147,8 +1069,8 @@ unsigned capblocks(const Engine32* e) { return (unsigned)std::max<int64_t>(1, (e
 void enqueue_estep(Engine32* e, int step) {
     hipStream_t s = e->stream;
     ECtl* c = e->d_c;
-    const unsigned tb = (unsigned)std::max(1, std::min(e->ntiles, kPersistE));
-    const unsigned pb = (unsigned)std::max(1, std::min(e->tile_cap, kPersistE));
+    const unsigned tb = (unsigned)std::max(1, std::min(e->ntiles, e->persist));
+    const unsigned pb = (unsigned)std::max(1, std::min(e->tile_cap, e->persist));
     tk(e, 0, step, [&] { k_ebegin<<<1, WG, 0, s>>>(c, e->d_slots, e->vmx); });
     tk(e, 1, step, [&] { k_ereduce<<<e->nblk, WG, 0, s>>>(c, e->tile_count, e->ntiles, e->bsum); });
     tk(e, 2, step, [&] { k_escan<<<1, WG, 0, s>>>(c, e->bsum, e->nblk, e->boff); });
@@ -1157,10 +1079,13 @@ void enqueue_estep(Engine32* e, int step) {
                                            e->pstamp, e->gstamp, e->tile_pos);
     });
     tk(e, 4, step, [&] { k_escatter<<<capblocks(e), WG, 0, s>>>(c, e->key, e->tslot, e->tile_begin, e->tmp); });
-    // sort and P2G run fused (timed under "p2g")
+    tk(e, 5, step, [&] {
+        k_esort<<<pb, WG, 0, s>>>(c, e->A, e->B, e->key, e->ptiles, e->tile_count, e->tile_begin, e->tmp, e->idx,
+                                  e->cell_begin);
+    });
     tk(e, 6, step, [&] {
-        k_esortp2g<<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->key, e->ptiles, e->tile_count, e->tile_begin,
-                                     e->tmp, e->idx, e->cell_begin, e->tgx, e->tgy, e->tcth, e->tpart);
+        k_ep2g<<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
+                                 e->tpart);
     });
     tk(e, 7, step, [&] {
         k_egrid<<<tb, WG, 0, s>>>(c, e->d_slots, e->gtiles, e->pstamp, e->tile_pos, e->tpart, e->tgx, e->tgy, e->tcth,
@@ -1738,6 +1663,80 @@ int engine_use_graph(Engine32* e, int on) {
     e->use_graph = on != 0;
     if (!on) drop_graph(e);
     return 0;
+}
+
+bool engine_autotune(Engine32* e, int trial_steps, int out[2], std::string* err) {
+    // Candidate launch shapes are timed on the live workload: each trial runs
+    // real steps of the simulation (the numbers do not depend on the shape),
+    // first the storage re-sort period at the default workgroup count, then
+    // the workgroup count at the best period.
+    trial_steps = std::min(std::max(trial_steps, 2), 256);
+    static const int kPeriods[] = {1, 4, 16, 64};
+    static const int kPersist[] = {1024, 2048, 4096, 8192};
+    hipEvent_t a = nullptr, b = nullptr;
+    if (!hip_ok(hipEventCreate(&a), "event", err) || !hip_ok(hipEventCreate(&b), "event", err)) return false;
+    const bool tm = e->timing;
+    e->timing = false;
+    bool ok = true, finished = false;
+    auto trial = [&](int k, int p, double* ms_step) {
+        if (!engine_set_reorder(e, k, err)) return false;
+        e->persist = p;
+        drop_graph(e);
+        if (!engine_step(e, 1, err)) return false; // captures the graph, warms caches
+        // at least two re-sort periods, so a trial pays for its re-sorts and
+        // for the storage order drifting between them
+        const int want = std::max(trial_steps, 2 * k);
+        double ms_sum = 0.0;
+        int steps = 0;
+        for (int rep = 0; rep < 8 && steps < want && e->h_c->stop != STOP_DONE; ++rep) {
+            if (!hip_ok(hipEventRecord(a, e->stream), "event", err)) return false;
+            if (!engine_step(e, want - steps, err)) return false;
+            if (!hip_ok(hipEventRecord(b, e->stream), "event", err) || !hip_ok(hipEventSynchronize(b), "event", err))
+                return false;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            ms_sum += ms;
+            steps += e->h_c->steps_in_call;
+        }
+        if (steps == 0) {
+            finished = true;
+            *ms_step = 1e30;
+        } else {
+            *ms_step = ms_sum / steps;
+        }
+        return true;
+    };
+    int best_k = e->h_c->reorder_every, best_p = e->persist;
+    double best = 1e30;
+    for (int k : kPeriods) {
+        double ms = 0;
+        if (!(ok = trial(k, kPersistE, &ms)) || finished) break;
+        if (ms < best) {
+            best = ms;
+            best_k = k;
+            best_p = kPersistE;
+        }
+    }
+    if (ok && !finished) {
+        for (int p : kPersist) {
+            if (p == kPersistE) continue;
+            double ms = 0;
+            if (!(ok = trial(best_k, p, &ms)) || finished) break;
+            if (ms < best) {
+                best = ms;
+                best_p = p;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    e->timing = tm;
+    if (!ok) return false;
+    e->persist = best_p;
+    drop_graph(e);
+    out[0] = best_k;
+    out[1] = best_p;
+    return engine_set_reorder(e, best_k, err);
 }
 
 bool engine_set_reorder(Engine32* e, int k, std::string* err) {

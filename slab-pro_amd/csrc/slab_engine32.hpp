@@ -98,5 +98,9 @@ int engine_use_graph(Engine32* e, int on);             // capture the step seque
 // order to the other buffer (dropping released ones); other steps update them
 // in place.  Results do not depend on k (the computation order is canonical).
 bool engine_set_reorder(Engine32* e, int k, std::string* err);
+// Times candidate launch shapes (storage re-sort period, workgroups of the
+// tile-list kernels) on the next steps of the live run and keeps the fastest;
+// out = {period, workgroups}.  The steps it runs are ordinary steps.
+bool engine_autotune(Engine32* e, int trial_steps, int out[2], std::string* err);
 
 } // namespace slab

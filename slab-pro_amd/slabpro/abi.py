@@ -100,6 +100,8 @@ SIGNATURES = {
     "slab_batch_destroy": (None, [_VP]),
     "slab_batch_set_resort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "slab_run_set_resort": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "slab_batch_autotune": (ctypes.c_int, [_VP, ctypes.c_int32, _IP]),
+    "slab_run_autotune": (ctypes.c_int, [_VP, ctypes.c_int32, _IP]),
     # include/slab_frontend.h
     "slab_sim_load_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
     "slab_sim_initialize": (ctypes.c_int, [_VP]),
@@ -285,6 +287,11 @@ class Run:
     def set_resort(self, steps):
         _check(lib().slab_run_set_resort(self.h, int(steps)))
 
+    def autotune(self, trial_steps=8):
+        out = (ctypes.c_int32 * 2)()
+        _check(lib().slab_run_autotune(self.h, int(trial_steps), out))
+        return {"resort_period": out[0], "workgroups": out[1]}
+
     def set_time(self, t, frame):
         _check(lib().slab_run_set_time(self.h, float(t), int(frame)))
 
@@ -368,6 +375,11 @@ class Batch:
 
     def set_resort(self, steps):
         _check(lib().slab_batch_set_resort(self.h, int(steps)))
+
+    def autotune(self, trial_steps=8):
+        out = (ctypes.c_int32 * 2)()
+        _check(lib().slab_batch_autotune(self.h, int(trial_steps), out))
+        return {"resort_period": out[0], "workgroups": out[1]}
 
     def timing(self, enable):
         n = lib().slab_kernel_count()

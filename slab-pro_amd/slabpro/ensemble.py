@@ -28,7 +28,7 @@ def _footprint(s):
 
 
 def run_batched(dem, dx, scenarios, max_slots=256, capacity=None, steps_per_call=32, graph=True, results=False,
-                progress=None, every_s=15.0):
+                progress=None, every_s=15.0, autotune=True):
     """Advance every scenario in one FP32 batch.  Scenarios are admitted in
     order whenever a slot and particle capacity are free; finished slots are
     collected (max maps) and released.  Returns wall time and counts.
@@ -45,6 +45,7 @@ def run_batched(dem, dx, scenarios, max_slots=256, capacity=None, steps_per_call
     done = {}
     pupd = 0
     calls = 0
+    tuned = None
     t0 = time.perf_counter()
     last = t0
     while todo or slot_of:
@@ -56,6 +57,11 @@ def run_batched(dem, dx, scenarios, max_slots=256, capacity=None, steps_per_call
             slot_of[slot] = todo.popleft()
         if not slot_of:
             raise RuntimeError(f"scenario {todo[0]} does not fit an empty batch")
+        if autotune and tuned is None and calls > 0:
+            # launch shapes timed on the live batch (its trial steps are real steps)
+            tuned = b.autotune(4)
+            if progress:
+                progress(f"batched: autotuned {tuned}")
         b.step(steps_per_call)
         calls += 1
         st, tm = b.statuses()
@@ -74,7 +80,7 @@ def run_batched(dem, dx, scenarios, max_slots=256, capacity=None, steps_per_call
     wall = time.perf_counter() - t0
     b.close()
     failed = sum(1 for v in done.values() if v[0]["status"] == abi.SLOT_FAILED)
-    return {"wall_s": wall, "setup_s": t_setup, "scenarios": n, "failed": failed,
+    return {"wall_s": wall, "setup_s": t_setup, "scenarios": n, "failed": failed, "autotuned": tuned,
             "scenarios_per_hour": n / wall * 3600.0, "particle_updates": pupd,
             "particle_updates_per_s": pupd / wall, "results": done}
 

@@ -120,6 +120,28 @@ def test_fp32_storage_resort_period_does_not_change_results(period):
             assert np.array_equal(a[k], b[k]), k
 
 
+def test_fp32_autotune_keeps_results():
+    """The autotuner's trial steps are ordinary steps: a tuned run ends bitwise
+    where an untuned one does."""
+    from slabpro import configs
+    cfg = configs.tiny(fric={"type": 2, "mu1": 0.38, "mu2": 0.64, "I0": 0.279, "d": 0.5, "phi": 0.6})
+    outs = []
+    for tune in (True, False):
+        r = _mk(cfg, 32, keep=False)
+        r.prime()
+        if tune:
+            chosen = r.autotune(2)
+            print("autotune", chosen, flush=True)
+            assert chosen["resort_period"] in (1, 4, 16, 64) and chosen["workgroups"] in (1024, 2048, 4096, 8192)
+        n, info = r.run()
+        outs.append((r.particles(), r.max_maps(), info["step"]))
+        r.close()
+    assert outs[0][2] == outs[1][2]
+    for k in outs[0][0]:
+        assert np.array_equal(outs[0][0][k], outs[1][0][k]), k
+    assert np.array_equal(outs[0][1][0], outs[1][1][0]) and np.array_equal(outs[0][1][1], outs[1][1][1])
+
+
 def test_fp32_c0_full_run_error_report():
     import math
     from slabpro import configs

@@ -127,6 +127,8 @@ def main():
                     help="full run of --workload in both forms: FP32 error vs FP64 (max maps, runout, deposit)")
     ap.add_argument("--rates", default="2,5,10")
     ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--slots", type=int, default=128, help="serving: batch slots")
+    ap.add_argument("--capacity", type=int, default=24_000_000, help="serving: device particle capacity")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build's step runs on a single MI355X "
@@ -310,7 +312,7 @@ def serve_main(args):
     log(f"serve: {n} requests generated in {time.time() - t0:.1f}s, {sum(len(r['x']) for r in reqs)} particles")
     for lam in [float(v) for v in args.rates.split(",")]:
         arr, _ = configs.c4_requests(n, lam, seed_index=int(lam * 10))
-        r = ensemble.serve(dem, 5.0, reqs, arr, max_slots=64, capacity=12_000_000, progress=log)
+        r = ensemble.serve(dem, 5.0, reqs, arr, max_slots=args.slots, capacity=args.capacity, progress=log)
         res[f"lambda_{lam:g}"] = r
         log(f"lambda {lam:g}/s: p50 {r['p50_s']:.2f}s p95 {r['p95_s']:.2f}s p99 {r['p99_s']:.2f}s, "
             f"{r['scenarios_per_hour']:.0f} scen/h over {r['wall_s']:.1f}s")
@@ -318,7 +320,8 @@ def serve_main(args):
     out = {"metric": "p99 completion latency (serving)", "value": first["p99_s"], "unit": "s", "n_gpus": 1,
            "higher_is_better": False, "dtype": "f32", "data": "synthetic (seeded requests, Poisson arrivals)",
            "config": {"workload": f"c4: {n} of the 2,000 seeded requests per rate, rates {args.rates}/s, "
-                                  "c3 scenario distribution on the 2000x2000 DEM, 64 slots, 12M-particle capacity"},
+                                  f"c3 scenario distribution on the 2000x2000 DEM, {args.slots} slots, "
+                                  f"{args.capacity / 1e6:g}M-particle capacity"},
            "per_rate": res}
     print(json.dumps(out), flush=True)
 

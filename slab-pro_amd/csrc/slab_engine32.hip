@@ -454,9 +454,13 @@ __global__ void k_escatter(ECtl* c, const int* __restrict__ key, const int* __re
     tmp[tile_begin[k >> 8] + tslot[s]] = s;
 }
 
-// Per particle tile: counting sort by cell in LDS, then each cell's run sorted
-// by creation index: canonical order (slot, tile, cell, creation index).
-// Parking tiles keep their bucket order (their particles are only carried).
+// Per particle tile: counting sort by cell, then each cell's run sorted by
+// creation index: canonical order (slot, tile, cell, creation index).  Tiles
+// of at most ES_CAP particles sort in LDS (cell and creation index staged by
+// one coalesced pass); larger ones sort through global memory.  Parking tiles
+// keep their bucket order (their particles are only carried).
+constexpr int ES_CAP = 4096;
+
 __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int* __restrict__ key,
                                               const int* __restrict__ ptiles, int* __restrict__ tile_count,
                                               const int* __restrict__ tile_begin, const int* __restrict__ tmp,
@@ -466,6 +470,9 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
     __shared__ int s_off[TILE_CELLS];
     __shared__ int s_w[4];
     __shared__ int s_n;
+    __shared__ int s_pid[ES_CAP];
+    __shared__ unsigned short s_ord[ES_CAP];
+    __shared__ unsigned char s_cell[ES_CAP];
     const int32_t* __restrict__ pid = ecur(c, A, B).pid;
     const int nt = c->n_ptiles, live = c->nslots * c->ntd;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
@@ -486,12 +493,23 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
             __syncthreads();
             continue;
         }
-        for (int r = threadIdx.x; r < n; r += WG) atomicAdd(&s_hist[key[tmp[b + r]] & 255], 1);
+        const bool lds = n <= ES_CAP;
+        if (lds) {
+            for (int r = threadIdx.x; r < n; r += WG) {
+                const int sv = tmp[b + r];
+                const int cc = key[sv] & 255;
+                s_cell[r] = (unsigned char)cc;
+                s_pid[r] = pid[sv];
+                atomicAdd(&s_hist[cc], 1);
+            }
+        } else {
+            for (int r = threadIdx.x; r < n; r += WG) atomicAdd(&s_hist[key[tmp[b + r]] & 255], 1);
+        }
         __syncthreads();
+        const int cnt = s_hist[threadIdx.x];
         {
             const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-            const int v = s_hist[threadIdx.x];
-            int inc = v;
+            int inc = cnt;
             for (int o = 1; o < 64; o <<= 1) {
                 int u = __shfl_up(inc, o, 64);
                 if (lane >= o) inc += u;
@@ -500,25 +518,40 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
             __syncthreads();
             int pre = 0;
             for (int q = 0; q < wv; ++q) pre += s_w[q];
-            s_off[threadIdx.x] = pre + inc - v;
-            cb[threadIdx.x] = b + pre + inc - v;
+            s_off[threadIdx.x] = pre + inc - cnt;
+            cb[threadIdx.x] = b + pre + inc - cnt;
             if (threadIdx.x == 0) {
                 cb[TILE_CELLS] = b + n;
                 tile_count[t] = 0;
             }
         }
         __syncthreads();
-        s_hist[threadIdx.x] = s_off[threadIdx.x];
+        const int off = s_off[threadIdx.x];
+        s_hist[threadIdx.x] = off; // cursor
         __syncthreads();
-        for (int r = threadIdx.x; r < n; r += WG) {
-            const int s = tmp[b + r];
-            const int pos = atomicAdd(&s_hist[key[s] & 255], 1);
-            idx[b + pos] = s;
-        }
-        __syncthreads();
-        {
-            const int lo = b + s_off[threadIdx.x];
-            const int hi = b + s_hist[threadIdx.x];
+        if (lds) {
+            for (int r = threadIdx.x; r < n; r += WG) s_ord[atomicAdd(&s_hist[s_cell[r]], 1)] = (unsigned short)r;
+            __syncthreads();
+            for (int q = off + 1; q < off + cnt; ++q) {
+                const unsigned short rv = s_ord[q];
+                const int pv = s_pid[rv];
+                int k = q - 1;
+                while (k >= off && s_pid[s_ord[k]] > pv) {
+                    s_ord[k + 1] = s_ord[k];
+                    --k;
+                }
+                s_ord[k + 1] = rv;
+            }
+            __syncthreads();
+            for (int q = threadIdx.x; q < n; q += WG) idx[b + q] = tmp[b + s_ord[q]];
+        } else {
+            for (int r = threadIdx.x; r < n; r += WG) {
+                const int sv = tmp[b + r];
+                const int pos = atomicAdd(&s_hist[key[sv] & 255], 1);
+                idx[b + pos] = sv;
+            }
+            __syncthreads();
+            const int lo = b + off, hi = b + off + cnt;
             for (int q = lo + 1; q < hi; ++q) {
                 const int sv = idx[q];
                 const int pv = pid[sv];
@@ -536,13 +569,22 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
 
 // --- P2G -------------------------------------------------------------------
 
+// Threads per cell in P2G: with 2, the particles of a cell alternate between
+// two threads (halving the wait on dense cells) and the 9 flush phases run
+// once per thread group, in group order.
+#ifndef SLAB_P2G_SPLIT
+#define SLAB_P2G_SPLIT 1
+#endif
+constexpr int P2G_SPLIT = SLAB_P2G_SPLIT;
+constexpr int P2G_WG = WG * P2G_SPLIT;
+
 // P2G, src/simulation.cpp:726-798, as a per-tile segmented reduction without
 // atomics: one thread per stencil-base cell sums its particles (canonical
 // order) into 9 stencil slots x 7 fields in registers; 9 phases then add slot
 // (d,e) of every cell into node cell+(d,e) of an 18x18-node LDS region (one-to
 // -one within a phase: no conflicts, fixed order); the region is this tile's
 // partial, merged per node by k_egrid.
-__global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+__global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                              const int* __restrict__ idx, const int* __restrict__ ptiles,
                                              const int* __restrict__ cell_begin, const float* __restrict__ tgx,
                                              const float* __restrict__ tgy, const float* __restrict__ tcth,
@@ -552,7 +594,8 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
     const PB32 P = ecur(c, A, B);
     const int nt = c->n_ptiles, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
     const float inv = (float)c->inv_dx;
-    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    const int cid = threadIdx.x & (TILE_CELLS - 1), part = threadIdx.x >> 8;
+    const int lx = cid >> 4, ly = cid & 15;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         const int t = ptiles[w];
         if (t >= live) continue;
@@ -561,15 +604,15 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         const float g = (float)slots[slot].gravity;
         const int tx = local / nty, ty = local - tx * nty;
         const int ib = tx * TILE + lx, jb = ty * TILE + ly;
-        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += P2G_WG) s_acc[q] = 0.f;
         float acc[9][P2G_FIELDS];
 #pragma unroll
         for (int k = 0; k < 9; ++k)
 #pragma unroll
             for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
         const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
-        const int r0 = cb[threadIdx.x], r1 = cb[threadIdx.x + 1];
-        for (int r = r0; r < r1; ++r) {
+        const int r0 = cb[cid], r1 = cb[cid + 1];
+        for (int r = r0 + part; r < r1; r += P2G_SPLIT) {
             const int s = idx[r];
             const int cell = P.cell[s];
             const int ci = cell >> 16, cj = cell & 0xffff;
@@ -604,15 +647,19 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
             }
         }
         __syncthreads();
+        for (int pp = 0; pp < P2G_SPLIT; ++pp) {
 #pragma unroll
-        for (int ph = 0; ph < 9; ++ph) {
-            const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
+            for (int ph = 0; ph < 9; ++ph) {
+                const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
+                if (part == pp) {
 #pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) s_acc[f * REGION_NODES + node] += acc[ph][f];
-            __syncthreads();
+                    for (int f = 0; f < P2G_FIELDS; ++f) s_acc[f * REGION_NODES + node] += acc[ph][f];
+                }
+                __syncthreads();
+            }
         }
         float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
-        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) dst[q] = s_acc[q];
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += P2G_WG) dst[q] = s_acc[q];
         __syncthreads();
     }
 }
@@ -1084,7 +1131,7 @@ void enqueue_estep(Engine32* e, int step) {
                                   e->cell_begin);
     });
     tk(e, 6, step, [&] {
-        k_ep2g<<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
+        k_ep2g<<<pb, P2G_WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
                                  e->tpart);
     });
     tk(e, 7, step, [&] {

@@ -69,6 +69,7 @@ struct slab_run {
     int *tile_count = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr;
     int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr, *cell_begin = nullptr;
     int* tile_pos = nullptr; // tile -> position in the particle-tile list
+    double* tpart = nullptr; // P2G partial region per listed particle tile
     // particles
     int64_t np = 0;
     PBuf64 A{}, B{};
@@ -182,16 +183,19 @@ void enqueue_sort(slab_run* r, int step) {
         k_tile_sort<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->A.pid, r->B.pid, r->key, r->ptiles, r->tile_count,
                                               r->tile_begin, r->tmp, r->idx, r->cell_begin);
     });
-    timed(r, 6, step, [&] {
-        k_payload<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->tgx, r->tgy, r->tcth, r->Y);
-    });
 }
 
+// P2G: per-particle payload in sorted order, then the per-tile register /
+// LDS reduction into tile partials; grid: partial merge + update per node.
 void enqueue_p2g(slab_run* r, int step) {
     hipStream_t s = r->stream;
+    timed(r, 6, step, [&] {
+        k_payload<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->tgx, r->tgy, r->tcth, r->Y);
+        k_p2g64<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->Y, r->ptiles, r->cell_begin, r->tpart);
+    });
     timed(r, 7, step, [&] {
-        k_p2g_grid<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->Y, r->gtiles, r->pstamp, r->cell_begin, r->tgx, r->tgy,
-                                             r->tcth, r->hmax, r->vmax2, r->vf, r->gm, r->gh, r->gvz);
+        k_grid64<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->gtiles, r->pstamp, r->tile_pos, r->tpart, r->tgx, r->tgy,
+                                           r->tcth, r->hmax, r->vmax2, r->vf, r->gm, r->gh, r->gvz);
     });
 }
 
@@ -505,6 +509,9 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
     P(&r->tslot);
     P(&r->idx);
     P(&r->tmp);
+    // P2G partials: one region per listed particle tile (at most one tile per particle)
+    if (st == SLAB_OK)
+        st = dalloc(&r->tpart, (size_t)std::min<int64_t>(r->ntiles, np) * kP2G64PartialDoubles, r->part_allocs);
     if (st == SLAB_OK) st = dalloc(&r->part, 5 * (size_t)((np + WG - 1) / WG), r->part_allocs);
     if (st != SLAB_OK) return st;
     TRY(pull_ctl(r));

@@ -426,6 +426,201 @@ __global__ void __launch_bounds__(WG) k_payload(Ctl* c, PBuf64 A, PBuf64 B, cons
     Y.s00[r] = -0.5 * rho * g * cth * cth * hp;
 }
 
+// P2G (src/simulation.cpp:726-798) per particle tile, without atomics: one
+// thread per stencil-base cell sums the payload of its particles (canonical
+// order) into its 9 stencil slots in registers; 9 phases then add slot (d,e)
+// of every cell into node cell+(d,e) of an 18x18-node LDS region (one-to-one
+// within a phase, fixed order) and the region is written as the tile's
+// partial.  The 7 FP64 fields run in two groups (mass + momenta, then height +
+// forces) to bound the registers.  k_grid64 merges the partials per node.
+constexpr int P2G64_FIELDS = 7;
+constexpr int REGION64 = TILE + 2;
+constexpr int REGION64_NODES = REGION64 * REGION64;
+
+__global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, Pay64 Y, const int* __restrict__ ptiles,
+                                              const int* __restrict__ cell_begin, double* __restrict__ tpart) {
+    if (c->stop) return;
+    __shared__ double s_acc[4 * REGION64_NODES];
+    const int nt = c->n_ptiles, nty = c->nty;
+    const double inv = c->inv_dx, dx = c->dx;
+    const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
+    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        const int t = ptiles[w];
+        const int tx = t / nty, ty = t - tx * nty;
+        const int ib = tx * TILE + lx, jb = ty * TILE + ly;
+        const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
+        const int r0 = cb[threadIdx.x], r1 = cb[threadIdx.x + 1];
+        double* dst = tpart + (size_t)w * (P2G64_FIELDS * REGION64_NODES);
+        double X[3], Yn[3];
+        for (int d = 0; d < 3; ++d) {
+            X[d] = (double)(ib + d) * dx;
+            Yn[d] = (double)(jb + d) * dx;
+        }
+        for (int grp = 0; grp < 2; ++grp) {
+            const int nf = grp == 0 ? 4 : 3;
+            for (int q = threadIdx.x; q < 4 * REGION64_NODES; q += WG) s_acc[q] = 0.0;
+            double acc[9][4];
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+#pragma unroll
+                for (int f = 0; f < 4; ++f) acc[k][f] = 0.0;
+            for (int r = r0; r < r1; ++r) {
+                const double xp = Y.x[r], yp = Y.y[r];
+                double ux[3], uy[3], wx[3], wy[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    ux[d] = (xp - X[d]) * inv;
+                    uy[d] = (yp - Yn[d]) * inv;
+                    wx[d] = Nq(ux[d]);
+                    wy[d] = Nq(uy[d]);
+                }
+                if (grp == 0) {
+                    const double m = Y.m[r], vmx = Y.vmx[r], vmy = Y.vmy[r], vzm = Y.vzm[r];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) {
+                            const double wgt = wx[d] * wy[e];
+                            if (!(wgt > 3e-3)) continue; // simulation.cpp:780
+                            double* a = acc[d * 3 + e];
+                            a[0] += m * wgt;
+                            a[1] += vmx * wgt;
+                            a[2] += vmy * wgt;
+                            a[3] += vzm * wgt;
+                        }
+                    }
+                } else {
+                    const double hm = Y.hm[r], hA = Y.hA[r], mbx = Y.mbx[r], mby = Y.mby[r], s00 = Y.s00[r];
+                    double dwx[3], dwy[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        dwx[d] = dNq(ux[d]) * inv;
+                        dwy[d] = dNq(uy[d]) * inv;
+                    }
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) {
+                            const double wgt = wx[d] * wy[e];
+                            if (!(wgt > 3e-3)) continue;
+                            const double gwx = dwx[d] * wy[e], gwy = dwy[e] * wx[d];
+                            double* a = acc[d * 3 + e];
+                            a[0] += hm * wgt;
+                            a[1] += mbx * wgt - hA * (s00 * gwx + 0.0 * gwy);
+                            a[2] += mby * wgt - hA * (s00 * gwy + 0.0 * gwx);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ph = 0; ph < 9; ++ph) {
+                const int node = (lx + ph / 3) * REGION64 + (ly + ph % 3);
+#pragma unroll
+                for (int f = 0; f < 4; ++f)
+                    if (f < nf) s_acc[f * REGION64_NODES + node] += acc[ph][f];
+                __syncthreads();
+            }
+            double* gd = dst + (size_t)(grp == 0 ? 0 : 4) * REGION64_NODES;
+            for (int q = threadIdx.x; q < nf * REGION64_NODES; q += WG) gd[q] = s_acc[q];
+            __syncthreads();
+        }
+    }
+}
+
+// computeGridVelocity (src/simulation.cpp:800-875) per grid tile, one thread
+// per node, after merging the node's P2G sums from the particle tiles that
+// reach it (own, left, below, diagonal; fixed order).
+__global__ void __launch_bounds__(WG) k_grid64(Ctl* c, const int* __restrict__ gtiles, const int* __restrict__ pstamp,
+                                               const int* __restrict__ tile_pos, const double* __restrict__ tpart,
+                                               const double* __restrict__ tgx, const double* __restrict__ tgy,
+                                               const double* __restrict__ tcth, double* __restrict__ h_max,
+                                               double* __restrict__ v_max2, double4* __restrict__ vf,
+                                               double* __restrict__ out_m, double* __restrict__ out_h,
+                                               double* __restrict__ out_vz) {
+    if (c->stop) return;
+    const int nt = c->n_gtiles, ep = c->epoch, nx = c->nx, ny = c->ny, nty = c->nty;
+    const double g = c->gravity, dt = c->dt;
+    const int out = c->out_step;
+    const int a = threadIdx.x >> 4, b = threadIdx.x & 15;
+    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        const int t = gtiles[w];
+        const int tx = t / nty, ty = t - tx * nty;
+        const int i = tx * TILE + a, j = ty * TILE + b;
+        if (i >= nx || j >= ny) continue;
+        double acc[P2G64_FIELDS] = {0, 0, 0, 0, 0, 0, 0};
+        int src[4], rn[4];
+        int ns = 0;
+        src[ns] = t;
+        rn[ns++] = a * REGION64 + b;
+        if (tx > 0 && a < 2) {
+            src[ns] = t - nty;
+            rn[ns++] = (a + TILE) * REGION64 + b;
+        }
+        if (ty > 0 && b < 2) {
+            src[ns] = t - 1;
+            rn[ns++] = a * REGION64 + (b + TILE);
+        }
+        if (tx > 0 && ty > 0 && a < 2 && b < 2) {
+            src[ns] = t - nty - 1;
+            rn[ns++] = (a + TILE) * REGION64 + (b + TILE);
+        }
+        for (int k = 0; k < ns; ++k) {
+            if (pstamp[src[k]] != ep) continue;
+            const double* q = tpart + (size_t)tile_pos[src[k]] * (P2G64_FIELDS * REGION64_NODES) + rn[k];
+#pragma unroll
+            for (int f = 0; f < P2G64_FIELDS; ++f) acc[f] += q[f * REGION64_NODES];
+        }
+        // acc: m, P_x, P_y, P_z | H, F_x, F_y
+        const double am = acc[0], apx = acc[1], apy = acc[2], apz = acc[3], ah = acc[4], afx = acc[5], afy = acc[6];
+        const size_t k = (size_t)i * ny + j;
+        double nvx = 0.0, nvy = 0.0, flx = 0.0, fly = 0.0, hh = 0.0, vz = 0.0;
+        if (am > 1e-6) {
+            const double vx = apx / am, vy = apy / am;
+            vz = apz / am;
+            hh = ah / am;
+            const double ax = afx / am, ay = afy / am;
+            const double vtx = vx + dt * ax, vty = vy + dt * ay;
+            const double vsq = vtx * vtx + vty * vty;
+            if (vsq > 1e-16) {
+                const double Gx = tgx[k], Gy = tgy[k], cth = tcth[k];
+                const double v_z = vtx * Gx + vty * Gy;
+                const double v3sq = vsq + v_z * v_z;
+                const double vn = sqrt(v3sq);
+                double af;
+                if (c->fric == 0) {
+                    af = c->mu * g * cth;
+                } else if (c->fric == 1) {
+                    af = c->mu * g * cth + g * v3sq / (c->xi * hh * cth);
+                } else {
+                    const double I = 5. * c->d * vn / (2. * hh * cth * cth * sqrt(g * hh * c->phi));
+                    const double muI = c->mu1 + (c->mu2 - c->mu1) / (1. + c->I0 / I);
+                    af = muI * g * cth;
+                }
+                const double dv = dt * af;
+                if (dv < vn) {
+                    const double f = 1. - dv / vn;
+                    nvx = vtx * f;
+                    nvy = vty * f;
+                }
+            }
+            flx = nvx - vx;
+            fly = nvy - vy;
+            const double hm0 = h_max[k];
+            if (hm0 < hh) h_max[k] = hh;
+            const double v2 = nvx * nvx + nvy * nvy;
+            const double vm0 = v_max2[k];
+            if (vm0 < v2) v_max2[k] = v2;
+        }
+        vf[k] = make_double4(nvx, nvy, flx, fly);
+        if (out) {
+            out_m[k] = am;
+            out_h[k] = hh;
+            out_vz[k] = vz;
+        }
+    }
+}
+
 // One workgroup per grid tile (persistent over the list), one thread per node.
 // P2G (src/simulation.cpp:726-798) as a gather: the node sums the
 // contributions of the particles of the 9 cells whose stencil covers it, in

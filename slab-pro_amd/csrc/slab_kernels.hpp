@@ -36,6 +36,11 @@ __global__ void k_payload(Ctl* c, PBuf64 A, PBuf64 B, const int* idx, const doub
 __global__ void k_p2g_grid(Ctl* c, Pay64 Y, const int* gtiles, const int* pstamp, const int* cell_begin,
                            const double* tgx, const double* tgy, const double* tcth, double* h_max, double* v_max2,
                            double4* vf, double* out_m, double* out_h, double* out_vz);
+__global__ void k_p2g64(Ctl* c, Pay64 Y, const int* ptiles, const int* cell_begin, double* tpart);
+__global__ void k_grid64(Ctl* c, const int* gtiles, const int* pstamp, const int* tile_pos, const double* tpart,
+                         const double* tgx, const double* tgy, const double* tcth, double* h_max, double* v_max2,
+                         double4* vf, double* out_m, double* out_h, double* out_vz);
+constexpr size_t kP2G64PartialDoubles = 7 * 18 * 18; // per listed particle tile
 __global__ void k_g2p(Ctl* c, PBuf64 A, PBuf64 B, const int* idx, const double4* vf, const double* tz,
                       const double* tgx, const double* tgy, int* key, int* tslot, int* tile_count,
                       unsigned long long* part);

@@ -31,9 +31,9 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes (compulsory state traffic, each persistent field read once /
 # written once), per particle and per active node (m > 1e-6), for each kernel.
-# FP64 form: "p2g" = per-particle P2G terms (reads x y vx vy vz h A m),
-#            "grid" = node gather + grid update (terrain Gx Gy cos, h_max v_max2
-#            read+write, write vx vy flipx flipy).
+# FP64 form: "p2g" = per-particle P2G terms (reads x y vx vy vz h A m) and the
+#            per-tile reduction, "grid" = partial merge + grid update (terrain
+#            Gx Gy cos, h_max v_max2 read+write, write vx vy flipx flipy).
 ALG_BYTES_FP64 = {
     "p2g": {"particle": 64, "node": 0},
     "grid": {"particle": 0, "node": 24 + 32 + 32},
@@ -58,7 +58,7 @@ STEP_BYTES_FP32 = {"particle": 60, "node": 60}
 # kernel names of the event timers -> rocprof kernel names, per form
 ROCPROF_NAMES = {
     "fp32": {"g2p": "k_eg2p", "p2g": "k_ep2g", "grid": "k_egrid", "tile_sort": "k_esort", "scatter": "k_escatter"},
-    "fp64": {"g2p": "k_g2p", "p2g": "k_payload", "grid": "k_p2g_grid", "tile_sort": "k_tile_sort",
+    "fp64": {"g2p": "k_g2p", "p2g": "k_p2g64", "grid": "k_grid64", "tile_sort": "k_tile_sort",
              "scatter": "k_scatter"},
 }
 

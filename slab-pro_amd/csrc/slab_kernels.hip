@@ -448,7 +448,7 @@ __global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, Pay64 Y, const int* __rest
         const int t = ptiles[w];
         const int tx = t / nty, ty = t - tx * nty;
         const int ib = tx * TILE + lx, jb = ty * TILE + ly;
-        const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
+        const int* cb = cell_begin + (size_t)t * CELL_STRIDE; // indexed by tile id (k_tile_sort)
         const int r0 = cb[threadIdx.x], r1 = cb[threadIdx.x + 1];
         double* dst = tpart + (size_t)w * (P2G64_FIELDS * REGION64_NODES);
         double X[3], Yn[3];
@@ -620,106 +620,6 @@ __global__ void __launch_bounds__(WG) k_grid64(Ctl* c, const int* __restrict__ g
         }
     }
 }
-
-// One workgroup per grid tile (persistent over the list), one thread per node.
-// P2G (src/simulation.cpp:726-798) as a gather: the node sums the
-// contributions of the particles of the 9 cells whose stencil covers it, in
-// canonical order -- no atomics, reproducible.  Then computeGridVelocity
-// (src/simulation.cpp:800-875) for that node in registers.
-__global__ void __launch_bounds__(WG) k_p2g_grid(Ctl* c, Pay64 Y, const int* __restrict__ gtiles,
-                                                 const int* __restrict__ pstamp, const int* __restrict__ cell_begin,
-                                                 const double* __restrict__ tgx, const double* __restrict__ tgy,
-                                                 const double* __restrict__ tcth, double* __restrict__ h_max,
-                                                 double* __restrict__ v_max2, double4* __restrict__ vf,
-                                                 double* __restrict__ out_m, double* __restrict__ out_h,
-                                                 double* __restrict__ out_vz) {
-    if (c->stop) return;
-    const int nt = c->n_gtiles, ep = c->epoch, nx = c->nx, ny = c->ny, nty = c->nty;
-    const double inv = c->inv_dx, dx = c->dx, g = c->gravity, dt = c->dt;
-    const int out = c->out_step;
-    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
-        const int t = gtiles[w];
-        const int tx = t / nty, ty = t - tx * nty;
-        const int i = tx * TILE + (threadIdx.x >> 4);
-        const int j = ty * TILE + (threadIdx.x & 15);
-        if (i >= nx || j >= ny) continue;
-        const double X = (double)i * dx, Y0 = (double)j * dx;
-        double am = 0.0, apx = 0.0, apy = 0.0, apz = 0.0, ah = 0.0, afx = 0.0, afy = 0.0;
-        for (int ci = i - 2; ci <= i; ++ci) {
-            if (ci < 0 || ci > nx - 3) continue;
-            for (int cj = j - 2; cj <= j; ++cj) {
-                if (cj < 0 || cj > ny - 3) continue;
-                const int tc = (ci >> 4) * nty + (cj >> 4);
-                if (pstamp[tc] != ep) continue;
-                const int lc = ((ci & 15) << 4) | (cj & 15);
-                const int r0 = cell_begin[(size_t)tc * CELL_STRIDE + lc];
-                const int r1 = cell_begin[(size_t)tc * CELL_STRIDE + lc + 1];
-                for (int r = r0; r < r1; ++r) {
-                    const double ux = (Y.x[r] - X) * inv, uy = (Y.y[r] - Y0) * inv;
-                    const double wx = Nq(ux), wy = Nq(uy);
-                    const double wgt = wx * wy;
-                    if (!(wgt > 3e-3)) continue; // simulation.cpp:780
-                    const double gwx = dNq(ux) * inv * wy, gwy = dNq(uy) * inv * wx;
-                    const double hA = Y.hA[r], s00 = Y.s00[r];
-                    am += Y.m[r] * wgt;
-                    apx += Y.vmx[r] * wgt;
-                    apy += Y.vmy[r] * wgt;
-                    apz += Y.vzm[r] * wgt;
-                    ah += Y.hm[r] * wgt;
-                    afx += Y.mbx[r] * wgt - hA * (s00 * gwx + 0.0 * gwy);
-                    afy += Y.mby[r] * wgt - hA * (s00 * gwy + 0.0 * gwx);
-                }
-            }
-        }
-        const size_t k = (size_t)i * ny + j;
-        double nvx = 0.0, nvy = 0.0, flx = 0.0, fly = 0.0, hh = 0.0, vz = 0.0;
-        if (am > 1e-6) {
-            const double vx = apx / am, vy = apy / am;
-            vz = apz / am;
-            hh = ah / am;
-            const double ax = afx / am, ay = afy / am;
-            const double vtx = vx + dt * ax, vty = vy + dt * ay;
-            const double vsq = vtx * vtx + vty * vty;
-            if (vsq > 1e-16) {
-                const double Gx = tgx[k], Gy = tgy[k], cth = tcth[k];
-                const double v_z = vtx * Gx + vty * Gy;
-                const double v3sq = vsq + v_z * v_z;
-                const double vn = sqrt(v3sq);
-                double af;
-                if (c->fric == 0) {
-                    af = c->mu * g * cth;
-                } else if (c->fric == 1) {
-                    af = c->mu * g * cth + g * v3sq / (c->xi * hh * cth);
-                } else {
-                    const double I = 5. * c->d * vn / (2. * hh * cth * cth * sqrt(g * hh * c->phi));
-                    const double muI = c->mu1 + (c->mu2 - c->mu1) / (1. + c->I0 / I);
-                    af = muI * g * cth;
-                }
-                const double dv = dt * af;
-                if (dv < vn) {
-                    const double f = 1. - dv / vn;
-                    nvx = vtx * f;
-                    nvy = vty * f;
-                }
-            }
-            flx = nvx - vx;
-            fly = nvy - vy;
-            const double hm0 = h_max[k];
-            if (hm0 < hh) h_max[k] = hh;
-            const double v2 = nvx * nvx + nvy * nvy;
-            const double vm0 = v_max2[k];
-            if (vm0 < v2) v_max2[k] = v2;
-        }
-        vf[k] = make_double4(nvx, nvy, flx, fly);
-        if (out) {
-            out_m[k] = am;
-            out_h[k] = hh;
-            out_vz[k] = vz;
-        }
-    }
-}
-
-// ---------------------------------------------------------------- G2P
 
 // One thread per particle in sorted order (src/simulation.cpp:877-940).  Reads
 // the particle from the current buffer through idx and writes the updated

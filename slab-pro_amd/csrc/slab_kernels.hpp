@@ -33,9 +33,6 @@ __global__ void k_tile_sort(Ctl* c, const int32_t* pidA, const int32_t* pidB, co
                             int* tile_count, const int* tile_begin, const int* tmp, int* idx, int* cell_begin);
 __global__ void k_payload(Ctl* c, PBuf64 A, PBuf64 B, const int* idx, const double* tgx, const double* tgy,
                           const double* tcth, Pay64 Y);
-__global__ void k_p2g_grid(Ctl* c, Pay64 Y, const int* gtiles, const int* pstamp, const int* cell_begin,
-                           const double* tgx, const double* tgy, const double* tcth, double* h_max, double* v_max2,
-                           double4* vf, double* out_m, double* out_h, double* out_vz);
 __global__ void k_p2g64(Ctl* c, Pay64 Y, const int* ptiles, const int* cell_begin, double* tpart);
 __global__ void k_grid64(Ctl* c, const int* gtiles, const int* pstamp, const int* tile_pos, const double* tpart,
                          const double* tgx, const double* tgy, const double* tcth, double* h_max, double* v_max2,

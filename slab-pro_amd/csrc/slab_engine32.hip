@@ -54,6 +54,7 @@ __device__ __forceinline__ int ebase(int ci, float o, int nmax) {
 
 __device__ __forceinline__ bool slot_active(int st) { return st == SLOT_PRIMING || st == SLOT_RUNNING; }
 
+
 // True when a cell lies outside [1, N-3] on either axis (outflow).
 __device__ __forceinline__ bool cell_out(const ECtl* c, int ci, int cj) {
     return ci < 1 || ci > c->nx - 3 || cj < 1 || cj > c->ny - 3;
@@ -455,11 +456,45 @@ __global__ void k_escatter(ECtl* c, const int* __restrict__ key, const int* __re
 }
 
 // Per particle tile: counting sort by cell, then each cell's run sorted by
-// creation index: canonical order (slot, tile, cell, creation index).  Tiles
-// of at most ES_CAP particles sort in LDS (cell and creation index staged by
-// one coalesced pass); larger ones sort through global memory.  Parking tiles
-// keep their bucket order (their particles are only carried).
+// creation index: canonical order (slot, tile, cell, creation index).
+//  * Tiles of at most ES_CAP particles stage (cell, creation index) in LDS with
+//    one coalesced pass.  If no cell holds more than ES_INSERT particles, each
+//    cell's run is insertion-sorted by its thread; otherwise the whole tile is
+//    bitonic-sorted on 64-bit keys (cell, creation index, arrival position),
+//    whose cost does not depend on how disordered the arrivals are.
+//  * Larger tiles place their particles by cell in global memory, then sort
+//    the cells in rounds: each round bitonic-sorts a contiguous range of whole
+//    cells (at most ES_ROUND particles) in LDS; only a single cell of more than
+//    ES_ROUND particles sorts in global memory.
+//  * Parking tiles keep their bucket order (their particles are only carried).
 constexpr int ES_CAP = 4096;
+constexpr int ES_ROUND = 2048;
+constexpr int ES_INSERT = 32;
+constexpr int ES_LDS_BYTES = ES_CAP * 8;
+
+// Ascending bitonic sort of k[0, P), P a power of two, by one workgroup.
+__device__ __forceinline__ void bitonic_lds(unsigned long long* k, int P) {
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < (P >> 1); i += WG) {
+                const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                const bool asc = (lo & size) == 0;
+                const unsigned long long x = k[lo], y = k[hi];
+                if ((x > y) == asc) {
+                    k[lo] = y;
+                    k[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ int pow2_ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
 
 __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int* __restrict__ key,
                                               const int* __restrict__ ptiles, int* __restrict__ tile_count,
@@ -469,15 +504,24 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
     __shared__ int s_hist[TILE_CELLS];
     __shared__ int s_off[TILE_CELLS];
     __shared__ int s_w[4];
-    __shared__ int s_n;
-    __shared__ int s_pid[ES_CAP];
-    __shared__ unsigned short s_ord[ES_CAP];
-    __shared__ unsigned char s_cell[ES_CAP];
+    __shared__ int s_n, s_c1, s_cmax;
+    // one LDS region, three uses: (a) creation index, arrival order and cell of
+    // a small tile; (b) 64-bit keys of a small tile; (c) keys + storage index of
+    // a round
+    __shared__ __align__(16) unsigned char s_mem[ES_LDS_BYTES];
+    int* s_pid = reinterpret_cast<int*>(s_mem);                                         // (a) ES_CAP x i32
+    unsigned short* s_ord = reinterpret_cast<unsigned short*>(s_mem + ES_CAP * 4);      // (a) ES_CAP x u16
+    unsigned char* s_cell = s_mem + ES_CAP * 6;                                         // (a) ES_CAP x u8
+    unsigned long long* s_key = reinterpret_cast<unsigned long long*>(s_mem);           // (b) ES_CAP, (c) ES_ROUND
+    int* s_sv = reinterpret_cast<int*>(s_mem + ES_ROUND * 8);                           // (c) ES_ROUND x i32
     const int32_t* __restrict__ pid = ecur(c, A, B).pid;
     const int nt = c->n_ptiles, live = c->nslots * c->ntd;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         const int t = ptiles[w];
-        if (threadIdx.x == 0) s_n = tile_count[t];
+        if (threadIdx.x == 0) {
+            s_n = tile_count[t];
+            s_cmax = 0;
+        }
         s_hist[threadIdx.x] = 0;
         __syncthreads();
         const int n = s_n;
@@ -493,8 +537,8 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
             __syncthreads();
             continue;
         }
-        const bool lds = n <= ES_CAP;
-        if (lds) {
+        const bool small = n <= ES_CAP;
+        if (small) {
             for (int r = threadIdx.x; r < n; r += WG) {
                 const int sv = tmp[b + r];
                 const int cc = key[sv] & 255;
@@ -507,6 +551,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
         }
         __syncthreads();
         const int cnt = s_hist[threadIdx.x];
+        if (cnt > ES_INSERT) atomicMax(&s_cmax, cnt);
         {
             const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
             int inc = cnt;
@@ -528,8 +573,9 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
         __syncthreads();
         const int off = s_off[threadIdx.x];
         s_hist[threadIdx.x] = off; // cursor
+        const bool dense = s_cmax > ES_INSERT;
         __syncthreads();
-        if (lds) {
+        if (small && !dense) {
             for (int r = threadIdx.x; r < n; r += WG) s_ord[atomicAdd(&s_hist[s_cell[r]], 1)] = (unsigned short)r;
             __syncthreads();
             for (int q = off + 1; q < off + cnt; ++q) {
@@ -544,6 +590,27 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
             }
             __syncthreads();
             for (int q = threadIdx.x; q < n; q += WG) idx[b + q] = tmp[b + s_ord[q]];
+        } else if (small) {
+            // keys (cell << 44 | creation index << 13 | arrival position), built
+            // in registers first because they overwrite the staged arrays
+            unsigned long long kv[ES_CAP / WG];
+#pragma unroll
+            for (int k = 0; k < ES_CAP / WG; ++k) {
+                const int r = threadIdx.x + k * WG;
+                kv[k] = r < n ? ((unsigned long long)s_cell[r] << 44) | ((unsigned long long)(uint32_t)s_pid[r] << 13) |
+                                    (unsigned long long)r
+                              : ~0ull;
+            }
+            __syncthreads();
+            const int P = pow2_ceil(n);
+#pragma unroll
+            for (int k = 0; k < ES_CAP / WG; ++k) {
+                const int r = threadIdx.x + k * WG;
+                if (r < P) s_key[r] = kv[k];
+            }
+            __syncthreads();
+            bitonic_lds(s_key, P);
+            for (int q = threadIdx.x; q < n; q += WG) idx[b + q] = tmp[b + (int)(s_key[q] & 0x1fffull)];
         } else {
             for (int r = threadIdx.x; r < n; r += WG) {
                 const int sv = tmp[b + r];
@@ -551,16 +618,46 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
                 idx[b + pos] = sv;
             }
             __syncthreads();
-            const int lo = b + off, hi = b + off + cnt;
-            for (int q = lo + 1; q < hi; ++q) {
-                const int sv = idx[q];
-                const int pv = pid[sv];
-                int k = q - 1;
-                while (k >= lo && pid[idx[k]] > pv) {
-                    idx[k + 1] = idx[k];
-                    --k;
+            for (int c0 = 0; c0 < TILE_CELLS;) {
+                if (threadIdx.x == 0) { // extend the round by whole cells up to ES_ROUND particles
+                    const int base = s_off[c0];
+                    int c1 = c0 + 1;
+                    while (c1 < TILE_CELLS && ((c1 + 1 < TILE_CELLS) ? s_off[c1 + 1] : n) - base <= ES_ROUND) ++c1;
+                    s_c1 = c1;
                 }
-                idx[k + 1] = sv;
+                __syncthreads();
+                const int c1 = s_c1;
+                const int lo = s_off[c0], len = ((c1 < TILE_CELLS) ? s_off[c1] : n) - lo;
+                if (len <= ES_ROUND) {
+                    const int P = pow2_ceil(len > 0 ? len : 1);
+                    for (int q = threadIdx.x; q < P; q += WG) {
+                        unsigned long long kq = ~0ull;
+                        if (q < len) {
+                            const int sv = idx[b + lo + q];
+                            s_sv[q] = sv;
+                            kq = ((unsigned long long)(key[sv] & 255) << 44) |
+                                 ((unsigned long long)(uint32_t)pid[sv] << 13) | (unsigned long long)q;
+                        }
+                        s_key[q] = kq;
+                    }
+                    __syncthreads();
+                    bitonic_lds(s_key, P);
+                    for (int q = threadIdx.x; q < len; q += WG) idx[b + lo + q] = s_sv[(int)(s_key[q] & 0x1fffull)];
+                } else if ((int)threadIdx.x == c0) { // one cell beyond ES_ROUND: in global memory
+                    const int glo = b + off, ghi = b + off + cnt;
+                    for (int q = glo + 1; q < ghi; ++q) {
+                        const int sv = idx[q];
+                        const int pv = pid[sv];
+                        int k = q - 1;
+                        while (k >= glo && pid[idx[k]] > pv) {
+                            idx[k + 1] = idx[k];
+                            --k;
+                        }
+                        idx[k + 1] = sv;
+                    }
+                }
+                __syncthreads();
+                c0 = c1;
             }
         }
         __syncthreads();

@@ -216,3 +216,50 @@ def test_resort_period_does_not_change_batch_results(period):
         for k in ("x", "y", "vx", "vy", "h", "z"):
             assert np.array_equal(pa[k], pb[k]), k
         assert np.array_equal(ma[0], mb[0]) and np.array_equal(ma[1], mb[1])
+
+
+def _dense(n_per_axis, cells, x0, y0, h=1.0):
+    """A block of `cells` x `cells` terrain cells (dx = 2 m) holding n_per_axis^2
+    particles each on a regular sub-lattice: dense tiles and cells."""
+    from slabpro import configs
+    dx = 2.0
+    o = (np.arange(n_per_axis) + 0.5) / n_per_axis
+    ci = np.arange(cells)
+    px = (x0 / dx + ci[:, None, None, None] + o[None, None, :, None]) * dx
+    py = (y0 / dx + ci[None, :, None, None] + o[None, None, None, :]) * dx
+    px, py = np.broadcast_arrays(px, py)
+    px, py = px.ravel().copy(), py.ravel().copy()
+    hh = np.full(len(px), h)
+    area = (cells * dx) ** 2
+    return configs.finish_particles(px, py, hh, area * h, 300.0)
+
+
+@pytest.mark.parametrize("n_per_axis,cells,corner", [(6, 14, 60.0), (20, 4, 66.0)])
+def test_dense_tiles_sort_canonically(n_per_axis, cells, corner):
+    """Tiles beyond the in-LDS sort capacity (a tile of 4,761 particles in cells
+    of 36; one of 6,400 in cells of ~100) are ordered canonically in LDS rounds:
+    results are bitwise the same alone, in a mix and with another storage
+    re-sort period, and stay close to the FP64 form."""
+    from slabpro import abi
+    dense = (_dense(n_per_axis, cells, corner, corner), FRICS[1])
+    other = _scenario(21, FRICS[0])
+    a = _run_batch([dense], cap=400000, total=0.3, resort=1)
+    b = _run_batch([other, dense], cap=400000, total=0.3)
+    (ma, pa, ia), = a
+    (_, _, _), (mb, pb, ib) = b
+    assert ia["step"] == ib["step"] and ia["status"] == abi.SLOT_DONE
+    for k in ("x", "y", "vx", "vy", "h"):
+        assert np.array_equal(pa[k], pb[k]), k
+    assert np.array_equal(ma[0], mb[0])
+    # FP64 form of the same run (its own global-memory sort)
+    z, dx = _terrain()
+    part = dense[0]
+    r = abi.Run(z, dx, abi.params(rho=300.0, total_time=0.3, output_dt=0.5, precision=abi.FP64),
+                abi.friction(**dense[1]))
+    r.upload(part["x"], part["y"], part["h"], part["A"], part["m"])
+    r.prime()
+    r.run()
+    q = r.particles()
+    r.close()
+    assert np.max(np.abs(pa["x"] - q["x"])) < 1e-3
+    assert np.max(np.abs(pa["vx"] - q["vx"])) < 1e-2 * max(1.0, np.abs(q["vx"]).max())

@@ -699,6 +699,8 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
         const int slot = t / ntd, local = t - slot * ntd;
         if (!slot_active(slots[slot].status)) continue;
         const float g = (float)slots[slot].gravity;
+        // P_z only feeds the grid's vz output, written on output steps
+        const bool need_vz = slots[slot].out_step != 0;
         const int tx = local / nty, ty = local - tx * nty;
         const int ib = tx * TILE + lx, jb = ty * TILE + ly;
         for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += P2G_WG) s_acc[q] = 0.f;
@@ -711,14 +713,18 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
         const int r0 = cb[cid], r1 = cb[cid + 1];
         for (int r = r0 + part; r < r1; r += P2G_SPLIT) {
             const int s = idx[r];
-            const int cell = P.cell[s];
-            const int ci = cell >> 16, cj = cell & 0xffff;
-            const size_t fc = (size_t)ci * ny + cj;
-            const float fx = (float)(ci - ib) + P.ox[s], fy = (float)(cj - jb) + P.oy[s];
+            // live particles sit in cells [1, N-3] (outflow parks the others),
+            // so the stencil base is never clamped and the floor cell is the
+            // base plus (offset < 1/2)
+            const float ox = P.ox[s], oy = P.oy[s];
+            const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
+            const size_t fc = (size_t)(ib + ax) * ny + (jb + ay);
+            const float fx = (float)ax + ox, fy = (float)ay + oy;
             const float mp = P.m[s], hp = P.h[s];
             const float cth = tcth[fc];
             const float coeff = -mp * g * cth * cth;
-            const float vmx = P.vx[s] * mp, vmy = P.vy[s] * mp, vzm = P.vz[s] * mp, hm = hp * mp;
+            const float vmx = P.vx[s] * mp, vmy = P.vy[s] * mp, hm = hp * mp;
+            const float vzm = need_vz ? P.vz[s] * mp : 0.f;
             const float mbx = coeff * tgx[fc], mby = coeff * tgy[fc];
             // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
             const float ps = -0.5f * mp * g * cth * cth * hp;

@@ -44,10 +44,12 @@ ALG_BYTES_FP64 = {
 }
 STEP_BYTES_FP64 = {"particle": 104, "node": 120}
 # FP32 form: particle = cell int32 + offsets + vx vy vz h m (32 B read, 28 B
-# written by G2P); "p2g" = per-tile LDS reduction of the particle state,
-# "grid" = partial merge + grid update (terrain 12 B, max maps 16 B, vf 16 B).
+# written by G2P); "p2g" = per-tile LDS reduction of the particle state (reads
+# offsets vx vy h m: the cell follows from the tile and the offsets, vz is read
+# on output steps only), "grid" = partial merge + grid update (terrain 12 B,
+# max maps 16 B, vf 16 B).
 ALG_BYTES_FP32 = {
-    "p2g": {"particle": 32, "node": 0},
+    "p2g": {"particle": 24, "node": 0},
     "grid": {"particle": 0, "node": 12 + 16 + 16},
     "g2p": {"particle": 28 + 28, "node": 16 + 4},
     "tile_sort": {"particle": 8, "node": 0},
@@ -106,6 +108,19 @@ def cpu_baseline(cfg, steps, warm):
     return np_ * steps / el, el
 
 
+def copy_probe():
+    """A device copy of known size: FETCH_SIZE / WRITE_SIZE per dispatch of its
+    kernel give the factors that turn those counters into HBM bytes."""
+    import torch
+    n = 1 << 28  # 256 Mi float32 = 1 GiB
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied 5 x {a.numel() * 4} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,11 +144,16 @@ def main():
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--slots", type=int, default=128, help="serving: batch slots")
     ap.add_argument("--capacity", type=int, default=24_000_000, help="serving: device particle capacity")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 5 plain 1 GiB device copies (calibration of the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build's step runs on a single MI355X "
+        print(f"bench.py measures one GPU: the step runs on a single MI355X "
               f"(--gpus {args.gpus} is not supported; multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        return copy_probe()
 
     import numpy as np
     import slabpro

@@ -870,11 +870,10 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
 // --- G2P -------------------------------------------------------------------
 
 // G2P (src/simulation.cpp:877-940) for running slots; priming / finished
-// slots are carried unchanged.  Walks the particles in canonical order; on a
-// re-sort step writes them to the other buffer in that order, otherwise
-// updates them in place (the constant mass and creation index are not
-// rewritten).  Also writes the next keys and tile slots (indexed like the
-// storage) and the per-slot max |v|^2.
+// slots are carried unchanged.  On a re-sort step writes the particles to the
+// other buffer in canonical order, otherwise updates them in place (the
+// constant mass and creation index are not rewritten).  Also writes the next
+// keys and tile slots (indexed like the storage) and the per-slot max |v|^2.
 __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                              const int* __restrict__ idx, const float4* __restrict__ vf,
                                              const float* __restrict__ tz, const float* __restrict__ tgx,
@@ -885,16 +884,22 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
     const PB32 P = ecur(c, A, B);
     const bool reo = c->reorder != 0;
     const PB32 Q = reo ? enxt(c, A, B) : P;
+    // A re-sort step walks the particles in canonical order (idx) and writes
+    // them to the other buffer in that order.  Other steps update storage in
+    // place in storage order: every stored particle is keyed then (release
+    // compacts at once), storage was canonical at the last re-sort, and no
+    // result depends on the walk order (the sort orders each tile by cell and
+    // creation index).
     const int r = blockIdx.x * WG + threadIdx.x;
-    const bool in = r < c->n_sorted;
+    const bool in = r < (reo ? c->n_sorted : c->n_store);
     int t = 0, kk = -1, sl = 0, o = 0;
     float v2 = 0.f;
     bool pub = false;
     if (in) {
         const int nx = c->nx, ny = c->ny;
         const size_t nn = (size_t)nx * ny;
-        const int s = idx[r];
-        o = reo ? r : s; // storage index written
+        const int s = reo ? idx[r] : r;
+        o = r; // storage index written
         const int raw = P.slot[s];
         sl = raw & SLOT_MASK;
         bool out = (raw & SLOT_OUT) != 0;

@@ -687,9 +687,11 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
                                              const float* __restrict__ tgy, const float* __restrict__ tcth,
                                              float* __restrict__ tpart) {
     if (c->stop || c->compact) return;
+    constexpr int TCELLS = TILE + 1;
     __shared__ float s_acc[P2G_FIELDS * REGION_NODES];
+    __shared__ float s_ter[3 * TCELLS * TCELLS];
     const PB32 P = ecur(c, A, B);
-    const int nt = c->n_ptiles, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
+    const int nt = c->n_ptiles, nx = c->nx, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
     const float inv = (float)c->inv_dx;
     const int cid = threadIdx.x & (TILE_CELLS - 1), part = threadIdx.x >> 8;
     const int lx = cid >> 4, ly = cid & 15;
@@ -704,6 +706,22 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
         const int tx = local / nty, ty = local - tx * nty;
         const int ib = tx * TILE + lx, jb = ty * TILE + ly;
         for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += P2G_WG) s_acc[q] = 0.f;
+        // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged once
+        for (int q = threadIdx.x; q < TCELLS * TCELLS; q += P2G_WG) {
+            const int a = q / TCELLS, b = q - a * TCELLS;
+            const int gi = tx * TILE + a, gj = ty * TILE + b;
+            float vx = 0.f, vy = 0.f, vc = 0.f;
+            if (gi < nx && gj < ny) {
+                const size_t k = (size_t)gi * ny + gj;
+                vx = tgx[k];
+                vy = tgy[k];
+                vc = tcth[k];
+            }
+            s_ter[q] = vx;
+            s_ter[TCELLS * TCELLS + q] = vy;
+            s_ter[2 * TCELLS * TCELLS + q] = vc;
+        }
+        __syncthreads();
         float acc[9][P2G_FIELDS];
 #pragma unroll
         for (int k = 0; k < 9; ++k)
@@ -711,21 +729,32 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
             for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
         const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
         const int r0 = cb[cid], r1 = cb[cid + 1];
+        // the next particle's loads are issued before the current one is summed
+        float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
+        if (r0 + part < r1) {
+            const int sn = idx[r0 + part];
+            nox = P.ox[sn]; noy = P.oy[sn]; nmp = P.m[sn]; nhp = P.h[sn];
+            nvx = P.vx[sn]; nvy = P.vy[sn];
+            if (need_vz) nvz = P.vz[sn];
+        }
         for (int r = r0 + part; r < r1; r += P2G_SPLIT) {
-            const int s = idx[r];
+            const float ox = nox, oy = noy, mp = nmp, hp = nhp, pvx = nvx, pvy = nvy, pvz = nvz;
+            if (r + P2G_SPLIT < r1) {
+                const int sn = idx[r + P2G_SPLIT];
+                nox = P.ox[sn]; noy = P.oy[sn]; nmp = P.m[sn]; nhp = P.h[sn];
+                nvx = P.vx[sn]; nvy = P.vy[sn];
+                if (need_vz) nvz = P.vz[sn];
+            }
             // live particles sit in cells [1, N-3] (outflow parks the others),
             // so the stencil base is never clamped and the floor cell is the
             // base plus (offset < 1/2)
-            const float ox = P.ox[s], oy = P.oy[s];
             const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
-            const size_t fc = (size_t)(ib + ax) * ny + (jb + ay);
+            const int fl = (lx + ax) * TCELLS + (ly + ay);
             const float fx = (float)ax + ox, fy = (float)ay + oy;
-            const float mp = P.m[s], hp = P.h[s];
-            const float cth = tcth[fc];
+            const float cth = s_ter[2 * TCELLS * TCELLS + fl];
             const float coeff = -mp * g * cth * cth;
-            const float vmx = P.vx[s] * mp, vmy = P.vy[s] * mp, hm = hp * mp;
-            const float vzm = need_vz ? P.vz[s] * mp : 0.f;
-            const float mbx = coeff * tgx[fc], mby = coeff * tgy[fc];
+            const float vmx = pvx * mp, vmy = pvy * mp, hm = hp * mp, vzm = pvz * mp;
+            const float mbx = coeff * s_ter[fl], mby = coeff * s_ter[TCELLS * TCELLS + fl];
             // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
             const float ps = -0.5f * mp * g * cth * cth * hp;
             float wx[3], wy[3], dwx[3], dwy[3];

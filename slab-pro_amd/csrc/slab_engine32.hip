@@ -147,6 +147,22 @@ __device__ __forceinline__ int wave_slot_any(int t, bool valid, int* __restrict_
     return base + (lane - start);
 }
 
+// Counts each lane's particle into its tile: one atomic per run of equal tiles
+// among consecutive valid lanes, and no value returned (the slot inside the
+// tile is taken later, by the scatter).  Every lane of the wave must call this.
+__device__ __forceinline__ void wave_count_any(int t, bool valid, int* __restrict__ tile_count) {
+    const int lane = threadIdx.x & 63;
+    const int tprev = __shfl_up(t, 1, 64);
+    const bool vprev = __shfl_up((int)valid, 1, 64) != 0;
+    const bool head = valid && (lane == 0 || !vprev || tprev != t);
+    const unsigned long long vmask = __ballot(valid);
+    const unsigned long long hmask = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long brk = (hmask | ~vmask) & ~upto; // next head or next invalid lane
+    const int end = brk ? (__ffsll((long long)brk) - 1) : 64;
+    if (head) atomicAdd(&tile_count[t], end - lane);
+}
+
 // --- control ---------------------------------------------------------------
 
 __global__ void k_eclear(ECtl* c, Slot* __restrict__ slots) {
@@ -176,17 +192,28 @@ __global__ void __launch_bounds__(WG) k_ebegin(ECtl* c, Slot* __restrict__ slots
     }
     __shared__ int s_act, s_out;
     if (threadIdx.x == 0) s_act = s_out = 0;
+    // each slot's shards -> shard 0 (one wave per slot, shards cleared)
+    {
+        const int lane = threadIdx.x & 63;
+        for (int s = threadIdx.x >> 6; s < c->nslots; s += WG / 64) {
+            unsigned long long* b = vmx + (size_t)s * VMAX_SHARDS;
+            unsigned long long v = 0;
+            for (int k = lane; k < VMAX_SHARDS; k += 64) {
+                const unsigned long long q = b[k];
+                v = q > v ? q : v;
+                if (k) b[k] = 0ull;
+            }
+            v = wave_max_u64(v);
+            if (lane == 0) b[0] = v;
+        }
+    }
     __syncthreads();
     const int ep = c->epoch + 1;
     int act = 0;
     for (int s = threadIdx.x; s < c->nslots; s += WG) {
         Slot& S = slots[s];
-        unsigned long long v = 0;
-        for (int k = 0; k < VMAX_SHARDS; ++k) {
-            const unsigned long long q = vmx[(size_t)s * VMAX_SHARDS + k];
-            v = q > v ? q : v;
-            vmx[(size_t)s * VMAX_SHARDS + k] = 0ull;
-        }
+        const unsigned long long v = vmx[(size_t)s * VMAX_SHARDS];
+        vmx[(size_t)s * VMAX_SHARDS] = 0ull;
         if (S.status == SLOT_PRIMING) {
             S.dt = 0.0;
             S.out_step = 1;
@@ -283,8 +310,7 @@ __global__ void __launch_bounds__(WG) k_eend(ECtl* c, Slot* __restrict__ slots) 
 // --- keys of admitted particles ---------------------------------------------
 
 __global__ void __launch_bounds__(WG) k_ekeys(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B, int s0,
-                                              int s1, int* __restrict__ key, int* __restrict__ tslot,
-                                              int* __restrict__ tile_count) {
+                                              int s1, int* __restrict__ key, int* __restrict__ tile_count) {
     const PB32 P = ecur(c, A, B);
     const int s = s0 + blockIdx.x * WG + threadIdx.x;
     const bool in = s < s1;
@@ -293,12 +319,8 @@ __global__ void __launch_bounds__(WG) k_ekeys(ECtl* c, const Slot* __restrict__ 
         const int raw = P.slot[s], sl = raw & SLOT_MASK;
         k = ekey(c, slots[sl].status, sl, (raw & SLOT_OUT) != 0, P.cell[s], P.ox[s], P.oy[s], &t);
     }
-    const bool valid = in && k >= 0;
-    const int q = wave_slot_any(t, valid, tile_count);
-    if (in) {
-        key[s] = k;
-        tslot[s] = q;
-    }
+    wave_count_any(t, in && k >= 0, tile_count);
+    if (in) key[s] = k;
 }
 
 // Initial particle z, src/simulation.cpp:455-465 (4x4 quadratic weights).
@@ -445,14 +467,47 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const int* __restrict_
 
 // --- sort ------------------------------------------------------------------
 
-__global__ void k_escatter(ECtl* c, const int* __restrict__ key, const int* __restrict__ tslot,
-                           const int* __restrict__ tile_begin, int* __restrict__ tmp) {
+// Buckets the stored particles by tile: each takes the next slot of its tile
+// from a per-tile cursor (the sort clears the cursors of the tiles it orders).
+// A wave handles SC_PER groups of 64 consecutive particles and issues the
+// cursor atomics of all groups (one per run of equal tiles) before it waits
+// for any of them.  Arrival order inside a tile is arbitrary: the sort makes
+// it canonical.
+constexpr int SC_PER = 4;
+__global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict__ key,
+                                                 const int* __restrict__ tile_begin, int* __restrict__ cursor,
+                                                 int* __restrict__ tmp) {
     if (c->stop) return;
-    const int s = blockIdx.x * WG + threadIdx.x;
-    if (s >= c->n_store) return;
-    const int k = key[s];
-    if (k < 0) return;
-    tmp[tile_begin[k >> 8] + tslot[s]] = s;
+    const int n = c->n_store, lane = threadIdx.x & 63;
+    const int s0 = (blockIdx.x * WG + (threadIdx.x & ~63)) * SC_PER + lane;
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    int t[SC_PER], start[SC_PER], base[SC_PER];
+    bool valid[SC_PER];
+#pragma unroll
+    for (int j = 0; j < SC_PER; ++j) {
+        const int s = s0 + 64 * j;
+        const int k = s < n ? key[s] : -1;
+        valid[j] = k >= 0;
+        t[j] = valid[j] ? k >> 8 : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < SC_PER; ++j) {
+        const int tprev = __shfl_up(t[j], 1, 64);
+        const bool vprev = __shfl_up((int)valid[j], 1, 64) != 0;
+        const bool head = valid[j] && (lane == 0 || !vprev || tprev != t[j]);
+        const unsigned long long vmask = __ballot(valid[j]);
+        const unsigned long long hmask = __ballot(head);
+        const unsigned long long hle = hmask & upto;
+        const unsigned long long brk = (hmask | ~vmask) & ~upto;
+        start[j] = hle ? 63 - __clzll((long long)hle) : 0;
+        const int end = brk ? (__ffsll((long long)brk) - 1) : 64;
+        base[j] = head ? atomicAdd(&cursor[t[j]], end - lane) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < SC_PER; ++j) {
+        const int b = __shfl(base[j], start[j], 64);
+        if (valid[j]) tmp[tile_begin[t[j]] + b + (lane - start[j])] = s0 + 64 * j;
+    }
 }
 
 // Per particle tile: counting sort by cell, then each cell's run sorted by
@@ -498,7 +553,8 @@ __device__ __forceinline__ int pow2_ceil(int v) {
 
 __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int* __restrict__ key,
                                               const int* __restrict__ ptiles, int* __restrict__ tile_count,
-                                              const int* __restrict__ tile_begin, const int* __restrict__ tmp,
+                                              int* __restrict__ cursor, const int* __restrict__ tile_begin,
+                                              const int* __restrict__ tmp,
                                               int* __restrict__ idx, int* __restrict__ cell_begin) {
     if (c->stop) return;
     __shared__ int s_hist[TILE_CELLS];
@@ -533,6 +589,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
             if (threadIdx.x == 0) {
                 cb[TILE_CELLS] = b + n;
                 tile_count[t] = 0;
+                cursor[t] = 0;
             }
             __syncthreads();
             continue;
@@ -568,6 +625,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
             if (threadIdx.x == 0) {
                 cb[TILE_CELLS] = b + n;
                 tile_count[t] = 0;
+                cursor[t] = 0;
             }
         }
         __syncthreads();
@@ -704,7 +762,6 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
         // P_z only feeds the grid's vz output, written on output steps
         const bool need_vz = slots[slot].out_step != 0;
         const int tx = local / nty, ty = local - tx * nty;
-        const int ib = tx * TILE + lx, jb = ty * TILE + ly;
         for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += P2G_WG) s_acc[q] = 0.f;
         // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged once
         for (int q = threadIdx.x; q < TCELLS * TCELLS; q += P2G_WG) {
@@ -907,8 +964,7 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
                                              const int* __restrict__ idx, const float4* __restrict__ vf,
                                              const float* __restrict__ tz, const float* __restrict__ tgx,
                                              const float* __restrict__ tgy, int* __restrict__ key,
-                                             int* __restrict__ tslot, int* __restrict__ tile_count,
-                                             unsigned long long* __restrict__ vmx) {
+                                             int* __restrict__ tile_count, unsigned long long* __restrict__ vmx) {
     if (c->stop) return;
     const PB32 P = ecur(c, A, B);
     const bool reo = c->reorder != 0;
@@ -1014,11 +1070,8 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
         pub = slot_active(st) && !out;
         v2 = vxp * vxp + vyp * vyp;
     }
-    const int q = wave_slot_any(t, in && kk >= 0, tile_count);
-    if (in) {
-        key[o] = kk;
-        tslot[o] = q;
-    }
+    wave_count_any(t, in && kk >= 0, tile_count);
+    if (in) key[o] = kk;
     publish_vmax(sl, pub, v2, vmx);
 }
 
@@ -1170,12 +1223,13 @@ struct Engine32 {
     float4* vf = nullptr;
     float *om = nullptr, *oh = nullptr, *ovz = nullptr;
     unsigned long long* vmx = nullptr;
-    int *tile_count = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr, *tile_pos = nullptr;
+    int *tile_count = nullptr, *cursor = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr,
+        *tile_pos = nullptr;
     int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr;
     int* cell_begin = nullptr;
     float* tpart = nullptr;
     PB32 A{}, B{};
-    int *key = nullptr, *tslot = nullptr, *idx = nullptr, *tmp = nullptr;
+    int *key = nullptr, *idx = nullptr, *tmp = nullptr;
     double* dpart = nullptr;
     std::vector<void*> allocs, pallocs;
     std::vector<int64_t> admit_base; // storage offset of each slot's particles at admission
@@ -1262,10 +1316,13 @@ void enqueue_estep(Engine32* e, int step) {
         k_eoffsets<<<e->nblk, WG, 0, s>>>(c, e->tile_count, e->ntiles, e->boff, e->tile_begin, e->ptiles, e->gtiles,
                                            e->pstamp, e->gstamp, e->tile_pos);
     });
-    tk(e, 4, step, [&] { k_escatter<<<capblocks(e), WG, 0, s>>>(c, e->key, e->tslot, e->tile_begin, e->tmp); });
+    tk(e, 4, step, [&] {
+        const unsigned nb = (unsigned)std::max<int64_t>(1, (e->cap + WG * SC_PER - 1) / (WG * SC_PER));
+        k_escatter<<<nb, WG, 0, s>>>(c, e->key, e->tile_begin, e->cursor, e->tmp);
+    });
     tk(e, 5, step, [&] {
-        k_esort<<<pb, WG, 0, s>>>(c, e->A, e->B, e->key, e->ptiles, e->tile_count, e->tile_begin, e->tmp, e->idx,
-                                  e->cell_begin);
+        k_esort<<<pb, WG, 0, s>>>(c, e->A, e->B, e->key, e->ptiles, e->tile_count, e->cursor, e->tile_begin, e->tmp,
+                                  e->idx, e->cell_begin);
     });
     tk(e, 6, step, [&] {
         k_ep2g<<<pb, P2G_WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
@@ -1277,7 +1334,7 @@ void enqueue_estep(Engine32* e, int step) {
     });
     tk(e, 8, step, [&] {
         k_eg2p<<<capblocks(e), WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->tz, e->tgx, e->tgy, e->key,
-                                           e->tslot, e->tile_count, e->vmx);
+                                           e->tile_count, e->vmx);
     });
     tk(e, 9, step, [&] { k_eend<<<1, WG, 0, s>>>(c, e->d_slots); });
 }
@@ -1352,8 +1409,8 @@ void terrain32(const double* z, int nx, int ny, double inv, std::vector<float>& 
 } // namespace
 
 int64_t engine_particle_bytes() {
-    // two state buffers (cell, ox, oy, vx, vy, vz, h, m, z, pid, slot = 44 B) + key, tslot, idx, tmp
-    return 2 * 44 + 16;
+    // two state buffers (cell, ox, oy, vx, vy, vz, h, m, z, pid, slot = 44 B) + key, idx, tmp
+    return 2 * 44 + 12;
 }
 
 int64_t engine_slot_bytes(const Engine32* e) {
@@ -1408,7 +1465,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
               ealloc(e->allocs, &e->tgy, e->nn, &why) && ealloc(e->allocs, &e->tcth, e->nn, &why) &&
               ealloc(e->allocs, &e->hmax, sn, &why) && ealloc(e->allocs, &e->vmax2, sn, &why) &&
               ealloc(e->allocs, &e->vf, sn, &why) && ealloc(e->allocs, &e->vmx, (size_t)nslots * VMAX_SHARDS, &why) &&
-              ealloc(e->allocs, &e->tile_count, e->ntiles, &why) && ealloc(e->allocs, &e->tile_begin, e->ntiles, &why) &&
+              ealloc(e->allocs, &e->tile_count, e->ntiles, &why) && ealloc(e->allocs, &e->cursor, e->ntiles, &why) &&
+              ealloc(e->allocs, &e->tile_begin, e->ntiles, &why) &&
               ealloc(e->allocs, &e->pstamp, e->ntiles, &why) && ealloc(e->allocs, &e->gstamp, e->ntiles, &why) &&
               ealloc(e->allocs, &e->tile_pos, e->ntiles, &why) && ealloc(e->allocs, &e->ptiles, e->ntiles, &why) &&
               ealloc(e->allocs, &e->gtiles, e->ntiles, &why) && ealloc(e->allocs, &e->bsum, 3 * (size_t)e->nblk, &why) &&
@@ -1425,8 +1483,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
             ok = ok && ealloc(e->pallocs, q, max_particles, &why);
         ok = ok && ealloc(e->pallocs, &b->pid, max_particles, &why) && ealloc(e->pallocs, &b->slot, max_particles, &why);
     }
-    ok = ok && ealloc(e->pallocs, &e->key, max_particles, &why) && ealloc(e->pallocs, &e->tslot, max_particles, &why) &&
-         ealloc(e->pallocs, &e->idx, max_particles, &why) && ealloc(e->pallocs, &e->tmp, max_particles, &why);
+    ok = ok && ealloc(e->pallocs, &e->key, max_particles, &why) && ealloc(e->pallocs, &e->idx, max_particles, &why) &&
+         ealloc(e->pallocs, &e->tmp, max_particles, &why);
     if (!ok) return fail(why);
     std::vector<float> gx, gy, cth, zf(e->nn);
     terrain32(z, nx, ny, e->inv, gx, gy, cth);
@@ -1440,6 +1498,7 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
          hipMemsetAsync(e->vf, 0, sn * sizeof(float4), s) == hipSuccess &&
          hipMemsetAsync(e->vmx, 0, (size_t)nslots * VMAX_SHARDS * 8, s) == hipSuccess &&
          hipMemsetAsync(e->tile_count, 0, (size_t)e->ntiles * 4, s) == hipSuccess &&
+         hipMemsetAsync(e->cursor, 0, (size_t)e->ntiles * 4, s) == hipSuccess &&
          hipMemsetAsync(e->pstamp, 0xff, (size_t)e->ntiles * 4, s) == hipSuccess &&
          hipMemsetAsync(e->gstamp, 0xff, (size_t)e->ntiles * 4, s) == hipSuccess;
     if (ok && single)
@@ -1590,8 +1649,7 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
     if (!push_all(e, err)) return -1;
     const unsigned nb = (unsigned)((n + WG - 1) / WG);
     k_ez<<<nb, WG, 0, s>>>(e->d_c, e->A, e->B, (int)base, (int)(base + np), e->tz);
-    k_ekeys<<<nb, WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, (int)base, (int)(base + np), e->key, e->tslot,
-                              e->tile_count);
+    k_ekeys<<<nb, WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, (int)base, (int)(base + np), e->key, e->tile_count);
     if (!hip_ok(hipGetLastError(), "admit kernels", err) || !hip_ok(hipStreamSynchronize(s), "admit", err)) return -1;
     e->admit_base[slot] = base;
     return slot;
@@ -1641,8 +1699,10 @@ bool engine_release(Engine32* e, int slot, std::string* err) {
     hipStream_t s = e->stream;
     const int n = e->h_c->n_store;
     if (n > 0) {
-        if (!hip_ok(hipMemsetAsync(e->tile_count, 0, (size_t)e->ntiles * 4, s), "release", err)) return false;
-        k_ekeys<<<(unsigned)((n + WG - 1) / WG), WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, 0, n, e->key, e->tslot,
+        if (!hip_ok(hipMemsetAsync(e->tile_count, 0, (size_t)e->ntiles * 4, s), "release", err) ||
+            !hip_ok(hipMemsetAsync(e->cursor, 0, (size_t)e->ntiles * 4, s), "release", err))
+            return false;
+        k_ekeys<<<(unsigned)((n + WG - 1) / WG), WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, 0, n, e->key,
                                                              e->tile_count);
         e->h_c->compact = 1;
         e->h_c->stop = RUNNING;

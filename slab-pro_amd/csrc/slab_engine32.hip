@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const int* __restrict_
 // cursor atomics of all groups (one per run of equal tiles) before it waits
 // for any of them.  Arrival order inside a tile is arbitrary: the sort makes
 // it canonical.
-constexpr int SC_PER = 4;
+constexpr int SC_PER = 8;
 __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict__ key,
                                                  const int* __restrict__ tile_begin, int* __restrict__ cursor,
                                                  int* __restrict__ tmp) {

@@ -57,7 +57,7 @@ def build(verbose=False):
     objs = []
     for src, flags in SOURCES.items():
         obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
-        extra = os.environ.get("SLAB_HIP_FLAGS", "").split()  # e.g. -DSLAB_P2G_SPLIT=2
+        extra = os.environ.get("SLAB_HIP_FLAGS", "").split()  # extra hipcc flags, e.g. -save-temps
         out = _run([HIPCC] + COMMON + flags + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
         if verbose and out:
             print(out)

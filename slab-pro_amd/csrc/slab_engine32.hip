@@ -54,6 +54,17 @@ __device__ __forceinline__ int ebase(int ci, float o, int nmax) {
 
 __device__ __forceinline__ bool slot_active(int st) { return st == SLOT_PRIMING || st == SLOT_RUNNING; }
 
+// The 8 XCDs (each with its own L2) take workgroups round-robin: block b runs
+// on XCD b % 8.  G2P renumbers its blocks so that each XCD walks one
+// contiguous eighth of the particles, whose grid nodes then stay in that XCD's
+// L2.  (The scatter keeps the plain order: there the renumbering puts
+// concurrent blocks on the same tile cursors and the atomics contend.)  The
+// grid size is a multiple of 8.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_block() {
+    return (int)(blockIdx.x % N_XCD) * (int)(gridDim.x / N_XCD) + (int)(blockIdx.x / N_XCD);
+}
+
 
 // True when a cell lies outside [1, N-3] on either axis (outflow).
 __device__ __forceinline__ bool cell_out(const ECtl* c, int ci, int cj) {
@@ -82,14 +93,11 @@ __device__ __forceinline__ int ekey(const ECtl* c, int st, int slot, bool out, i
 }
 
 // Per-slot max |v|^2 as a sharded integer max of the IEEE bits (|v|^2 >= 0;
-// NaN sorts above +inf).  Particles are sorted by slot, so a block almost
-// always holds one slot: it reduces in LDS and issues one atomic on shard
-// blockIdx % VMAX_SHARDS; blocks that straddle slots fall back to one atomic
-// per wave (or per lane for a wave that straddles).  All threads must call.
+// NaN sorts above +inf).  Particles are stored by slot, so a wave almost
+// always holds one slot: it reduces across lanes and issues one atomic on its
+// shard (wave index % VMAX_SHARDS); a wave that straddles slots issues one
+// atomic per lane.  Every lane of the wave must call this.
 __device__ __forceinline__ void publish_vmax(int slot, bool valid, float v2, unsigned long long* __restrict__ vmx) {
-    __shared__ int s_slot[4];
-    __shared__ int s_uni[4];
-    __shared__ unsigned long long s_max[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const unsigned long long bits = valid ? (unsigned long long)__double_as_longlong((double)v2) : 0ull;
     const unsigned long long vmask = __ballot(valid);
@@ -97,34 +105,12 @@ __device__ __forceinline__ void publish_vmax(int slot, bool valid, float v2, uns
     const int s0 = __shfl(slot, first, 64);
     const unsigned long long same = __ballot(valid && slot == s0);
     const unsigned long long m = wave_max_u64(bits);
-    const bool uni = same == vmask;
-    if (lane == 0) {
-        s_slot[wv] = vmask ? s0 : -1;
-        s_uni[wv] = uni;
-        s_max[wv] = m;
+    const unsigned sh = (blockIdx.x * (WG / 64) + wv) & (VMAX_SHARDS - 1);
+    if (same == vmask) {
+        if (lane == 0 && vmask) atomicMax(&vmx[(size_t)s0 * VMAX_SHARDS + sh], m);
+    } else if (valid) {
+        atomicMax(&vmx[(size_t)slot * VMAX_SHARDS + sh], bits);
     }
-    if (!uni && valid) atomicMax(&vmx[(size_t)slot * VMAX_SHARDS + (blockIdx.x & (VMAX_SHARDS - 1))], bits);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned sh = blockIdx.x & (VMAX_SHARDS - 1);
-        int bs = -1;
-        bool block_uni = true;
-        for (int k = 0; k < 4; ++k) {
-            if (s_slot[k] < 0) continue;
-            if (!s_uni[k] || (bs >= 0 && s_slot[k] != bs)) block_uni = false;
-            if (bs < 0) bs = s_slot[k];
-        }
-        if (bs >= 0 && block_uni) {
-            unsigned long long mm = 0;
-            for (int k = 0; k < 4; ++k)
-                if (s_slot[k] >= 0) mm = s_max[k] > mm ? s_max[k] : mm;
-            atomicMax(&vmx[(size_t)bs * VMAX_SHARDS + sh], mm);
-        } else {
-            for (int k = 0; k < 4; ++k)
-                if (s_slot[k] >= 0 && s_uni[k]) atomicMax(&vmx[(size_t)s_slot[k] * VMAX_SHARDS + sh], s_max[k]);
-        }
-    }
-    __syncthreads(); // the LDS slots are reused by the next call
 }
 
 // Tile slots for arbitrary valid masks: one atomic per run of equal tiles
@@ -975,7 +961,7 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
     // compacts at once), storage was canonical at the last re-sort, and no
     // result depends on the walk order (the sort orders each tile by cell and
     // creation index).
-    const int r = blockIdx.x * WG + threadIdx.x;
+    const int r = xcd_block() * WG + threadIdx.x;
     const bool in = r < (reo ? c->n_sorted : c->n_store);
     int t = 0, kk = -1, sl = 0, o = 0;
     float v2 = 0.f;
@@ -1302,7 +1288,8 @@ void tk(Engine32* e, int kid, int step, F&& f) {
     e->pending.push_back({kid, step, a, b});
 }
 
-unsigned capblocks(const Engine32* e) { return (unsigned)std::max<int64_t>(1, (e->cap + WG - 1) / WG); }
+// at least n blocks, a multiple of N_XCD (for xcd_block)
+unsigned xcd_grid(int64_t n) { return (unsigned)((std::max<int64_t>(1, n) + N_XCD - 1) / N_XCD * N_XCD); }
 
 void enqueue_estep(Engine32* e, int step) {
     hipStream_t s = e->stream;
@@ -1333,7 +1320,7 @@ void enqueue_estep(Engine32* e, int step) {
                                   e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz);
     });
     tk(e, 8, step, [&] {
-        k_eg2p<<<capblocks(e), WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->tz, e->tgx, e->tgy, e->key,
+        k_eg2p<<<xcd_grid((e->cap + WG - 1) / WG), WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->tz, e->tgx, e->tgy, e->key,
                                            e->tile_count, e->vmx);
     });
     tk(e, 9, step, [&] { k_eend<<<1, WG, 0, s>>>(c, e->d_slots); });

@@ -710,22 +710,13 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
 
 // --- P2G -------------------------------------------------------------------
 
-// Threads per cell in P2G: with 2, the particles of a cell alternate between
-// two threads (halving the wait on dense cells) and the 9 flush phases run
-// once per thread group, in group order.
-#ifndef SLAB_P2G_SPLIT
-#define SLAB_P2G_SPLIT 1
-#endif
-constexpr int P2G_SPLIT = SLAB_P2G_SPLIT;
-constexpr int P2G_WG = WG * P2G_SPLIT;
-
 // P2G, src/simulation.cpp:726-798, as a per-tile segmented reduction without
 // atomics: one thread per stencil-base cell sums its particles (canonical
 // order) into 9 stencil slots x 7 fields in registers; 9 phases then add slot
 // (d,e) of every cell into node cell+(d,e) of an 18x18-node LDS region (one-to
 // -one within a phase: no conflicts, fixed order); the region is this tile's
 // partial, merged per node by k_egrid.
-__global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+__global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                              const int* __restrict__ idx, const int* __restrict__ ptiles,
                                              const int* __restrict__ cell_begin, const float* __restrict__ tgx,
                                              const float* __restrict__ tgy, const float* __restrict__ tcth,
@@ -737,7 +728,7 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
     const PB32 P = ecur(c, A, B);
     const int nt = c->n_ptiles, nx = c->nx, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
     const float inv = (float)c->inv_dx;
-    const int cid = threadIdx.x & (TILE_CELLS - 1), part = threadIdx.x >> 8;
+    const int cid = threadIdx.x;
     const int lx = cid >> 4, ly = cid & 15;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         const int t = ptiles[w];
@@ -748,9 +739,9 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
         // P_z only feeds the grid's vz output, written on output steps
         const bool need_vz = slots[slot].out_step != 0;
         const int tx = local / nty, ty = local - tx * nty;
-        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += P2G_WG) s_acc[q] = 0.f;
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
         // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged once
-        for (int q = threadIdx.x; q < TCELLS * TCELLS; q += P2G_WG) {
+        for (int q = threadIdx.x; q < TCELLS * TCELLS; q += WG) {
             const int a = q / TCELLS, b = q - a * TCELLS;
             const int gi = tx * TILE + a, gj = ty * TILE + b;
             float vx = 0.f, vy = 0.f, vc = 0.f;
@@ -774,16 +765,16 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
         const int r0 = cb[cid], r1 = cb[cid + 1];
         // the next particle's loads are issued before the current one is summed
         float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
-        if (r0 + part < r1) {
-            const int sn = idx[r0 + part];
+        if (r0 < r1) {
+            const int sn = idx[r0];
             nox = P.ox[sn]; noy = P.oy[sn]; nmp = P.m[sn]; nhp = P.h[sn];
             nvx = P.vx[sn]; nvy = P.vy[sn];
             if (need_vz) nvz = P.vz[sn];
         }
-        for (int r = r0 + part; r < r1; r += P2G_SPLIT) {
+        for (int r = r0; r < r1; ++r) {
             const float ox = nox, oy = noy, mp = nmp, hp = nhp, pvx = nvx, pvy = nvy, pvz = nvz;
-            if (r + P2G_SPLIT < r1) {
-                const int sn = idx[r + P2G_SPLIT];
+            if (r + 1 < r1) {
+                const int sn = idx[r + 1];
                 nox = P.ox[sn]; noy = P.oy[sn]; nmp = P.m[sn]; nhp = P.h[sn];
                 nvx = P.vx[sn]; nvy = P.vy[sn];
                 if (need_vz) nvz = P.vz[sn];
@@ -822,19 +813,15 @@ __global__ void __launch_bounds__(P2G_WG) k_ep2g(ECtl* c, const Slot* __restrict
             }
         }
         __syncthreads();
-        for (int pp = 0; pp < P2G_SPLIT; ++pp) {
 #pragma unroll
-            for (int ph = 0; ph < 9; ++ph) {
-                const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
-                if (part == pp) {
+        for (int ph = 0; ph < 9; ++ph) {
+            const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
 #pragma unroll
-                    for (int f = 0; f < P2G_FIELDS; ++f) s_acc[f * REGION_NODES + node] += acc[ph][f];
-                }
-                __syncthreads();
-            }
+            for (int f = 0; f < P2G_FIELDS; ++f) s_acc[f * REGION_NODES + node] += acc[ph][f];
+            __syncthreads();
         }
         float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
-        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += P2G_WG) dst[q] = s_acc[q];
+        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) dst[q] = s_acc[q];
         __syncthreads();
     }
 }
@@ -1312,7 +1299,7 @@ void enqueue_estep(Engine32* e, int step) {
                                   e->idx, e->cell_begin);
     });
     tk(e, 6, step, [&] {
-        k_ep2g<<<pb, P2G_WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
+        k_ep2g<<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
                                  e->tpart);
     });
     tk(e, 7, step, [&] {

@@ -165,6 +165,7 @@ unsigned pblocks(const slab_run* r) { return (unsigned)((r->np + WG - 1) / WG); 
 unsigned tblocks(const slab_run* r) { return (unsigned)std::min(r->ntiles, kPersist); }
 
 void enqueue_keys(slab_run* r) {
+    k_part_init<<<1, WG, 0, r->stream>>>(r->part);
     k_keys<<<pblocks(r), WG, 0, r->stream>>>(r->d_ctl, r->A, r->B, r->key, r->tslot, r->tile_count, r->part);
 }
 
@@ -512,7 +513,7 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
     // P2G partials: one region per listed particle tile (at most one tile per particle)
     if (st == SLAB_OK)
         st = dalloc(&r->tpart, (size_t)std::min<int64_t>(r->ntiles, np) * kP2G64PartialDoubles, r->part_allocs);
-    if (st == SLAB_OK) st = dalloc(&r->part, 5 * (size_t)((np + WG - 1) / WG), r->part_allocs);
+    if (st == SLAB_OK) st = dalloc(&r->part, 5 * (size_t)PART_SHARDS, r->part_allocs);
     if (st != SLAB_OK) return st;
     TRY(pull_ctl(r));
     const double rho = r->h_ctl->rho;

@@ -18,6 +18,7 @@ constexpr int CELL_STRIDE = TILE_CELLS + 1; // per-tile cell_begin entries (last
 constexpr int WG = 256;
 constexpr int SCAN_ITEMS = 4; // tiles per thread in the tile scan
 constexpr int SCAN_BLOCK = WG * SCAN_ITEMS;
+constexpr int PART_SHARDS = 64; // shards of the FP64 {bbox, max |v|^2} partials (5 x u64 each)
 
 enum StopCode { RUNNING = 0, STOP_FRAME = 1, STOP_DONE = 2, STOP_ERROR = 3 };
 

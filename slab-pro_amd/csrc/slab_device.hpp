@@ -22,9 +22,10 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 }
 
 // Block-wide (256 threads = 4 waves) reduction of |v|^2 max and the bounding
-// box of new positions into one partial per block (no global atomics: the
-// consumer reduces the partials).  |v|^2 >= 0, so its IEEE bits order like
-// integers; NaN bits sort above +inf and are caught by ctl_begin.
+// box of new positions, merged by one integer min/max atomic per field into
+// shard blockIdx % PART_SHARDS (order-independent); the consumer reduces the
+// shards and resets them (reset_partials).  |v|^2 >= 0, so its IEEE bits order
+// like integers; NaN bits sort above +inf and are caught by ctl_begin.
 __device__ __forceinline__ void block_partials(double v2, unsigned long long bx0, unsigned long long bx1, unsigned long long by0,
                                unsigned long long by1, unsigned long long* __restrict__ part) {
     __shared__ unsigned long long s_b[4][5];
@@ -51,20 +52,30 @@ __device__ __forceinline__ void block_partials(double v2, unsigned long long bx0
             a2 = s_b[k][2] < a2 ? s_b[k][2] : a2;
             a3 = s_b[k][3] > a3 ? s_b[k][3] : a3;
         }
-        unsigned long long* q = part + 5 * (size_t)blockIdx.x;
-        q[0] = a0;
-        q[1] = a1;
-        q[2] = a2;
-        q[3] = a3;
-        q[4] = vmax;
+        unsigned long long* q = part + 5 * (size_t)(blockIdx.x % PART_SHARDS);
+        atomicMin(&q[0], a0);
+        atomicMax(&q[1], a1);
+        atomicMin(&q[2], a2);
+        atomicMax(&q[3], a3);
+        atomicMax(&q[4], vmax);
     }
 }
 
-// Reduces the per-block partials (all 256 threads of a 1-block launch);
-// result in out[5] on thread 0.
-__device__ __forceinline__ void reduce_partials(const unsigned long long* __restrict__ part, int nb, unsigned long long out[5]) {
+// Sets every shard back to the identity: ~0 for the min fields (0, 2), 0 for
+// the max fields (1, 3, 4).  Threads of one block.
+__device__ __forceinline__ void reset_partials(unsigned long long* __restrict__ part) {
+    for (int q = threadIdx.x; q < 5 * PART_SHARDS; q += blockDim.x) {
+        const int f = q % 5;
+        part[q] = (f == 0 || f == 2) ? ~0ull : 0ull;
+    }
+}
+
+// Reduces the partial shards (all 256 threads of a 1-block launch); result in
+// out[5] on thread 0.
+__device__ __forceinline__ void reduce_partials(const unsigned long long* __restrict__ part, unsigned long long out[5]) {
     __shared__ unsigned long long s_r[4][5];
     unsigned long long a0 = ~0ull, a1 = 0, a2 = ~0ull, a3 = 0, v = 0;
+    const int nb = PART_SHARDS;
     for (int b = threadIdx.x; b < nb; b += WG) {
         const unsigned long long* q = part + 5 * (size_t)b;
         a0 = q[0] < a0 ? q[0] : a0;

@@ -79,25 +79,33 @@ __global__ void k_ctl_clear(Ctl* c) {
 }
 
 // updateDt, src/simulation.cpp:565-582, plus the bookkeeping of one step.
-// One block of 256 threads: reduces the partials of the previous g2p (or of
-// k_keys after upload) to max |v_p|^2 and the bounding box of the positions
-// this step's P2G uses.
-__global__ void __launch_bounds__(WG) k_ctl_begin(Ctl* c, const unsigned long long* __restrict__ part) {
+// One block of 256 threads: reduces the partial shards of the previous g2p (or
+// of k_keys after upload) to max |v_p|^2 and the bounding box of the positions
+// this step's P2G uses; a step that proceeds resets the shards for its g2p
+// (a stopped one keeps them for diagnostics).
+__global__ void __launch_bounds__(WG) k_ctl_begin(Ctl* c, unsigned long long* __restrict__ part) {
     if (c->stop) return;
     __shared__ unsigned long long s_out[5];
-    reduce_partials(part, (c->np + WG - 1) / WG, s_out);
-    if (threadIdx.x != 0) return;
-    if (!(c->time < c->total_time)) { // while condition, simulation.cpp:500
-        c->stop = STOP_DONE;
-        return;
+    __shared__ int s_go;
+    reduce_partials(part, s_out);
+    if (threadIdx.x == 0) {
+        s_go = 0;
+        if (!(c->time < c->total_time)) { // while condition, simulation.cpp:500
+            c->stop = STOP_DONE;
+        } else if (!(__longlong_as_double(s_out[4]) <= 1.7976931348623157e308)) {
+            c->vmax2_bits = s_out[4];
+            c->err = 5; // SLAB_ENAN
+            c->stop = STOP_ERROR;
+        } else {
+            s_go = 1;
+        }
     }
+    __syncthreads();
+    if (!s_go) return;
+    reset_partials(part);
+    if (threadIdx.x != 0) return;
     c->vmax2_bits = s_out[4];
     const double v2 = __longlong_as_double(s_out[4]);
-    if (!(v2 <= 1.7976931348623157e308)) {
-        c->err = 5; // SLAB_ENAN
-        c->stop = STOP_ERROR;
-        return;
-    }
     const double max_speed = sqrt(v2);
     double dt;
     if (max_speed > 0.0) {
@@ -134,9 +142,11 @@ __global__ void k_ctl_end(Ctl* c) {
 __global__ void k_prime_pre(Ctl* c) { c->stop = RUNNING; }
 
 // simulate() priming with dt = 0 (src/simulation.cpp:493-495, simulation.hpp:39).
-__global__ void __launch_bounds__(WG) k_ctl_prime(Ctl* c, const unsigned long long* __restrict__ part) {
+__global__ void __launch_bounds__(WG) k_ctl_prime(Ctl* c, unsigned long long* __restrict__ part) {
     __shared__ unsigned long long s_out[5];
-    reduce_partials(part, (c->np + WG - 1) / WG, s_out);
+    reduce_partials(part, s_out);
+    __syncthreads();
+    reset_partials(part);
     if (threadIdx.x != 0) return;
     c->dt = 0.0;
     c->epoch += 1;
@@ -147,7 +157,8 @@ __global__ void __launch_bounds__(WG) k_ctl_prime(Ctl* c, const unsigned long lo
 }
 
 // Keys, tile counts, |v|^2 max and bounding box of the current particles
-// (used once after upload; afterwards g2p produces them).
+// (used once after upload; afterwards g2p produces them).  Partial shards
+// start at the identity (k_part_init).
 __global__ void __launch_bounds__(WG) k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* __restrict__ key,
                                              int* __restrict__ tslot, int* __restrict__ tile_count,
                                              unsigned long long* __restrict__ part) {
@@ -172,6 +183,8 @@ __global__ void __launch_bounds__(WG) k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* __
     }
     block_partials(v2, bx0, bx1, by0, by1, part);
 }
+
+__global__ void k_part_init(unsigned long long* __restrict__ part) { reset_partials(part); }
 
 // Initial particle z: 4x4 quadratic interpolation of terrain z,
 // src/simulation.cpp:455-465.
@@ -793,7 +806,7 @@ __global__ void __launch_bounds__(WG) k_diag_final(const Ctl* c, const double* _
                                                    const double* __restrict__ pp, int npb,
                                                    const unsigned long long* __restrict__ part, double* __restrict__ out) {
     __shared__ unsigned long long s_out[5];
-    reduce_partials(part, (c->np + WG - 1) / WG, s_out);
+    reduce_partials(part, s_out);
     if (threadIdx.x != 0) return;
     out[6] = sqrt(__longlong_as_double(s_out[4]));
     double g[4] = {0, 0, 0, 0}, p[2] = {0, 0};

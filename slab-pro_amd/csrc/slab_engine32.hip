@@ -165,8 +165,27 @@ __global__ void k_eclear(ECtl* c, Slot* __restrict__ slots) {
 // Start of a step: per slot, the while condition (src/simulation.cpp:500) and
 // updateDt (565-582) from the slot's own max |v|; priming slots get dt = 0
 // (simulation.hpp:39, simulation.cpp:493-495).
+// Each slot's dt shards -> shard 0, the others cleared: one wave per slot.
+__device__ __forceinline__ void vmax_slot(unsigned long long* __restrict__ b) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long v = 0;
+    for (int k = lane; k < VMAX_SHARDS; k += 64) {
+        const unsigned long long q = b[k];
+        v = q > v ? q : v;
+        if (k) b[k] = 0ull;
+    }
+    v = wave_max_u64(v);
+    if (lane == 0) b[0] = v;
+}
+
+// Batches: the shard reduction of every slot, one wave (block) per slot.
+__global__ void __launch_bounds__(64) k_evmax(const ECtl* c, unsigned long long* __restrict__ vmx) {
+    if (c->stop || c->compact) return;
+    vmax_slot(vmx + (size_t)blockIdx.x * VMAX_SHARDS);
+}
+
 __global__ void __launch_bounds__(WG) k_ebegin(ECtl* c, Slot* __restrict__ slots,
-                                               unsigned long long* __restrict__ vmx) {
+                                               unsigned long long* __restrict__ vmx, int reduced) {
     if (c->stop) return;
     if (c->compact) { // re-sort pass: no slot advances
         if (threadIdx.x == 0) {
@@ -178,21 +197,9 @@ __global__ void __launch_bounds__(WG) k_ebegin(ECtl* c, Slot* __restrict__ slots
     }
     __shared__ int s_act, s_out;
     if (threadIdx.x == 0) s_act = s_out = 0;
-    // each slot's shards -> shard 0 (one wave per slot, shards cleared)
-    {
-        const int lane = threadIdx.x & 63;
-        for (int s = threadIdx.x >> 6; s < c->nslots; s += WG / 64) {
-            unsigned long long* b = vmx + (size_t)s * VMAX_SHARDS;
-            unsigned long long v = 0;
-            for (int k = lane; k < VMAX_SHARDS; k += 64) {
-                const unsigned long long q = b[k];
-                v = q > v ? q : v;
-                if (k) b[k] = 0ull;
-            }
-            v = wave_max_u64(v);
-            if (lane == 0) b[0] = v;
-        }
-    }
+    // each slot's shards -> shard 0, unless k_evmax did it (batches)
+    if (!reduced)
+        for (int s = threadIdx.x >> 6; s < c->nslots; s += WG / 64) vmax_slot(vmx + (size_t)s * VMAX_SHARDS);
     __syncthreads();
     const int ep = c->epoch + 1;
     int act = 0;
@@ -1283,7 +1290,11 @@ void enqueue_estep(Engine32* e, int step) {
     ECtl* c = e->d_c;
     const unsigned tb = (unsigned)std::max(1, std::min(e->ntiles, e->persist));
     const unsigned pb = (unsigned)std::max(1, std::min(e->tile_cap, e->persist));
-    tk(e, 0, step, [&] { k_ebegin<<<1, WG, 0, s>>>(c, e->d_slots, e->vmx); });
+    tk(e, 0, step, [&] {
+        const int many = e->nslots > 4;
+        if (many) k_evmax<<<e->nslots, 64, 0, s>>>(c, e->vmx);
+        k_ebegin<<<1, WG, 0, s>>>(c, e->d_slots, e->vmx, many);
+    });
     tk(e, 1, step, [&] { k_ereduce<<<e->nblk, WG, 0, s>>>(c, e->tile_count, e->ntiles, e->bsum); });
     tk(e, 2, step, [&] { k_escan<<<1, WG, 0, s>>>(c, e->bsum, e->nblk, e->boff); });
     tk(e, 3, step, [&] {

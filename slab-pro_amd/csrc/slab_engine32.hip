@@ -939,7 +939,8 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
 // slots are carried unchanged.  On a re-sort step writes the particles to the
 // other buffer in canonical order, otherwise updates them in place (the
 // constant mass and creation index are not rewritten).  Also writes the next
-// keys and tile slots (indexed like the storage) and the per-slot max |v|^2.
+// keys and tile counts (keys indexed like the storage) and the per-slot max
+// |v|^2.
 __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                              const int* __restrict__ idx, const float4* __restrict__ vf,
                                              const float* __restrict__ tz, const float* __restrict__ tgx,

@@ -142,11 +142,9 @@ __global__ void k_ctl_end(Ctl* c) {
 __global__ void k_prime_pre(Ctl* c) { c->stop = RUNNING; }
 
 // simulate() priming with dt = 0 (src/simulation.cpp:493-495, simulation.hpp:39).
-__global__ void __launch_bounds__(WG) k_ctl_prime(Ctl* c, unsigned long long* __restrict__ part) {
+__global__ void __launch_bounds__(WG) k_ctl_prime(Ctl* c, const unsigned long long* __restrict__ part) {
     __shared__ unsigned long long s_out[5];
-    reduce_partials(part, s_out);
-    __syncthreads();
-    reset_partials(part);
+    reduce_partials(part, s_out); // kept: step 1's ctl_begin reads them again
     if (threadIdx.x != 0) return;
     c->dt = 0.0;
     c->epoch += 1;

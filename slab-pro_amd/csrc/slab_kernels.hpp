@@ -21,7 +21,7 @@ __global__ void k_ctl_clear(Ctl* c);
 __global__ void k_ctl_begin(Ctl* c, unsigned long long* part);
 __global__ void k_ctl_end(Ctl* c);
 __global__ void k_prime_pre(Ctl* c);
-__global__ void k_ctl_prime(Ctl* c, unsigned long long* part);
+__global__ void k_ctl_prime(Ctl* c, const unsigned long long* part);
 __global__ void k_part_init(unsigned long long* part);
 __global__ void k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* key, int* tslot, int* tile_count, unsigned long long* part);
 __global__ void k_init_z(Ctl* c, PBuf64 A, const double* tz);

@@ -181,3 +181,37 @@ def test_c0_full_run_max_maps():
     p = run.particles()
     o = orc.particles()
     assert _rel(p["x"], o["x"]) <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dt_maxima_follow_the_particles(precision):
+    """The per-step max |v_p|^2 partials (integer atomics into shards, consumed
+    and reset by the next step's dt) track the particles: diag's max |v_p| is
+    the downloaded particles' after priming and after each step, and step 1's
+    dt is the CFL dt of the velocities set before priming
+    (src/simulation.cpp:565-582)."""
+    from slabpro import abi
+    import slabpro
+    cfg = _cfgs().tiny()
+    run = slabpro.make_run(cfg, precision=abi.FP64 if precision == "fp64" else abi.FP32)
+    n = len(cfg["x"])
+    g = np.random.default_rng(3)
+    vx, vy = g.uniform(-150.0, 150.0, n), g.uniform(-150.0, 150.0, n)  # CFL-limited dt
+    run.set_velocity(vx, vy)
+    run.prime()
+    tol = 1e-14 if precision == "fp64" else 1e-6
+    v0 = math.sqrt(float(np.max(vx * vx + vy * vy)))
+    if precision == "fp64":
+        assert run.diag()["vmax"] == pytest.approx(v0, rel=tol)
+    info = run.step(1)
+    prm = cfg["params"]
+    dt_cfl = prm["C_cfl"] * cfg["dx"] / v0
+    assert dt_cfl < prm["dt_max"]
+    assert info["dt"] == pytest.approx(dt_cfl, rel=tol)
+    for _ in range(6):
+        info = run.step(1)
+        p = run.particles()
+        vm = math.sqrt(float(np.max(p["vx"] ** 2 + p["vy"] ** 2)))
+        if precision == "fp64":
+            assert run.diag()["vmax"] == pytest.approx(vm, rel=tol)
+    run.close()

@@ -310,7 +310,8 @@ __global__ void __launch_bounds__(WG) k_ekeys(ECtl* c, const Slot* __restrict__ 
     int t = 0, k = -1;
     if (in) {
         const int raw = P.slot[s], sl = raw & SLOT_MASK;
-        k = ekey(c, slots[sl].status, sl, (raw & SLOT_OUT) != 0, P.cell[s], P.ox[s], P.oy[s], &t);
+        const float2 o = P.o[s];
+        k = ekey(c, slots[sl].status, sl, (raw & SLOT_OUT) != 0, P.cell[s], o.x, o.y, &t);
     }
     wave_count_any(t, in && k >= 0, tile_count);
     if (in) key[s] = k;
@@ -323,7 +324,8 @@ __global__ void k_ez(ECtl* c, PB32 A, PB32 B, int s0, int s1, const float* __res
     if (s >= s1) return;
     const int cell = P.cell[s];
     const int ci = cell >> 16, cj = cell & 0xffff;
-    const float ox = P.ox[s], oy = P.oy[s];
+    const float2 o = P.o[s];
+    const float ox = o.x, oy = o.y;
     float z = 0.0f;
     for (int i = ci - 1; i < ci + 3; ++i) {
         for (int j = cj - 1; j < cj + 3; ++j) {
@@ -774,16 +776,18 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
         if (r0 < r1) {
             const int sn = idx[r0];
-            nox = P.ox[sn]; noy = P.oy[sn]; nmp = P.m[sn]; nhp = P.h[sn];
-            nvx = P.vx[sn]; nvy = P.vy[sn];
+            const float2 po = P.o[sn], pv = P.v[sn];
+            nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
+            nvx = pv.x; nvy = pv.y;
             if (need_vz) nvz = P.vz[sn];
         }
         for (int r = r0; r < r1; ++r) {
             const float ox = nox, oy = noy, mp = nmp, hp = nhp, pvx = nvx, pvy = nvy, pvz = nvz;
             if (r + 1 < r1) {
                 const int sn = idx[r + 1];
-                nox = P.ox[sn]; noy = P.oy[sn]; nmp = P.m[sn]; nhp = P.h[sn];
-                nvx = P.vx[sn]; nvy = P.vy[sn];
+                const float2 po = P.o[sn], pv = P.v[sn];
+                nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
+                nvx = pv.x; nvy = pv.y;
                 if (need_vz) nvz = P.vz[sn];
             }
             // live particles sit in cells [1, N-3] (outflow parks the others),
@@ -973,7 +977,8 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
         const Slot& S = slots[sl];
         const int st = S.status;
         int cell = P.cell[s];
-        float ox = P.ox[s], oy = P.oy[s], vxp = P.vx[s], vyp = P.vy[s], hp = P.h[s];
+        const float2 po = P.o[s], pv = P.v[s];
+        float ox = po.x, oy = po.y, vxp = pv.x, vyp = pv.y, hp = P.h[s];
         float vzp, zp; // recomputed by a running particle, carried otherwise
         if (st == SLOT_RUNNING && !c->compact && !out) {
             const float inv = (float)c->inv_dx, dt = (float)S.dt, fr = (float)S.flip_ratio;
@@ -1035,10 +1040,8 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
             zp = P.z[s];
         }
         Q.cell[o] = cell;
-        Q.ox[o] = ox;
-        Q.oy[o] = oy;
-        Q.vx[o] = vxp;
-        Q.vy[o] = vyp;
+        Q.o[o] = make_float2(ox, oy);
+        Q.v[o] = make_float2(vxp, vyp);
         Q.vz[o] = vzp;
         Q.h[o] = hp;
         Q.z[o] = zp;
@@ -1071,10 +1074,11 @@ __global__ void k_ecollect(ECtl* c, PB32 A, PB32 B, int slot, int* __restrict__ 
     o_cell[q] = P.cell[s];
     o_pid[q] = P.pid[s];
     float* f = o_f + (size_t)q * 8;
-    f[0] = P.ox[s];
-    f[1] = P.oy[s];
-    f[2] = P.vx[s];
-    f[3] = P.vy[s];
+    const float2 po = P.o[s], pv = P.v[s];
+    f[0] = po.x;
+    f[1] = po.y;
+    f[2] = pv.x;
+    f[3] = pv.y;
     f[4] = P.vz[s];
     f[5] = P.z[s];
     f[6] = P.h[s];
@@ -1088,8 +1092,9 @@ __global__ void k_ebbox(ECtl* c, PB32 A, PB32 B, int slot, int n, unsigned long 
     const int s = blockIdx.x * WG + threadIdx.x;
     if (s >= n || (P.slot[s] & SLOT_MASK) != slot) return;
     const int cell = P.cell[s];
-    const double x = ((double)(cell >> 16) + (double)P.ox[s]) * c->dx;
-    const double y = ((double)(cell & 0xffff) + (double)P.oy[s]) * c->dx;
+    const float2 po = P.o[s];
+    const double x = ((double)(cell >> 16) + (double)po.x) * c->dx;
+    const double y = ((double)(cell & 0xffff) + (double)po.y) * c->dx;
     atomicMin(&bb[0], ord_bits(x));
     atomicMax(&bb[1], ord_bits(x));
     atomicMin(&bb[2], ord_bits(y));
@@ -1465,7 +1470,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
              ealloc(e->allocs, &e->ovz, sn, &why);
     for (PB32* b : {&e->A, &e->B}) {
         ok = ok && ealloc(e->pallocs, &b->cell, max_particles, &why);
-        for (float** q : {&b->ox, &b->oy, &b->vx, &b->vy, &b->vz, &b->h, &b->m, &b->z})
+        ok = ok && ealloc(e->pallocs, &b->o, max_particles, &why) && ealloc(e->pallocs, &b->v, max_particles, &why);
+        for (float** q : {&b->vz, &b->h, &b->m, &b->z})
             ok = ok && ealloc(e->pallocs, q, max_particles, &why);
         ok = ok && ealloc(e->pallocs, &b->pid, max_particles, &why) && ealloc(e->pallocs, &b->slot, max_particles, &why);
     }
@@ -1565,7 +1571,8 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
     }
     const size_t n = (size_t)np;
     std::vector<int32_t> cell(n), pid(n), sl(n, slot);
-    std::vector<float> ox(n), oy(n), fvx(n), fvy(n), fvz(n), fh(n), fm(n), zz(n, 0.f);
+    std::vector<float2> fo(n), fv(n);
+    std::vector<float> fvz(n), fh(n), fm(n), zz(n, 0.f);
     for (size_t p = 0; p < n; ++p) {
         const double fx = x[p] * e->inv, fy = y[p] * e->inv;
         double ci = std::floor(fx), cj = std::floor(fy);
@@ -1587,10 +1594,8 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
         if (!(b >= 0.f)) b = 0.f;
         cell[p] = (ii << 16) | jj;
         if (ii < 1 || ii > e->nx - 3 || jj < 1 || jj > e->ny - 3) sl[p] = slot | SLOT_OUT; // outside: parked
-        ox[p] = a;
-        oy[p] = b;
-        fvx[p] = vx ? (float)vx[p] : 0.f;
-        fvy[p] = vy ? (float)vy[p] : 0.f;
+        fo[p] = make_float2(a, b);
+        fv[p] = make_float2(vx ? (float)vx[p] : 0.f, vy ? (float)vy[p] : 0.f);
         fvz[p] = vz ? (float)vz[p] : 0.f;
         fh[p] = (float)h[p];
         fm[p] = (float)(m ? m[p] : prm.rho * h[p] * A[p]); // simulation.cpp:452
@@ -1599,14 +1604,12 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
     const PB32& P = e->h_c->parity ? e->B : e->A;
     hipStream_t s = e->stream;
     bool ok = true;
-    auto up = [&](void* dst, const void* src) {
-        ok = ok && hipMemcpyAsync((char*)dst + base * 4, src, n * 4, hipMemcpyHostToDevice, s) == hipSuccess;
+    auto up = [&](void* dst, const void* src, size_t esz = 4) {
+        ok = ok && hipMemcpyAsync((char*)dst + base * esz, src, n * esz, hipMemcpyHostToDevice, s) == hipSuccess;
     };
     up(P.cell, cell.data());
-    up(P.ox, ox.data());
-    up(P.oy, oy.data());
-    up(P.vx, fvx.data());
-    up(P.vy, fvy.data());
+    up(P.o, fo.data(), sizeof(float2));
+    up(P.v, fv.data(), sizeof(float2));
     up(P.vz, fvz.data());
     up(P.h, fh.data());
     up(P.m, fm.data());
@@ -1854,14 +1857,10 @@ bool engine_set_velocity(Engine32* e, int slot, const double* vx, const double* 
         return false;
     }
     const size_t n = (size_t)S.np;
-    std::vector<float> a(n), b(n);
-    for (size_t k = 0; k < n; ++k) {
-        a[k] = (float)vx[k];
-        b[k] = (float)vy[k];
-    }
+    std::vector<float2> v(n);
+    for (size_t k = 0; k < n; ++k) v[k] = make_float2((float)vx[k], (float)vy[k]);
     const PB32& P = e->h_c->parity ? e->B : e->A;
-    return hip_ok(hipMemcpy(P.vx + base, a.data(), n * 4, hipMemcpyHostToDevice), "velocity", err) &&
-           hip_ok(hipMemcpy(P.vy + base, b.data(), n * 4, hipMemcpyHostToDevice), "velocity", err);
+    return hip_ok(hipMemcpy(P.v + base, v.data(), n * sizeof(float2), hipMemcpyHostToDevice), "velocity", err);
 }
 
 bool engine_set_time(Engine32* e, int slot, double time, int frame, std::string* err) {

@@ -32,7 +32,9 @@ struct Slot {
 // (ci << 16) | cj plus an in-cell offset (ox, oy) in [0,1) cell units.
 struct PB32 {
     int32_t* cell;
-    float *ox, *oy, *vx, *vy, *vz, *h, *m, *z;
+    float2* o;            // offsets in the floor cell (cell units), x and y in one 8-byte word
+    float2* v;            // vx, vy
+    float *vz, *h, *m, *z;
     int32_t *pid, *slot;
 };
 

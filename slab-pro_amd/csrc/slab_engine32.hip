@@ -33,13 +33,20 @@ __device__ __forceinline__ float dNf(float u) {
     return 0.0f;
 }
 // Weights / derivatives of the 3 stencil nodes of one axis for a position f
-// relative to the base node (u = f - d), src/tools.hpp:119-140.
+// relative to the base node (u = f - d), src/tools.hpp:119-140.  An unclamped
+// base puts f in [0.5, 1.5], where u = f, f - 1, f - 2 fall in fixed pieces of
+// N: the pieces are evaluated without branches, with the same operations as
+// Nf / dNf (so the same results).
 __device__ __forceinline__ void weights3f(float f, float inv, float w[3], float dw[3]) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        w[d] = Nf(f - (float)d);
-        dw[d] = dNf(f - (float)d) * inv;
-    }
+    const float t0 = 1.5f - f;        // 1.5 - |u0|
+    const float u1 = f - 1.0f;        // |u1| <= 0.5
+    const float t2 = 1.5f + (f - 2.0f); // 1.5 - |u2|
+    w[0] = 0.5f * t0 * t0;
+    w[1] = 0.75f - u1 * u1;
+    w[2] = 0.5f * t2 * t2;
+    dw[0] = (f - 1.5f) * inv;
+    dw[1] = (-2.0f * u1) * inv;
+    dw[2] = t2 * inv;
 }
 
 __device__ __forceinline__ PB32 ecur(const ECtl* c, const PB32& A, const PB32& B) { return c->parity ? B : A; }

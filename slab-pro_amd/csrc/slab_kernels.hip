@@ -23,19 +23,20 @@ namespace slab {
 
 // ---------------------------------------------------------------- helpers
 
-// Quadratic B-spline, src/tools.hpp:119-128.
+// Quadratic B-spline, src/tools.hpp:119-128.  Both pieces are evaluated with
+// the reference's operations and the piece is selected (no branch).
 __device__ __forceinline__ double Nq(double u) {
     const double a = fabs(u);
-    if (a < 0.5) return 0.75 - a * a;
-    if (a < 1.5) return 0.5 * (1.5 - a) * (1.5 - a);
-    return 0.0;
+    const double inner = 0.75 - a * a;
+    const double outer = 0.5 * (1.5 - a) * (1.5 - a);
+    return a < 0.5 ? inner : (a < 1.5 ? outer : 0.0);
 }
 // Its derivative, src/tools.hpp:131-140 (sgn as src/tools.hpp:90-92).
 __device__ __forceinline__ double dNq(double u) {
     const double a = fabs(u);
-    if (a < 0.5) return -2.0 * u;
-    if (a < 1.5) return u - 1.5 * (double)((0.0 < u) - (u < 0.0));
-    return 0.0;
+    const double inner = -2.0 * u;
+    const double outer = u - 1.5 * (double)((0.0 < u) - (u < 0.0));
+    return a < 0.5 ? inner : (a < 1.5 ? outer : 0.0);
 }
 
 // Stencil base trunc((x - x0)/dx - 0.5) of src/simulation.cpp:741-742 in

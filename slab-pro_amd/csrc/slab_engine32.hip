@@ -804,14 +804,22 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
             const int fl = (lx + ax) * TCELLS + (ly + ay);
             const float fx = (float)ax + ox, fy = (float)ay + oy;
             const float cth = s_ter[2 * TCELLS * TCELLS + fl];
-            const float coeff = -mp * g * cth * cth;
+            const float mgc = mp * g * cth * cth; // m g cos^2 theta
             const float vmx = pvx * mp, vmy = pvy * mp, hm = hp * mp, vzm = pvz * mp;
-            const float mbx = coeff * s_ter[fl], mby = coeff * s_ter[TCELLS * TCELLS + fl];
+            // body force m b = -m g cos^2 theta grad z
+            const float mbx = -mgc * s_ter[fl], mby = -mgc * s_ter[TCELLS * TCELLS + fl];
             // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
-            const float ps = -0.5f * mp * g * cth * cth * hp;
+            const float ps = -0.5f * mgc * hp;
             float wx[3], wy[3], dwx[3], dwy[3];
             weights3f(fx, inv, wx, dwx);
             weights3f(fy, inv, wy, dwy);
+            // stress term ps * grad w = (ps * dw_x[d]) * w_y[e], (ps * dw_y[e]) * w_x[d]
+            float psx[3], psy[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                psx[d] = ps * dwx[d];
+                psy[d] = ps * dwy[d];
+            }
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
 #pragma unroll
@@ -819,13 +827,13 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
                     const float wgt = wx[d] * wy[e];
                     if (wgt > 3e-3f) { // simulation.cpp:780
                         float* a = acc[d * 3 + e];
-                        a[0] += mp * wgt;
-                        a[1] += vmx * wgt;
-                        a[2] += vmy * wgt;
-                        a[3] += vzm * wgt;
-                        a[4] += hm * wgt;
-                        a[5] += mbx * wgt - ps * (dwx[d] * wy[e]);
-                        a[6] += mby * wgt - ps * (dwy[e] * wx[d]);
+                        a[0] = fmaf(mp, wgt, a[0]);
+                        a[1] = fmaf(vmx, wgt, a[1]);
+                        a[2] = fmaf(vmy, wgt, a[2]);
+                        a[3] = fmaf(vzm, wgt, a[3]);
+                        a[4] = fmaf(hm, wgt, a[4]);
+                        a[5] = fmaf(-psx[d], wy[e], fmaf(mbx, wgt, a[5]));
+                        a[6] = fmaf(-psy[e], wx[d], fmaf(mby, wgt, a[6]));
                     }
                 }
             }

@@ -68,8 +68,12 @@ __device__ __forceinline__ bool slot_active(int st) { return st == SLOT_PRIMING 
 // concurrent blocks on the same tile cursors and the atomics contend.)  The
 // grid size is a multiple of 8.
 constexpr int N_XCD = 8;
-__device__ __forceinline__ int xcd_block() {
-    return (int)(blockIdx.x % N_XCD) * (int)(gridDim.x / N_XCD) + (int)(blockIdx.x / N_XCD);
+// nblk = blocks that hold work (the live count, not the capacity the grid is
+// sized for): each XCD takes ceil(nblk / 8) consecutive ones; blocks past
+// that get an index >= nblk (no work).
+__device__ __forceinline__ int xcd_block(int nblk) {
+    const int per = (nblk + N_XCD - 1) / N_XCD, j = (int)(blockIdx.x / N_XCD);
+    return j < per ? (int)(blockIdx.x % N_XCD) * per + j : nblk;
 }
 
 
@@ -975,8 +979,9 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
     // compacts at once), storage was canonical at the last re-sort, and no
     // result depends on the walk order (the sort orders each tile by cell and
     // creation index).
-    const int r = xcd_block() * WG + threadIdx.x;
-    const bool in = r < (reo ? c->n_sorted : c->n_store);
+    const int nwork = reo ? c->n_sorted : c->n_store;
+    const int r = xcd_block((nwork + WG - 1) / WG) * WG + threadIdx.x;
+    const bool in = r < nwork;
     int t = 0, kk = -1, sl = 0, o = 0;
     float v2 = 0.f;
     bool pub = false;

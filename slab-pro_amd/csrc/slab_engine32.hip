@@ -601,13 +601,22 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
             continue;
         }
         const bool small = n <= ES_CAP;
+        // storage index of each arrival this thread stages (small tiles): the
+        // final write places it from registers instead of re-reading tmp
+        int svr[ES_CAP / WG];
         if (small) {
-            for (int r = threadIdx.x; r < n; r += WG) {
-                const int sv = tmp[b + r];
-                const int cc = key[sv] & 255;
-                s_cell[r] = (unsigned char)cc;
-                s_pid[r] = pid[sv];
-                atomicAdd(&s_hist[cc], 1);
+#pragma unroll
+            for (int k = 0; k < ES_CAP / WG; ++k) {
+                const int r = threadIdx.x + k * WG;
+                svr[k] = 0;
+                if (r < n) {
+                    const int sv = tmp[b + r];
+                    svr[k] = sv;
+                    const int cc = key[sv] & 255;
+                    s_cell[r] = (unsigned char)cc;
+                    s_pid[r] = pid[sv];
+                    atomicAdd(&s_hist[cc], 1);
+                }
             }
         } else {
             for (int r = threadIdx.x; r < n; r += WG) atomicAdd(&s_hist[key[tmp[b + r]] & 255], 1);
@@ -653,7 +662,15 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
                 s_ord[k + 1] = rv;
             }
             __syncthreads();
-            for (int q = threadIdx.x; q < n; q += WG) idx[b + q] = tmp[b + s_ord[q]];
+            // final position of each arrival (over the creation indices, no longer read)
+            unsigned short* s_pos = reinterpret_cast<unsigned short*>(s_mem);
+            for (int q = threadIdx.x; q < n; q += WG) s_pos[s_ord[q]] = (unsigned short)q;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < ES_CAP / WG; ++k) {
+                const int r = threadIdx.x + k * WG;
+                if (r < n) idx[b + s_pos[r]] = svr[k];
+            }
         } else if (small) {
             // keys (cell << 44 | creation index << 13 | arrival position), built
             // in registers first because they overwrite the staged arrays
@@ -674,7 +691,26 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
             }
             __syncthreads();
             bitonic_lds(s_key, P);
-            for (int q = threadIdx.x; q < n; q += WG) idx[b + q] = tmp[b + (int)(s_key[q] & 0x1fffull)];
+            // final position of each arrival: keys to registers, then positions
+            // over the keys, then each arrival placed from registers
+#pragma unroll
+            for (int k = 0; k < ES_CAP / WG; ++k) {
+                const int q = threadIdx.x + k * WG;
+                kv[k] = q < n ? s_key[q] : 0ull;
+            }
+            __syncthreads();
+            unsigned short* s_pos = reinterpret_cast<unsigned short*>(s_mem);
+#pragma unroll
+            for (int k = 0; k < ES_CAP / WG; ++k) {
+                const int q = threadIdx.x + k * WG;
+                if (q < n) s_pos[(int)(kv[k] & 0x1fffull)] = (unsigned short)q;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < ES_CAP / WG; ++k) {
+                const int r = threadIdx.x + k * WG;
+                if (r < n) idx[b + s_pos[r]] = svr[k];
+            }
         } else {
             for (int r = threadIdx.x; r < n; r += WG) {
                 const int sv = tmp[b + r];

@@ -1272,6 +1272,11 @@ struct Engine32 {
     float* tpart = nullptr;
     PB32 A{}, B{};
     int *key = nullptr, *idx = nullptr, *tmp = nullptr;
+    // stored particles the particle-range grids (G2P, scatter) are sized for:
+    // the live count rounded up to a power of two, so a lightly filled
+    // capacity launches few idle blocks (a captured step is rebuilt when it
+    // no longer fits)
+    int64_t launch_n = 0;
     double* dpart = nullptr;
     std::vector<void*> allocs, pallocs;
     std::vector<int64_t> admit_base; // storage offset of each slot's particles at admission
@@ -1364,7 +1369,7 @@ void enqueue_estep(Engine32* e, int step) {
                                            e->pstamp, e->gstamp, e->tile_pos);
     });
     tk(e, 4, step, [&] {
-        const unsigned nb = (unsigned)std::max<int64_t>(1, (e->cap + WG * SC_PER - 1) / (WG * SC_PER));
+        const unsigned nb = (unsigned)std::max<int64_t>(1, (e->launch_n + WG * SC_PER - 1) / (WG * SC_PER));
         k_escatter<<<nb, WG, 0, s>>>(c, e->key, e->tile_begin, e->cursor, e->tmp);
     });
     tk(e, 5, step, [&] {
@@ -1380,7 +1385,7 @@ void enqueue_estep(Engine32* e, int step) {
                                   e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz);
     });
     tk(e, 8, step, [&] {
-        k_eg2p<<<xcd_grid((e->cap + WG - 1) / WG), WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->tz, e->tgx, e->tgy, e->key,
+        k_eg2p<<<xcd_grid((e->launch_n + WG - 1) / WG), WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->tz, e->tgx, e->tgy, e->key,
                                            e->tile_count, e->vmx);
     });
     tk(e, 9, step, [&] { k_eend<<<1, WG, 0, s>>>(c, e->d_slots); });
@@ -1391,6 +1396,18 @@ void drop_graph(Engine32* e) {
     if (e->graph) (void)hipGraphDestroy(e->graph);
     e->gexec = nullptr;
     e->graph = nullptr;
+}
+
+// Re-sizes the particle-range grids to the stored count when it outgrew them
+// or fell below a quarter (the captured step is dropped and rebuilt).
+void size_particle_grids(Engine32* e) {
+    const int64_t need = std::max<int64_t>(1, e->h_c->n_store);
+    if (need <= e->launch_n && need * 4 >= e->launch_n) return;
+    int64_t want = 65536;
+    while (want < need) want <<= 1;
+    e->launch_n = std::min<int64_t>(e->cap, want);
+    if (e->launch_n < need) e->launch_n = need;
+    drop_graph(e);
 }
 
 bool build_graph(Engine32* e, std::string* err) {
@@ -1702,6 +1719,7 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
 
 bool engine_step(Engine32* e, int max_steps, std::string* err) {
     hipStream_t s = e->stream;
+    size_particle_grids(e);
     k_eclear<<<1, WG, 0, s>>>(e->d_c, e->d_slots);
     if (e->use_graph && !e->timing && !e->gexec && !build_graph(e, err)) return false;
     const int chunk_max = 16;
@@ -1749,6 +1767,7 @@ bool engine_release(Engine32* e, int slot, std::string* err) {
             return false;
         k_ekeys<<<(unsigned)((n + WG - 1) / WG), WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, 0, n, e->key,
                                                              e->tile_count);
+        size_particle_grids(e); // storage may have grown by admissions since the last step
         e->h_c->compact = 1;
         e->h_c->stop = RUNNING;
         if (!hip_ok(hipMemcpyAsync(e->d_c, e->h_c, sizeof(ECtl), hipMemcpyHostToDevice, s), "release", err))

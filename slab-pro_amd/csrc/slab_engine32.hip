@@ -392,9 +392,23 @@ __device__ __forceinline__ void block_exscan3e(int v[3], int out[3], int tot[3])
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, const int* __restrict__ tile_count, int ntiles,
+// True when the block's tiles all belong to one FREE slot: a released slot's
+// tiles hold no particles (release re-keys and compacts), so the block has
+// nothing to count or list.
+__device__ __forceinline__ bool scan_block_idle(const ECtl* c, const Slot* __restrict__ slots, int ntiles) {
+    const int t0 = blockIdx.x * SCAN_BLOCK, t1 = min(t0 + SCAN_BLOCK, ntiles) - 1;
+    const int s0 = t0 / c->ntd, s1 = t1 / c->ntd;
+    return s0 == s1 && s0 < c->nslots && slots[s0].status == SLOT_FREE;
+}
+
+__global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, const Slot* __restrict__ slots,
+                                                const int* __restrict__ tile_count, int ntiles,
                                                 int* __restrict__ bsum) {
     if (c->stop) return;
+    if (scan_block_idle(c, slots, ntiles)) {
+        if (threadIdx.x < 3) bsum[3 * blockIdx.x + threadIdx.x] = 0;
+        return;
+    }
     int acc[3] = {0, 0, 0};
     const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
     for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -436,12 +450,13 @@ __global__ void __launch_bounds__(WG) k_escan(ECtl* c, const int* __restrict__ b
     }
 }
 
-__global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const int* __restrict__ tile_count, int ntiles,
+__global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict__ slots,
+                                                 const int* __restrict__ tile_count, int ntiles,
                                                  const int* __restrict__ boff, int* __restrict__ tile_begin,
                                                  int* __restrict__ ptiles, int* __restrict__ gtiles,
                                                  int* __restrict__ pstamp, int* __restrict__ gstamp,
                                                  int* __restrict__ tile_pos) {
-    if (c->stop) return;
+    if (c->stop || scan_block_idle(c, slots, ntiles)) return;
     const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
     int f[SCAN_ITEMS][3];
     int acc[3] = {0, 0, 0};
@@ -1362,10 +1377,10 @@ void enqueue_estep(Engine32* e, int step) {
         if (many) k_evmax<<<e->nslots, 64, 0, s>>>(c, e->vmx);
         k_ebegin<<<1, WG, 0, s>>>(c, e->d_slots, e->vmx, many);
     });
-    tk(e, 1, step, [&] { k_ereduce<<<e->nblk, WG, 0, s>>>(c, e->tile_count, e->ntiles, e->bsum); });
+    tk(e, 1, step, [&] { k_ereduce<<<e->nblk, WG, 0, s>>>(c, e->d_slots, e->tile_count, e->ntiles, e->bsum); });
     tk(e, 2, step, [&] { k_escan<<<1, WG, 0, s>>>(c, e->bsum, e->nblk, e->boff); });
     tk(e, 3, step, [&] {
-        k_eoffsets<<<e->nblk, WG, 0, s>>>(c, e->tile_count, e->ntiles, e->boff, e->tile_begin, e->ptiles, e->gtiles,
+        k_eoffsets<<<e->nblk, WG, 0, s>>>(c, e->d_slots, e->tile_count, e->ntiles, e->boff, e->tile_begin, e->ptiles, e->gtiles,
                                            e->pstamp, e->gstamp, e->tile_pos);
     });
     tk(e, 4, step, [&] {

@@ -546,6 +546,7 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
 constexpr int ES_CAP = 4096;
 constexpr int ES_ROUND = 2048;
 constexpr int ES_INSERT = 32;
+constexpr int ES_GROUP = 4; // particles per thread whose global reads are issued together
 constexpr int ES_LDS_BYTES = ES_CAP * 8;
 
 // Ascending bitonic sort of k[0, P), P a power of two, by one workgroup.
@@ -620,21 +621,45 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
         // final write places it from registers instead of re-reading tmp
         int svr[ES_CAP / WG];
         if (small) {
+            // groups of ES_GROUP arrivals per thread: the bucket reads of a group
+            // are issued together, then their key and creation-index reads
+            // (clamped indices keep every read in the tile)
 #pragma unroll
-            for (int k = 0; k < ES_CAP / WG; ++k) {
-                const int r = threadIdx.x + k * WG;
-                svr[k] = 0;
-                if (r < n) {
-                    const int sv = tmp[b + r];
-                    svr[k] = sv;
-                    const int cc = key[sv] & 255;
-                    s_cell[r] = (unsigned char)cc;
-                    s_pid[r] = pid[sv];
-                    atomicAdd(&s_hist[cc], 1);
+            for (int k0 = 0; k0 < ES_CAP / WG; k0 += ES_GROUP) {
+                if (k0 * WG < n) {
+                    int sv[ES_GROUP], kc[ES_GROUP], pd[ES_GROUP];
+#pragma unroll
+                    for (int j = 0; j < ES_GROUP; ++j) sv[j] = tmp[b + min((int)threadIdx.x + (k0 + j) * WG, n - 1)];
+#pragma unroll
+                    for (int j = 0; j < ES_GROUP; ++j) {
+                        kc[j] = key[sv[j]];
+                        pd[j] = pid[sv[j]];
+                    }
+#pragma unroll
+                    for (int j = 0; j < ES_GROUP; ++j) {
+                        const int r = threadIdx.x + (k0 + j) * WG;
+                        svr[k0 + j] = sv[j];
+                        if (r < n) {
+                            const int cc = kc[j] & 255;
+                            s_cell[r] = (unsigned char)cc;
+                            s_pid[r] = pd[j];
+                            atomicAdd(&s_hist[cc], 1);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < ES_GROUP; ++j) svr[k0 + j] = 0;
                 }
             }
         } else {
-            for (int r = threadIdx.x; r < n; r += WG) atomicAdd(&s_hist[key[tmp[b + r]] & 255], 1);
+            for (int r0 = threadIdx.x; r0 < n; r0 += WG * ES_GROUP) {
+                int kc[ES_GROUP];
+#pragma unroll
+                for (int j = 0; j < ES_GROUP; ++j) kc[j] = key[tmp[b + min(r0 + j * WG, n - 1)]];
+#pragma unroll
+                for (int j = 0; j < ES_GROUP; ++j)
+                    if (r0 + j * WG < n) atomicAdd(&s_hist[kc[j] & 255], 1);
+            }
         }
         __syncthreads();
         const int cnt = s_hist[threadIdx.x];
@@ -727,10 +752,15 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
                 if (r < n) idx[b + s_pos[r]] = svr[k];
             }
         } else {
-            for (int r = threadIdx.x; r < n; r += WG) {
-                const int sv = tmp[b + r];
-                const int pos = atomicAdd(&s_hist[key[sv] & 255], 1);
-                idx[b + pos] = sv;
+            for (int r0 = threadIdx.x; r0 < n; r0 += WG * ES_GROUP) {
+                int sv[ES_GROUP], kc[ES_GROUP];
+#pragma unroll
+                for (int j = 0; j < ES_GROUP; ++j) sv[j] = tmp[b + min(r0 + j * WG, n - 1)];
+#pragma unroll
+                for (int j = 0; j < ES_GROUP; ++j) kc[j] = key[sv[j]];
+#pragma unroll
+                for (int j = 0; j < ES_GROUP; ++j)
+                    if (r0 + j * WG < n) idx[b + atomicAdd(&s_hist[kc[j] & 255], 1)] = sv[j];
             }
             __syncthreads();
             for (int c0 = 0; c0 < TILE_CELLS;) {
@@ -745,15 +775,29 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
                 const int lo = s_off[c0], len = ((c1 < TILE_CELLS) ? s_off[c1] : n) - lo;
                 if (len <= ES_ROUND) {
                     const int P = pow2_ceil(len > 0 ? len : 1);
-                    for (int q = threadIdx.x; q < P; q += WG) {
-                        unsigned long long kq = ~0ull;
-                        if (q < len) {
-                            const int sv = idx[b + lo + q];
-                            s_sv[q] = sv;
-                            kq = ((unsigned long long)(key[sv] & 255) << 44) |
-                                 ((unsigned long long)(uint32_t)pid[sv] << 13) | (unsigned long long)q;
+                    for (int q0 = threadIdx.x; q0 < P; q0 += WG * ES_GROUP) {
+                        int sv[ES_GROUP], kc[ES_GROUP], pd[ES_GROUP];
+                        if (len > 0) { // reads grouped as in the staging above
+#pragma unroll
+                            for (int j = 0; j < ES_GROUP; ++j) sv[j] = idx[b + lo + min(q0 + j * WG, len - 1)];
+#pragma unroll
+                            for (int j = 0; j < ES_GROUP; ++j) {
+                                kc[j] = key[sv[j]];
+                                pd[j] = pid[sv[j]];
+                            }
                         }
-                        s_key[q] = kq;
+#pragma unroll
+                        for (int j = 0; j < ES_GROUP; ++j) {
+                            const int q = q0 + j * WG;
+                            if (q >= P) continue;
+                            unsigned long long kq = ~0ull;
+                            if (q < len) {
+                                s_sv[q] = sv[j];
+                                kq = ((unsigned long long)(kc[j] & 255) << 44) |
+                                     ((unsigned long long)(uint32_t)pd[j] << 13) | (unsigned long long)q;
+                            }
+                            s_key[q] = kq;
+                        }
                     }
                     __syncthreads();
                     bitonic_lds(s_key, P);

@@ -996,11 +996,27 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
             src[ns] = t - nty - 1;
             rn[ns++] = (a + TILE) * REGION + (b + TILE);
         }
-        for (int k = 0; k < ns; ++k) {
-            if (pstamp[src[k]] != ep) continue;
-            const float* q = tpart + (size_t)tile_pos[src[k]] * (P2G_FIELDS * REGION_NODES) + rn[k];
+        // stamps, then list positions, then partial values of all sources are
+        // read before any is summed (sums in source order)
+        bool ok[4];
+        int pos[4];
+        float val[4][P2G_FIELDS];
 #pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) acc[f] += q[f * REGION_NODES];
+        for (int k = 0; k < 4; ++k) ok[k] = k < ns && pstamp[src[k]] == ep;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) pos[k] = ok[k] ? tile_pos[src[k]] : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!ok[k]) continue;
+            const float* q = tpart + (size_t)pos[k] * (P2G_FIELDS * REGION_NODES) + rn[k];
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) val[k][f] = q[f * REGION_NODES];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!ok[k]) continue;
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) acc[f] += val[k][f];
         }
         const float g = (float)S.gravity, dt = (float)S.dt;
         const float am = acc[0];

@@ -337,6 +337,14 @@ __global__ void k_scatter(Ctl* c, const int* __restrict__ key, const int* __rest
 // by creation index, giving the canonical order (tile, cell, creation index).
 // Writes idx (sorted position -> storage slot) and cell_begin, and clears the
 // tile's count for the next step's g2p.
+// Tiles of at most TS_CAP particles are staged in LDS (cell, creation index,
+// storage index), counted and placed by cell there, and each cell's run is
+// insertion-sorted by creation index in LDS; larger tiles do the same steps
+// in global memory.  The dependent global reads (bucket, then key and creation
+// index) are issued in groups of TS_GROUP per thread.
+constexpr int TS_CAP = 2048;
+constexpr int TS_GROUP = 4;
+
 __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, const int32_t* __restrict__ pidA,
                                                   const int32_t* __restrict__ pidB, const int* __restrict__ key,
                                                   const int* __restrict__ ptiles, int* __restrict__ tile_count,
@@ -346,6 +354,10 @@ __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, const int32_t* __restr
     __shared__ int s_hist[TILE_CELLS];
     __shared__ int s_off[TILE_CELLS];
     __shared__ int s_n;
+    __shared__ int s_pid[TS_CAP];
+    __shared__ int s_sv[TS_CAP];
+    __shared__ unsigned short s_ord[TS_CAP];
+    __shared__ unsigned char s_cell[TS_CAP];
     const int32_t* __restrict__ pid = c->parity ? pidB : pidA;
     const int nt = c->n_ptiles;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
@@ -355,7 +367,28 @@ __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, const int32_t* __restr
         __syncthreads();
         const int n = s_n;
         const int b = tile_begin[t];
-        for (int r = threadIdx.x; r < n; r += WG) atomicAdd(&s_hist[key[tmp[b + r]] & 255], 1);
+        const bool small = n <= TS_CAP;
+        for (int r0 = threadIdx.x; r0 < n; r0 += WG * TS_GROUP) {
+            int sv[TS_GROUP], kc[TS_GROUP], pd[TS_GROUP];
+#pragma unroll
+            for (int j = 0; j < TS_GROUP; ++j) sv[j] = tmp[b + min(r0 + j * WG, n - 1)];
+#pragma unroll
+            for (int j = 0; j < TS_GROUP; ++j) {
+                kc[j] = key[sv[j]] & 255;
+                pd[j] = small ? pid[sv[j]] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < TS_GROUP; ++j) {
+                const int r = r0 + j * WG;
+                if (r >= n) continue;
+                atomicAdd(&s_hist[kc[j]], 1);
+                if (small) {
+                    s_cell[r] = (unsigned char)kc[j];
+                    s_pid[r] = pd[j];
+                    s_sv[r] = sv[j];
+                }
+            }
+        }
         __syncthreads();
         // exclusive scan of the 256 cell counts
         {
@@ -382,14 +415,36 @@ __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, const int32_t* __restr
         // s_hist becomes the per-cell cursor
         s_hist[threadIdx.x] = s_off[threadIdx.x];
         __syncthreads();
-        for (int r = threadIdx.x; r < n; r += WG) {
-            const int s = tmp[b + r];
-            const int pos = atomicAdd(&s_hist[key[s] & 255], 1);
-            idx[b + pos] = s;
-        }
-        __syncthreads();
-        // insertion sort of each cell's run by creation index
-        {
+        if (small) {
+            for (int r = threadIdx.x; r < n; r += WG) s_ord[atomicAdd(&s_hist[s_cell[r]], 1)] = (unsigned short)r;
+            __syncthreads();
+            // insertion sort of each cell's run by creation index
+            const int lo = s_off[threadIdx.x], hi = s_hist[threadIdx.x];
+            for (int q = lo + 1; q < hi; ++q) {
+                const unsigned short rv = s_ord[q];
+                const int pv = s_pid[rv];
+                int k = q - 1;
+                while (k >= lo && s_pid[s_ord[k]] > pv) {
+                    s_ord[k + 1] = s_ord[k];
+                    --k;
+                }
+                s_ord[k + 1] = rv;
+            }
+            __syncthreads();
+            for (int q = threadIdx.x; q < n; q += WG) idx[b + q] = s_sv[s_ord[q]];
+        } else {
+            for (int r0 = threadIdx.x; r0 < n; r0 += WG * TS_GROUP) {
+                int sv[TS_GROUP], kc[TS_GROUP];
+#pragma unroll
+                for (int j = 0; j < TS_GROUP; ++j) sv[j] = tmp[b + min(r0 + j * WG, n - 1)];
+#pragma unroll
+                for (int j = 0; j < TS_GROUP; ++j) kc[j] = key[sv[j]] & 255;
+#pragma unroll
+                for (int j = 0; j < TS_GROUP; ++j)
+                    if (r0 + j * WG < n) idx[b + atomicAdd(&s_hist[kc[j]], 1)] = sv[j];
+            }
+            __syncthreads();
+            // insertion sort of each cell's run by creation index
             const int lo = b + s_off[threadIdx.x];
             const int hi = b + s_hist[threadIdx.x];
             for (int q = lo + 1; q < hi; ++q) {

@@ -531,8 +531,29 @@ __global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, Pay64 Y, const int* __rest
             for (int k = 0; k < 9; ++k)
 #pragma unroll
                 for (int f = 0; f < 4; ++f) acc[k][f] = 0.0;
+            // the next particle's payload is read before the current one is summed
+            double nx_ = 0.0, ny_ = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0, n4 = 0.0;
+            if (r0 < r1) {
+                nx_ = Y.x[r0];
+                ny_ = Y.y[r0];
+                if (grp == 0) {
+                    n0 = Y.m[r0]; n1 = Y.vmx[r0]; n2 = Y.vmy[r0]; n3 = Y.vzm[r0];
+                } else {
+                    n0 = Y.hm[r0]; n1 = Y.hA[r0]; n2 = Y.mbx[r0]; n3 = Y.mby[r0]; n4 = Y.s00[r0];
+                }
+            }
             for (int r = r0; r < r1; ++r) {
-                const double xp = Y.x[r], yp = Y.y[r];
+                const double xp = nx_, yp = ny_, f0 = n0, f1 = n1, f2 = n2, f3 = n3, f4 = n4;
+                if (r + 1 < r1) {
+                    nx_ = Y.x[r + 1];
+                    ny_ = Y.y[r + 1];
+                    if (grp == 0) {
+                        n0 = Y.m[r + 1]; n1 = Y.vmx[r + 1]; n2 = Y.vmy[r + 1]; n3 = Y.vzm[r + 1];
+                    } else {
+                        n0 = Y.hm[r + 1]; n1 = Y.hA[r + 1]; n2 = Y.mbx[r + 1]; n3 = Y.mby[r + 1];
+                        n4 = Y.s00[r + 1];
+                    }
+                }
                 double ux[3], uy[3], wx[3], wy[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -542,7 +563,7 @@ __global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, Pay64 Y, const int* __rest
                     wy[d] = Nq(uy[d]);
                 }
                 if (grp == 0) {
-                    const double m = Y.m[r], vmx = Y.vmx[r], vmy = Y.vmy[r], vzm = Y.vzm[r];
+                    const double m = f0, vmx = f1, vmy = f2, vzm = f3;
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
 #pragma unroll
@@ -557,7 +578,7 @@ __global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, Pay64 Y, const int* __rest
                         }
                     }
                 } else {
-                    const double hm = Y.hm[r], hA = Y.hA[r], mbx = Y.mbx[r], mby = Y.mby[r], s00 = Y.s00[r];
+                    const double hm = f0, hA = f1, mbx = f2, mby = f3, s00 = f4;
                     double dwx[3], dwy[3];
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {

@@ -37,9 +37,8 @@ slab_status fail(slab_status s, const std::string& msg) {
         if (s_ != SLAB_OK) return s_;      \
     } while (0)
 
-// FP64 form: "p2g" is the per-particle payload pass, "grid" the node gather +
-// grid update.  FP32 form: "p2g" is the per-tile LDS reduction, "grid" merges
-// the tile partials and updates the grid.
+// Both forms: "p2g" is the per-tile register/LDS reduction into tile partials,
+// "grid" merges the tile partials and updates the grid.
 const char* const kKernelNames[] = {"ctl_begin", "tile_reduce", "tile_scan", "tile_offsets", "scatter",
                                     "tile_sort", "p2g",         "grid",      "g2p",          "ctl_end"};
 constexpr int kNumKernels = 10;
@@ -73,7 +72,6 @@ struct slab_run {
     // particles
     int64_t np = 0;
     PBuf64 A{}, B{};
-    Pay64 Y{};
     int *key = nullptr, *tslot = nullptr, *idx = nullptr, *tmp = nullptr;
     unsigned long long* part = nullptr; // per-block |v|^2 max + bounding box partials
     double *diag_part = nullptr, *diag_out = nullptr;
@@ -186,13 +184,13 @@ void enqueue_sort(slab_run* r, int step) {
     });
 }
 
-// P2G: per-particle payload in sorted order, then the per-tile register /
-// LDS reduction into tile partials; grid: partial merge + update per node.
+// P2G: the per-tile register / LDS reduction into tile partials; grid:
+// partial merge + update per node.
 void enqueue_p2g(slab_run* r, int step) {
     hipStream_t s = r->stream;
     timed(r, 6, step, [&] {
-        k_payload<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->tgx, r->tgy, r->tcth, r->Y);
-        k_p2g64<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->Y, r->ptiles, r->cell_begin, r->tpart);
+        k_p2g64<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->tgx, r->tgy, r->tcth, r->ptiles,
+                                          r->cell_begin, r->tpart);
     });
     timed(r, 7, step, [&] {
         k_grid64<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->gtiles, r->pstamp, r->tile_pos, r->tpart, r->tgx, r->tgy,
@@ -503,9 +501,6 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
         for (double** q : {&b->x, &b->y, &b->vx, &b->vy, &b->vz, &b->h, &b->A, &b->m, &b->z}) P(q);
         P(&b->pid);
     }
-    for (double** q : {&r->Y.x, &r->Y.y, &r->Y.m, &r->Y.vmx, &r->Y.vmy, &r->Y.vzm, &r->Y.hm, &r->Y.hA, &r->Y.mbx,
-                       &r->Y.mby, &r->Y.s00})
-        P(q);
     P(&r->key);
     P(&r->tslot);
     P(&r->idx);

@@ -464,52 +464,29 @@ __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, const int32_t* __restr
 
 // ---------------------------------------------------------------- P2G + grid
 
-// Per-particle P2G terms of src/simulation.cpp:732-753 (the quantities that do
-// not depend on the node), computed once per particle in sorted order so the
-// node gather reads them contiguously by cell.
-__global__ void __launch_bounds__(WG) k_payload(Ctl* c, PBuf64 A, PBuf64 B, const int* __restrict__ idx,
-                                                const double* __restrict__ tgx, const double* __restrict__ tgy,
-                                                const double* __restrict__ tcth, Pay64 Y) {
-    if (c->stop) return;
-    const int r = blockIdx.x * WG + threadIdx.x;
-    if (r >= c->np) return;
-    const PBuf64 P = cur_buf(c, A, B);
-    const int s = idx[r];
-    const double inv = c->inv_dx, g = c->gravity, rho = c->rho;
-    const double xp = P.x[s], yp = P.y[s], hp = P.h[s], Ap = P.A[s], mp = P.m[s];
-    const size_t fc = (size_t)floor_index(xp, inv, c->nx - 1) * c->ny + floor_index(yp, inv, c->ny - 1);
-    const double cth = tcth[fc];
-    const double coeff = -mp * g * cth * cth;
-    Y.x[r] = xp;
-    Y.y[r] = yp;
-    Y.m[r] = mp;
-    Y.vmx[r] = P.vx[s] * mp;
-    Y.vmy[r] = P.vy[s] * mp;
-    Y.vzm[r] = P.vz[s] * mp;
-    Y.hm[r] = hp * mp;
-    Y.hA[r] = hp * Ap;
-    Y.mbx[r] = coeff * tgx[fc];
-    Y.mby[r] = coeff * tgy[fc];
-    Y.s00[r] = -0.5 * rho * g * cth * cth * hp;
-}
-
 // P2G (src/simulation.cpp:726-798) per particle tile, without atomics: one
-// thread per stencil-base cell sums the payload of its particles (canonical
-// order) into its 9 stencil slots in registers; 9 phases then add slot (d,e)
-// of every cell into node cell+(d,e) of an 18x18-node LDS region (one-to-one
-// within a phase, fixed order) and the region is written as the tile's
+// thread per stencil-base cell sums its particles (canonical order, read
+// through idx) into its 9 stencil slots in registers; 9 phases then add slot
+// (d,e) of every cell into node cell+(d,e) of an 18x18-node LDS region (one-to-
+// one within a phase, fixed order) and the region is written as the tile's
 // partial.  The 7 FP64 fields run in two groups (mass + momenta, then height +
-// forces) to bound the registers.  k_grid64 merges the partials per node.
+// forces) to bound the registers; the per-particle terms of
+// src/simulation.cpp:732-753 are formed in the group that uses them, with the
+// same operations (so the same values) as a separate per-particle pass would.
+// k_grid64 merges the partials per node.
 constexpr int P2G64_FIELDS = 7;
 constexpr int REGION64 = TILE + 2;
 constexpr int REGION64_NODES = REGION64 * REGION64;
 
-__global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, Pay64 Y, const int* __restrict__ ptiles,
+__global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, PBuf64 A, PBuf64 B, const int* __restrict__ idx,
+                                              const double* __restrict__ tgx, const double* __restrict__ tgy,
+                                              const double* __restrict__ tcth, const int* __restrict__ ptiles,
                                               const int* __restrict__ cell_begin, double* __restrict__ tpart) {
     if (c->stop) return;
     __shared__ double s_acc[4 * REGION64_NODES];
-    const int nt = c->n_ptiles, nty = c->nty;
-    const double inv = c->inv_dx, dx = c->dx;
+    const PBuf64 P = cur_buf(c, A, B);
+    const int nt = c->n_ptiles, nty = c->nty, nx = c->nx, ny = c->ny;
+    const double inv = c->inv_dx, dx = c->dx, g = c->gravity, rho = c->rho;
     const int lx = threadIdx.x >> 4, ly = threadIdx.x & 15;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         const int t = ptiles[w];
@@ -531,28 +508,14 @@ __global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, Pay64 Y, const int* __rest
             for (int k = 0; k < 9; ++k)
 #pragma unroll
                 for (int f = 0; f < 4; ++f) acc[k][f] = 0.0;
-            // the next particle's payload is read before the current one is summed
-            double nx_ = 0.0, ny_ = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0, n4 = 0.0;
-            if (r0 < r1) {
-                nx_ = Y.x[r0];
-                ny_ = Y.y[r0];
-                if (grp == 0) {
-                    n0 = Y.m[r0]; n1 = Y.vmx[r0]; n2 = Y.vmy[r0]; n3 = Y.vzm[r0];
-                } else {
-                    n0 = Y.hm[r0]; n1 = Y.hA[r0]; n2 = Y.mbx[r0]; n3 = Y.mby[r0]; n4 = Y.s00[r0];
-                }
-            }
             for (int r = r0; r < r1; ++r) {
-                const double xp = nx_, yp = ny_, f0 = n0, f1 = n1, f2 = n2, f3 = n3, f4 = n4;
-                if (r + 1 < r1) {
-                    nx_ = Y.x[r + 1];
-                    ny_ = Y.y[r + 1];
-                    if (grp == 0) {
-                        n0 = Y.m[r + 1]; n1 = Y.vmx[r + 1]; n2 = Y.vmy[r + 1]; n3 = Y.vzm[r + 1];
-                    } else {
-                        n0 = Y.hm[r + 1]; n1 = Y.hA[r + 1]; n2 = Y.mbx[r + 1]; n3 = Y.mby[r + 1];
-                        n4 = Y.s00[r + 1];
-                    }
+                const int sn = idx[r];
+                const double xp = P.x[sn], yp = P.y[sn];
+                double f0, f1, f2, f3 = 0.0;
+                if (grp == 0) {
+                    f0 = P.m[sn]; f1 = P.vx[sn]; f2 = P.vy[sn]; f3 = P.vz[sn];
+                } else {
+                    f0 = P.h[sn]; f1 = P.A[sn]; f2 = P.m[sn];
                 }
                 double ux[3], uy[3], wx[3], wy[3];
 #pragma unroll
@@ -563,7 +526,7 @@ __global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, Pay64 Y, const int* __rest
                     wy[d] = Nq(uy[d]);
                 }
                 if (grp == 0) {
-                    const double m = f0, vmx = f1, vmy = f2, vzm = f3;
+                    const double m = f0, vmx = f1 * f0, vmy = f2 * f0, vzm = f3 * f0;
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
 #pragma unroll
@@ -578,7 +541,13 @@ __global__ void __launch_bounds__(WG) k_p2g64(Ctl* c, Pay64 Y, const int* __rest
                         }
                     }
                 } else {
-                    const double hm = f0, hA = f1, mbx = f2, mby = f3, s00 = f4;
+                    const double hp = f0, Ap = f1, mp = f2;
+                    // terrain of the particle's floor cell (src/complex_terrain.cpp:113-130)
+                    const size_t fc = (size_t)floor_index(xp, inv, nx - 1) * ny + floor_index(yp, inv, ny - 1);
+                    const double cth = tcth[fc];
+                    const double coeff = -mp * g * cth * cth;
+                    const double hm = hp * mp, hA = hp * Ap, mbx = coeff * tgx[fc], mby = coeff * tgy[fc];
+                    const double s00 = -0.5 * rho * g * cth * cth * hp;
                     double dwx[3], dwy[3];
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {

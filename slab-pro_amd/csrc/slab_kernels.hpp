@@ -11,12 +11,6 @@ struct PBuf64 {
     int32_t* pid; // creation index
 };
 
-// Per-particle P2G terms in sorted order (node-independent part of
-// src/simulation.cpp:732-753).
-struct Pay64 {
-    double *x, *y, *m, *vmx, *vmy, *vzm, *hm, *hA, *mbx, *mby, *s00;
-};
-
 __global__ void k_ctl_clear(Ctl* c);
 __global__ void k_ctl_begin(Ctl* c, unsigned long long* part);
 __global__ void k_ctl_end(Ctl* c);
@@ -32,9 +26,8 @@ __global__ void k_tile_offsets(Ctl* c, const int* tile_count, int ntiles, const 
 __global__ void k_scatter(Ctl* c, const int* key, const int* tslot, const int* tile_begin, int* tmp);
 __global__ void k_tile_sort(Ctl* c, const int32_t* pidA, const int32_t* pidB, const int* key, const int* ptiles,
                             int* tile_count, const int* tile_begin, const int* tmp, int* idx, int* cell_begin);
-__global__ void k_payload(Ctl* c, PBuf64 A, PBuf64 B, const int* idx, const double* tgx, const double* tgy,
-                          const double* tcth, Pay64 Y);
-__global__ void k_p2g64(Ctl* c, Pay64 Y, const int* ptiles, const int* cell_begin, double* tpart);
+__global__ void k_p2g64(Ctl* c, PBuf64 A, PBuf64 B, const int* idx, const double* tgx, const double* tgy,
+                        const double* tcth, const int* ptiles, const int* cell_begin, double* tpart);
 __global__ void k_grid64(Ctl* c, const int* gtiles, const int* pstamp, const int* tile_pos, const double* tpart,
                          const double* tgx, const double* tgy, const double* tcth, double* h_max, double* v_max2,
                          double4* vf, double* out_m, double* out_h, double* out_vz);

@@ -31,8 +31,8 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes (compulsory state traffic, each persistent field read once /
 # written once), per particle and per active node (m > 1e-6), for each kernel.
-# FP64 form: "p2g" = per-particle P2G terms (reads x y vx vy vz h A m) and the
-#            per-tile reduction, "grid" = partial merge + grid update (terrain
+# FP64 form: "p2g" = per-tile reduction reading x y vx vy vz h A m (two field
+#            groups), "grid" = partial merge + grid update (terrain
 #            Gx Gy cos, h_max v_max2 read+write, write vx vy flipx flipy).
 ALG_BYTES_FP64 = {
     "p2g": {"particle": 64, "node": 0},

@@ -66,13 +66,14 @@ struct slab_run {
     Slot slot_prm{};
     // tiles
     int *tile_count = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr;
+    int* cursor = nullptr; // per-tile next free slot of the scatter
     int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr, *cell_begin = nullptr;
     int* tile_pos = nullptr; // tile -> position in the particle-tile list
     double* tpart = nullptr; // P2G partial region per listed particle tile
     // particles
     int64_t np = 0;
     PBuf64 A{}, B{};
-    int *key = nullptr, *tslot = nullptr, *idx = nullptr, *tmp = nullptr;
+    int *key = nullptr, *idx = nullptr, *tmp = nullptr;
     unsigned long long* part = nullptr; // per-block |v|^2 max + bounding box partials
     double *diag_part = nullptr, *diag_out = nullptr;
     bool primed = false;
@@ -164,7 +165,7 @@ unsigned tblocks(const slab_run* r) { return (unsigned)std::min(r->ntiles, kPers
 
 void enqueue_keys(slab_run* r) {
     k_part_init<<<1, WG, 0, r->stream>>>(r->part);
-    k_keys<<<pblocks(r), WG, 0, r->stream>>>(r->d_ctl, r->A, r->B, r->key, r->tslot, r->tile_count, r->part);
+    k_keys<<<pblocks(r), WG, 0, r->stream>>>(r->d_ctl, r->A, r->B, r->key, r->tile_count, r->part);
 }
 
 // Tile scan + bucketing + canonical sort + P2G terms: the particle order that
@@ -177,10 +178,13 @@ void enqueue_sort(slab_run* r, int step) {
         k_tile_offsets<<<r->nblk, WG, 0, s>>>(r->d_ctl, r->tile_count, r->ntiles, r->boff, r->tile_begin, r->ptiles,
                                                r->gtiles, r->pstamp, r->gstamp, r->tile_pos);
     });
-    timed(r, 4, step, [&] { k_scatter<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->key, r->tslot, r->tile_begin, r->tmp); });
+    timed(r, 4, step, [&] {
+        const unsigned nb = (unsigned)((r->np + WG * kScatterPerThread - 1) / (WG * kScatterPerThread));
+        k_scatter<<<nb, WG, 0, s>>>(r->d_ctl, r->key, r->tile_begin, r->cursor, r->tmp);
+    });
     timed(r, 5, step, [&] {
         k_tile_sort<<<tblocks(r), WG, 0, s>>>(r->d_ctl, r->A.pid, r->B.pid, r->key, r->ptiles, r->tile_count,
-                                              r->tile_begin, r->tmp, r->idx, r->cell_begin);
+                                              r->cursor, r->tile_begin, r->tmp, r->idx, r->cell_begin);
     });
 }
 
@@ -204,7 +208,7 @@ void enqueue_step(slab_run* r, int step) {
     enqueue_sort(r, step);
     enqueue_p2g(r, step);
     timed(r, 8, step, [&] {
-        k_g2p<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->vf, r->tz, r->tgx, r->tgy, r->key, r->tslot,
+        k_g2p<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->vf, r->tz, r->tgx, r->tgy, r->key,
                                         r->tile_count, r->part);
     });
     timed(r, 9, step, [&] { k_ctl_end<<<1, 1, 0, s>>>(r->d_ctl); });
@@ -409,6 +413,7 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     A(&r->gh, r->nn);
     A(&r->gvz, r->nn);
     A(&r->tile_count, r->ntiles);
+    A(&r->cursor, r->ntiles);
     A(&r->tile_begin, r->ntiles);
     A(&r->pstamp, r->ntiles);
     A(&r->gstamp, r->ntiles);
@@ -441,6 +446,7 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     zero(r->gh, r->nn * 8, 0);
     zero(r->gvz, r->nn * 8, 0);
     zero(r->tile_count, (size_t)r->ntiles * 4, 0);
+    zero(r->cursor, (size_t)r->ntiles * 4, 0);
     zero(r->pstamp, (size_t)r->ntiles * 4, 0xff);
     zero(r->gstamp, (size_t)r->ntiles * 4, 0xff);
     ok = ok && hipStreamSynchronize(s) == hipSuccess;
@@ -502,7 +508,6 @@ slab_status slab_run_upload_particles(slab_run* r, int64_t np, const double* x, 
         P(&b->pid);
     }
     P(&r->key);
-    P(&r->tslot);
     P(&r->idx);
     P(&r->tmp);
     // P2G partials: one region per listed particle tile (at most one tile per particle)
@@ -559,6 +564,7 @@ slab_status slab_run_prime(slab_run* r) {
     }
     hipStream_t s = r->stream;
     HIPCHK(hipMemsetAsync(r->tile_count, 0, (size_t)r->ntiles * 4, s));
+    HIPCHK(hipMemsetAsync(r->cursor, 0, (size_t)r->ntiles * 4, s));
     k_prime_pre<<<1, 1, 0, s>>>(r->d_ctl);
     enqueue_keys(r);
     k_ctl_prime<<<1, WG, 0, s>>>(r->d_ctl, r->part);

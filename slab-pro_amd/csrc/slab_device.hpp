@@ -1,6 +1,6 @@
 // slab_device.hpp -- device helpers shared by the FP64 and FP32 forms:
 // wavefront (64-lane) reductions, per-block partials of max |v|^2 and the
-// particle bounding box, and run-aggregated tile slot allocation.
+// particle bounding box, and run-aggregated tile counting.
 #pragma once
 #include "slab_common.hpp"
 
@@ -114,11 +114,12 @@ __device__ __forceinline__ void reduce_partials(const unsigned long long* __rest
     }
 }
 
-// Slot of each lane's particle inside its tile.  Lanes of a wave hold
-// consecutive sorted particles, so equal tiles form runs: one atomicAdd per run
-// (by its first lane) instead of one per particle.  Valid lanes must be a
+// Counts each lane's particle into its tile.  Lanes of a wave hold
+// consecutive particles, so equal tiles form runs: one atomicAdd per run (by
+// its first lane) instead of one per particle, and no value returned (the slot
+// inside the tile is taken later, by the scatter).  Valid lanes must be a
 // prefix of the wave; every lane of the wave must call this.
-__device__ __forceinline__ int wave_tile_slot(int t, bool valid, int* __restrict__ tile_count) {
+__device__ __forceinline__ void wave_tile_count(int t, bool valid, int* __restrict__ tile_count) {
     const int lane = threadIdx.x & 63;
     const int tprev = __shfl_up(t, 1, 64);
     const bool head = valid && (lane == 0 || tprev != t);
@@ -126,14 +127,9 @@ __device__ __forceinline__ int wave_tile_slot(int t, bool valid, int* __restrict
     const unsigned long long hmask = __ballot(head);
     const int nvalid = __popcll(vmask);
     const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const unsigned long long hle = hmask & upto;
     const unsigned long long hgt = hmask & ~upto;
-    const int start = hle ? 63 - __clzll((long long)hle) : 0;
     const int end = hgt ? (__ffsll((long long)hgt) - 1) : nvalid;
-    int base = 0;
-    if (head) base = atomicAdd(&tile_count[t], end - start);
-    base = __shfl(base, start, 64);
-    return base + (lane - start);
+    if (head) atomicAdd(&tile_count[t], end - lane);
 }
 
 } // namespace slab

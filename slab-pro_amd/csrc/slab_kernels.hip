@@ -159,8 +159,7 @@ __global__ void __launch_bounds__(WG) k_ctl_prime(Ctl* c, const unsigned long lo
 // (used once after upload; afterwards g2p produces them).  Partial shards
 // start at the identity (k_part_init).
 __global__ void __launch_bounds__(WG) k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* __restrict__ key,
-                                             int* __restrict__ tslot, int* __restrict__ tile_count,
-                                             unsigned long long* __restrict__ part) {
+                                             int* __restrict__ tile_count, unsigned long long* __restrict__ part) {
     const PBuf64 P = cur_buf(c, A, B);
     const int s = blockIdx.x * WG + threadIdx.x;
     const bool valid = s < c->np;
@@ -175,11 +174,8 @@ __global__ void __launch_bounds__(WG) k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* __
         bx0 = bx1 = ord_bits(x);
         by0 = by1 = ord_bits(y);
     }
-    const int slot = wave_tile_slot(t, valid, tile_count);
-    if (valid) {
-        key[s] = k;
-        tslot[s] = slot;
-    }
+    wave_tile_count(t, valid, tile_count);
+    if (valid) key[s] = k;
     block_partials(v2, bx0, bx1, by0, by1, part);
 }
 
@@ -323,20 +319,51 @@ __global__ void k_tile_offsets(Ctl* c, const int* __restrict__ tile_count, int n
 
 // ---------------------------------------------------------------- sort
 
-// Bucket storage slots by tile (order inside a tile still arbitrary).
-__global__ void k_scatter(Ctl* c, const int* __restrict__ key, const int* __restrict__ tslot,
-                          const int* __restrict__ tile_begin, int* __restrict__ tmp) {
+// Buckets the storage slots by tile: each particle takes the next slot of its
+// tile from a per-tile cursor (the tile sort clears the cursors of the tiles
+// it orders).  A wave handles SC_PER groups of 64 consecutive slots and issues
+// the cursor atomics of all groups (one per run of equal tiles) before it
+// waits for any of them.  Arrival order inside a tile is arbitrary: the tile
+// sort makes it canonical.
+constexpr int SC_PER = kScatterPerThread;
+__global__ void __launch_bounds__(WG) k_scatter(Ctl* c, const int* __restrict__ key,
+                                                const int* __restrict__ tile_begin, int* __restrict__ cursor,
+                                                int* __restrict__ tmp) {
     if (c->stop) return;
-    const int s = blockIdx.x * WG + threadIdx.x;
-    if (s >= c->np) return;
-    const int t = key[s] >> 8;
-    tmp[tile_begin[t] + tslot[s]] = s;
+    const int n = c->np, lane = threadIdx.x & 63;
+    const int s0 = (blockIdx.x * WG + (threadIdx.x & ~63)) * SC_PER + lane;
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    int t[SC_PER], start[SC_PER], base[SC_PER];
+    bool valid[SC_PER];
+#pragma unroll
+    for (int j = 0; j < SC_PER; ++j) {
+        const int s = s0 + 64 * j;
+        valid[j] = s < n;
+        t[j] = valid[j] ? key[s] >> 8 : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < SC_PER; ++j) {
+        const int tprev = __shfl_up(t[j], 1, 64);
+        const bool head = valid[j] && (lane == 0 || tprev != t[j]);
+        const unsigned long long vmask = __ballot(valid[j]);
+        const unsigned long long hmask = __ballot(head);
+        const unsigned long long hle = hmask & upto;
+        const unsigned long long brk = (hmask | ~vmask) & ~upto; // next head or first invalid lane
+        start[j] = hle ? 63 - __clzll((long long)hle) : 0;
+        const int end = brk ? (__ffsll((long long)brk) - 1) : 64;
+        base[j] = head ? atomicAdd(&cursor[t[j]], end - lane) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < SC_PER; ++j) {
+        const int b = __shfl(base[j], start[j], 64);
+        if (valid[j]) tmp[tile_begin[t[j]] + b + (lane - start[j])] = s0 + 64 * j;
+    }
 }
 
 // Per particle tile: counting sort by cell in LDS, then each cell's run sorted
 // by creation index, giving the canonical order (tile, cell, creation index).
 // Writes idx (sorted position -> storage slot) and cell_begin, and clears the
-// tile's count for the next step's g2p.
+// tile's count (for the next step's g2p) and its scatter cursor.
 // Tiles of at most TS_CAP particles are staged in LDS (cell, creation index,
 // storage index), counted and placed by cell there, and each cell's run is
 // insertion-sorted by creation index in LDS; larger tiles do the same steps
@@ -348,8 +375,9 @@ constexpr int TS_GROUP = 4;
 __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, const int32_t* __restrict__ pidA,
                                                   const int32_t* __restrict__ pidB, const int* __restrict__ key,
                                                   const int* __restrict__ ptiles, int* __restrict__ tile_count,
-                                                  const int* __restrict__ tile_begin, const int* __restrict__ tmp,
-                                                  int* __restrict__ idx, int* __restrict__ cell_begin) {
+                                                  int* __restrict__ cursor, const int* __restrict__ tile_begin,
+                                                  const int* __restrict__ tmp, int* __restrict__ idx,
+                                                  int* __restrict__ cell_begin) {
     if (c->stop) return;
     __shared__ int s_hist[TILE_CELLS];
     __shared__ int s_off[TILE_CELLS];
@@ -409,6 +437,7 @@ __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, const int32_t* __restr
             if (threadIdx.x == 0) {
                 cell_begin[(size_t)t * CELL_STRIDE + TILE_CELLS] = b + n;
                 tile_count[t] = 0;
+                cursor[t] = 0;
             }
         }
         __syncthreads();
@@ -685,8 +714,8 @@ __global__ void __launch_bounds__(WG) k_grid64(Ctl* c, const int* __restrict__ g
 __global__ void __launch_bounds__(WG) k_g2p(Ctl* c, PBuf64 A, PBuf64 B, const int* __restrict__ idx,
                                             const double4* __restrict__ vf, const double* __restrict__ tz,
                                             const double* __restrict__ tgx, const double* __restrict__ tgy,
-                                            int* __restrict__ key, int* __restrict__ tslot,
-                                            int* __restrict__ tile_count, unsigned long long* __restrict__ part) {
+                                            int* __restrict__ key, int* __restrict__ tile_count,
+                                            unsigned long long* __restrict__ part) {
     if (c->stop) return;
     const PBuf64 P = cur_buf(c, A, B);
     const PBuf64 Q = nxt_buf(c, A, B);
@@ -752,11 +781,8 @@ __global__ void __launch_bounds__(WG) k_g2p(Ctl* c, PBuf64 A, PBuf64 B, const in
         bx0 = bx1 = ord_bits(xn);
         by0 = by1 = ord_bits(yn);
     }
-    const int slot = wave_tile_slot(t, valid, tile_count);
-    if (valid) {
-        key[r] = kk;
-        tslot[r] = slot;
-    }
+    wave_tile_count(t, valid, tile_count);
+    if (valid) key[r] = kk;
     block_partials(v2, bx0, bx1, by0, by1, part);
 }
 

@@ -114,6 +114,53 @@ def test_tiny_steps_all_friction_laws(fric):
     assert _rel(hm, ohm) <= 1e-9 and _rel(vm, ovm) <= 1e-9
 
 
+def _dense_cfg(n_per_axis, cells, corner):
+    """`cells` x `cells` terrain cells (dx = 2 m) from (corner, corner) holding
+    n_per_axis^2 particles each on a regular sub-lattice, on a 96 x 80-node
+    inclined plane with a valley across it."""
+    from slabpro import configs
+    nx, ny, dx = 96, 80, 2.0
+    xs = np.arange(nx) * dx
+    ys = np.arange(ny) * dx
+    z = np.ascontiguousarray(500.0 - 0.45 * xs[:, None] + 0.002 * (ys[None, :] - 80.0) ** 2)
+    o = (np.arange(n_per_axis) + 0.5) / n_per_axis
+    ci = np.arange(cells)
+    px = (corner / dx + ci[:, None, None, None] + o[None, None, :, None]) * dx
+    py = (corner / dx + ci[None, :, None, None] + o[None, None, None, :]) * dx
+    px, py = np.broadcast_arrays(px, py)
+    px, py = px.ravel().copy(), py.ravel().copy()
+    part = configs.finish_particles(px, py, np.ones(len(px)), (cells * dx) ** 2, 300.0)
+    params = dict(gravity=9.81, rho=300.0, C_cfl=0.5, dt_max=0.01, flip_ratio=0.95, output_dt=0.5,
+                  total_time=0.3)
+    return {"name": "dense", "z": z, "dx": dx, "params": params,
+            "friction": {"type": 1, "mu": 0.155, "xi": 2000.0}, **part}
+
+
+@pytest.mark.parametrize("n_per_axis,cells,corner", [(6, 6, 68.0), (6, 14, 60.0), (20, 4, 66.0)],
+                         ids=["1296_in_lds", "5184_beyond_lds", "6400_cells_of_400"])
+def test_dense_tiles_match_oracle(n_per_axis, cells, corner):
+    """Tiles of 1,296 particles (sorted in LDS), 5,184 and 6,400 particles
+    (beyond the LDS capacity of 2,048: sorted in global memory; the last with
+    400 particles per cell) follow the oracle: 1e-12 after one step, 1e-9 after
+    twenty."""
+    cfg = _dense_cfg(n_per_axis, cells, corner)
+    run = _make(cfg)
+    orc = oracle_sim(cfg)
+    run.prime()
+    orc.prime()
+    _cmp_grid(run, orc, 1e-12, "prime")
+    for n in range(1, 21):
+        info = run.step(1)
+        orc.step()
+        sc = orc.scalars()
+        assert info["step"] == sc["step"]
+        assert abs(info["dt"] - sc["dt"]) <= 1e-12 * sc["dt"]
+        if n in (1, 20):
+            tol = 1e-12 if n == 1 else 1e-9
+            _cmp_grid(run, orc, tol, f"step{n}")
+            _cmp_particles(run, orc, tol, f"step{n}")
+
+
 def test_run_twice_bitwise_identical():
     cfg = _cfgs().tiny(fric={"type": 1, "mu": 0.2, "xi": 1000.0})
     outs = []

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -195,9 +196,10 @@ __global__ void __launch_bounds__(64) k_evmax(const ECtl* c, unsigned long long*
     vmax_slot(vmx + (size_t)blockIdx.x * VMAX_SHARDS);
 }
 
-__global__ void __launch_bounds__(WG) k_ebegin(ECtl* c, Slot* __restrict__ slots,
-                                               unsigned long long* __restrict__ vmx, int reduced) {
-    if (c->stop) return;
+// Run by one workgroup: the extra block of k_ereduce (the tile counting does
+// not read what this writes: its counts come from the previous G2P).
+__device__ void ebegin_block(ECtl* c, Slot* __restrict__ slots, unsigned long long* __restrict__ vmx,
+                             int reduced) {
     if (c->compact) { // re-sort pass: no slot advances
         if (threadIdx.x == 0) {
             c->epoch += 1;
@@ -401,10 +403,19 @@ __device__ __forceinline__ bool scan_block_idle(const ECtl* c, const Slot* __res
     return s0 == s1 && s0 < c->nslots && slots[s0].status == SLOT_FREE;
 }
 
-__global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, const Slot* __restrict__ slots,
+// Per scan block: tile counts -> the block's numbers of particles, particle
+// tiles and grid tiles.  The last block starts the step instead
+// (ebegin_block); the counting reads only the tile counts and which slots are
+// FREE, neither of which the start of a step changes.
+__global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, Slot* __restrict__ slots,
                                                 const int* __restrict__ tile_count, int ntiles,
-                                                int* __restrict__ bsum) {
+                                                int* __restrict__ bsum, unsigned long long* __restrict__ vmx,
+                                                int reduced) {
     if (c->stop) return;
+    if (blockIdx.x == gridDim.x - 1) {
+        ebegin_block(c, slots, vmx, reduced);
+        return;
+    }
     if (scan_block_idle(c, slots, ntiles)) {
         if (threadIdx.x < 3) bsum[3 * blockIdx.x + threadIdx.x] = 0;
         return;
@@ -424,6 +435,19 @@ __global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, const Slot* __restrict_
         for (int k = 0; k < 3; ++k) bsum[3 * blockIdx.x + k] = tot[k];
 }
 
+// The totals of k_escan (particles, particle tiles, grid tiles) into the
+// control block; an active-tile count above the capacity stops the step with
+// an error.  Thread 0 of one block.
+__device__ __forceinline__ void scan_totals(ECtl* c, const int tot[3]) {
+    c->n_sorted = tot[0];
+    c->n_ptiles = tot[1];
+    c->n_gtiles = tot[2];
+    if (tot[1] > c->tile_cap) {
+        c->err = 7;
+        c->stop = STOP_ERROR;
+    }
+}
+
 __global__ void __launch_bounds__(WG) k_escan(ECtl* c, const int* __restrict__ bsum, int nblk,
                                               int* __restrict__ boff) {
     if (c->stop) return;
@@ -439,24 +463,55 @@ __global__ void __launch_bounds__(WG) k_escan(ECtl* c, const int* __restrict__ b
             for (int k = 0; k < 3; ++k) boff[3 * b + k] = run[k] + ex[k];
         for (int k = 0; k < 3; ++k) run[k] += tot[k];
     }
-    if (threadIdx.x == 0) {
-        c->n_sorted = run[0];
-        c->n_ptiles = run[1];
-        c->n_gtiles = run[2];
-        if (run[1] > c->tile_cap) { // active-tile capacity exceeded: stop with an error
-            c->err = 7;
-            c->stop = STOP_ERROR;
-        }
-    }
+    if (threadIdx.x == 0) scan_totals(c, run);
 }
 
+// Exclusive prefix over the scan blocks before this one, and the total, of
+// the three block sums, summed by this workgroup from bsum directly (used
+// when there are few scan blocks: it saves the k_escan launch).
+__device__ __forceinline__ void block_prefix3(const int* __restrict__ bsum, int nblk, int pre[3], int tot[3]) {
+    __shared__ int s_r[4][6];
+    int v[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += WG)
+        for (int k = 0; k < 3; ++k) {
+            const int x = bsum[3 * b + k];
+            v[k] += b < (int)blockIdx.x ? x : 0;
+            v[3 + k] += x;
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 6; ++k) v[k] += __shfl_xor(v[k], o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 6; ++k) s_r[wv][k] = v[k];
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) {
+        pre[k] = s_r[0][k] + s_r[1][k] + s_r[2][k] + s_r[3][k];
+        tot[k] = s_r[0][3 + k] + s_r[1][3 + k] + s_r[2][3 + k] + s_r[3][3 + k];
+    }
+    __syncthreads();
+}
+
+// Per scan block: tile begin offsets and the particle / grid tile lists.  The
+// block's starting offsets come from k_escan (boff), or, with few scan blocks
+// (merged != 0), from block_prefix3, block 0 then writing the totals.
 __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict__ slots,
                                                  const int* __restrict__ tile_count, int ntiles,
-                                                 const int* __restrict__ boff, int* __restrict__ tile_begin,
+                                                 const int* __restrict__ boff, const int* __restrict__ bsum,
+                                                 int nblk, int merged, int* __restrict__ tile_begin,
                                                  int* __restrict__ ptiles, int* __restrict__ gtiles,
                                                  int* __restrict__ pstamp, int* __restrict__ gstamp,
                                                  int* __restrict__ tile_pos) {
-    if (c->stop || scan_block_idle(c, slots, ntiles)) return;
+    if (c->stop) return;
+    int base[3];
+    if (merged) {
+        int tot[3];
+        block_prefix3(bsum, nblk, base, tot);
+        if (blockIdx.x == 0 && threadIdx.x == 0) scan_totals(c, tot);
+        if (tot[1] > c->tile_cap) return; // every block sees the same totals
+    } else {
+        for (int q = 0; q < 3; ++q) base[q] = boff[3 * blockIdx.x + q];
+    }
+    if (scan_block_idle(c, slots, ntiles)) return;
     const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
     int f[SCAN_ITEMS][3];
     int acc[3] = {0, 0, 0};
@@ -467,7 +522,7 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
     int ex[3], tot[3];
     block_exscan3e(acc, ex, tot);
     int run[3];
-    for (int q = 0; q < 3; ++q) run[q] = boff[3 * blockIdx.x + q] + ex[q];
+    for (int q = 0; q < 3; ++q) run[q] = base[q] + ex[q];
     const int ep = c->epoch;
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         const int t = t0 + k;
@@ -1330,6 +1385,9 @@ struct Engine32 {
     Slot* d_slots = nullptr;
     Slot* h_slots = nullptr;
     int nslots = 1, nx = 0, ny = 0, ntx = 0, nty = 0, ntd = 0, ntiles = 0, nblk = 0;
+    // up to this many scan blocks, k_eoffsets forms the block offsets itself
+    // (each block reads all block sums) instead of a k_escan launch
+    int scan_merge_max = 2048;
     size_t nn = 0;
     double dx = 0, inv = 0;
     bool single = true;
@@ -1432,16 +1490,16 @@ void enqueue_estep(Engine32* e, int step) {
     ECtl* c = e->d_c;
     const unsigned tb = (unsigned)std::max(1, std::min(e->ntiles, e->persist));
     const unsigned pb = (unsigned)std::max(1, std::min(e->tile_cap, e->persist));
-    tk(e, 0, step, [&] {
-        const int many = e->nslots > 4;
-        if (many) k_evmax<<<e->nslots, 64, 0, s>>>(c, e->vmx);
-        k_ebegin<<<1, WG, 0, s>>>(c, e->d_slots, e->vmx, many);
+    const int many = e->nslots > 4;
+    if (many) tk(e, 0, step, [&] { k_evmax<<<e->nslots, 64, 0, s>>>(c, e->vmx); });
+    tk(e, 1, step, [&] {
+        k_ereduce<<<e->nblk + 1, WG, 0, s>>>(c, e->d_slots, e->tile_count, e->ntiles, e->bsum, e->vmx, many);
     });
-    tk(e, 1, step, [&] { k_ereduce<<<e->nblk, WG, 0, s>>>(c, e->d_slots, e->tile_count, e->ntiles, e->bsum); });
-    tk(e, 2, step, [&] { k_escan<<<1, WG, 0, s>>>(c, e->bsum, e->nblk, e->boff); });
+    const int merged = e->nblk <= e->scan_merge_max;
+    if (!merged) tk(e, 2, step, [&] { k_escan<<<1, WG, 0, s>>>(c, e->bsum, e->nblk, e->boff); });
     tk(e, 3, step, [&] {
-        k_eoffsets<<<e->nblk, WG, 0, s>>>(c, e->d_slots, e->tile_count, e->ntiles, e->boff, e->tile_begin, e->ptiles, e->gtiles,
-                                           e->pstamp, e->gstamp, e->tile_pos);
+        k_eoffsets<<<e->nblk, WG, 0, s>>>(c, e->d_slots, e->tile_count, e->ntiles, e->boff, e->bsum, e->nblk, merged,
+                                           e->tile_begin, e->ptiles, e->gtiles, e->pstamp, e->gstamp, e->tile_pos);
     });
     tk(e, 4, step, [&] {
         const unsigned nb = (unsigned)std::max<int64_t>(1, (e->launch_n + WG * SC_PER - 1) / (WG * SC_PER));
@@ -1584,6 +1642,7 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
         return nullptr;
     }
     e->nblk = (e->ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    if (const char* v = std::getenv("SLAB_SCAN_MERGE_MAX_BLOCKS")) e->scan_merge_max = std::atoi(v);
     e->cap = max_particles;
     e->tile_cap = (int)std::min<int64_t>(e->ntiles, std::max<int64_t>(65536, max_particles / 32));
     std::string why;

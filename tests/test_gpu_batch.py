@@ -204,6 +204,22 @@ def test_outflow_particles_are_parked_and_isolated():
     assert np.array_equal(hm_i[0], hm_a[0])
 
 
+def test_separate_tile_scan_launch_matches_merged(monkeypatch):
+    """Block offsets of the tile scan from a separate scan launch (the form
+    used beyond 2,048 scan blocks, forced here with
+    SLAB_SCAN_MERGE_MAX_BLOCKS=0) give bitwise the results of the offsets
+    formed inside the offsets kernel."""
+    scens = [_scenario(41, FRICS[0]), _scenario(42, FRICS[2])]
+    a = _run_batch(scens, total=1.0)
+    monkeypatch.setenv("SLAB_SCAN_MERGE_MAX_BLOCKS", "0")
+    b = _run_batch(scens, total=1.0)
+    for (ma, pa, ia), (mb, pb, ib) in zip(a, b):
+        assert ia["step"] == ib["step"] and ia["status"] == ib["status"]
+        for k in ("x", "y", "vx", "vy", "h", "z"):
+            assert np.array_equal(pa[k], pb[k]), k
+        assert np.array_equal(ma[0], mb[0]) and np.array_equal(ma[1], mb[1])
+
+
 @pytest.mark.parametrize("period", [1, 5])
 def test_resort_period_does_not_change_batch_results(period):
     """Every scenario of a mix is bitwise the same with storage re-sorted every

@@ -205,6 +205,7 @@ __device__ void ebegin_block(ECtl* c, Slot* __restrict__ slots, unsigned long lo
             c->epoch += 1;
             c->p2g_parity = c->parity;
             c->reorder = 1;
+            c->pend_end = 1;
         }
         return;
     }
@@ -268,19 +269,25 @@ __device__ void ebegin_block(ECtl* c, Slot* __restrict__ slots, unsigned long lo
         // a single run keeps the pre-G2P positions of output steps (its remesh
         // window is read from them), so those steps write the other buffer
         c->reorder = c->reorder_every <= 1 || ep % c->reorder_every == 0 || (c->single && s_out);
-        if (s_act == 0) c->stop = STOP_DONE; // nothing to advance
+        if (s_act == 0)
+            c->stop = STOP_DONE; // nothing to advance
+        else
+            c->pend_end = 1; // ended by the next step's first kernel or by k_eend
     }
 }
 
 // End of a step: time += dt and the frame trigger (src/simulation.cpp:503-509).
-__global__ void __launch_bounds__(WG) k_eend(ECtl* c, Slot* __restrict__ slots) {
-    if (c->stop) return;
+// Run by one workgroup, once per started step (pend_end), after its G2P:
+// by the step-start block of the next step's k_ereduce, or by k_eend after
+// the last step the host enqueued.
+__device__ void eend_block(ECtl* c, Slot* __restrict__ slots) {
     if (c->compact) {
         if (threadIdx.x == 0) {
             c->parity ^= 1;
             c->n_store = c->n_sorted;
             c->compact = 0;
             c->reorder = 0;
+            c->pend_end = 0;
         }
         return;
     }
@@ -310,7 +317,13 @@ __global__ void __launch_bounds__(WG) k_eend(ECtl* c, Slot* __restrict__ slots) 
         }
         if (c->n_active > 0) c->steps_in_call += 1;
         if (c->single && s_hit) c->stop = STOP_FRAME;
+        c->pend_end = 0;
     }
+}
+
+__global__ void __launch_bounds__(WG) k_eend(ECtl* c, Slot* __restrict__ slots) {
+    if (c->stop || !c->pend_end) return;
+    eend_block(c, slots);
 }
 
 // --- keys of admitted particles ---------------------------------------------
@@ -404,15 +417,24 @@ __device__ __forceinline__ bool scan_block_idle(const ECtl* c, const Slot* __res
 }
 
 // Per scan block: tile counts -> the block's numbers of particles, particle
-// tiles and grid tiles.  The last block starts the step instead
-// (ebegin_block); the counting reads only the tile counts and which slots are
-// FREE, neither of which the start of a step changes.
+// tiles and grid tiles.  The last block ends the previous step if it is
+// pending (eend_block) and starts this one (ebegin_block); the counting reads
+// only the tile counts and which slots are FREE, neither of which the end or
+// start of a step changes.
 __global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, Slot* __restrict__ slots,
                                                 const int* __restrict__ tile_count, int ntiles,
                                                 int* __restrict__ bsum, unsigned long long* __restrict__ vmx,
                                                 int reduced) {
     if (c->stop) return;
     if (blockIdx.x == gridDim.x - 1) {
+        __shared__ int s_stop;
+        if (c->pend_end) { // end of the previous step first (it may stop at a frame)
+            eend_block(c, slots);
+            __syncthreads();
+            if (threadIdx.x == 0) s_stop = c->stop;
+            __syncthreads();
+            if (s_stop) return;
+        }
         ebegin_block(c, slots, vmx, reduced);
         return;
     }
@@ -1521,7 +1543,12 @@ void enqueue_estep(Engine32* e, int step) {
         k_eg2p<<<xcd_grid((e->launch_n + WG - 1) / WG), WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->tz, e->tgx, e->tgy, e->key,
                                            e->tile_count, e->vmx);
     });
-    tk(e, 9, step, [&] { k_eend<<<1, WG, 0, s>>>(c, e->d_slots); });
+}
+
+// Ends the last enqueued step (a no-op when none is pending); enqueued before
+// the host reads the control block.
+void enqueue_eend(Engine32* e, int step) {
+    tk(e, 9, step, [&] { k_eend<<<1, WG, 0, e->stream>>>(e->d_c, e->d_slots); });
 }
 
 void drop_graph(Engine32* e) {
@@ -1867,6 +1894,7 @@ bool engine_step(Engine32* e, int max_steps, std::string* err) {
                 enqueue_estep(e, done + k);
             }
         }
+        enqueue_eend(e, done + chunk - 1);
         if (!hip_ok(hipGetLastError(), "step launch", err)) return false;
         if (!engine_pull(e, err)) return false;
         if (e->timing) collect(e, e->h_c->steps_in_call);
@@ -1909,6 +1937,7 @@ bool engine_release(Engine32* e, int slot, std::string* err) {
         const bool tm = e->timing;
         e->timing = false;
         enqueue_estep(e, 0);
+        enqueue_eend(e, 0);
         e->timing = tm;
         if (!hip_ok(hipGetLastError(), "release pass", err)) return false;
     }

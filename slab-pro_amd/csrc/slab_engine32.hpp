@@ -58,7 +58,7 @@ struct ECtl {
     int32_t compact;       // this pass only re-sorts (drops released particles)
     int32_t reorder;       // this step's G2P writes the other buffer in sorted order
     int32_t reorder_every; // storage re-sort period in steps (other steps update in place)
-    int32_t pad0;
+    int32_t pend_end;      // a started step still has to be ended (time, frames, buffer flip)
     double dx, inv_dx;
 };
 

@@ -178,11 +178,17 @@ def test_run_twice_bitwise_identical():
     assert np.array_equal(m1[0], m2[0]) and np.array_equal(m1[1], m2[1])
 
 
-def test_frames_and_stop_semantics():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_frames_and_stop_semantics(precision):
+    """The run returns at every output frame with the oracle's frame and step
+    counters and time, and stops at total_time (both forms: dt_max binds on
+    this case, so the FP32 form takes the same steps)."""
+    import slabpro
+    from slabpro import abi
     cfg = _cfgs().tiny()
     cfg["params"]["total_time"] = 1.2
     cfg["params"]["output_dt"] = 0.25
-    run = _make(cfg)
+    run = slabpro.make_run(cfg, precision=abi.FP64 if precision == "fp64" else abi.FP32, keep_grid=True)
     orc = oracle_sim(cfg)
     run.prime()
     orc.prime()
@@ -201,7 +207,7 @@ def test_frames_and_stop_semantics():
             ofr.append((s["frame"], s["step"], s["time"]))
         if r == -1 or orc.scalars()["time"] >= 1.2:
             break
-    assert [f[:2] for f in frames] == [f[:2] for f in ofr]
+    assert len(ofr) == 4 and [f[:2] for f in frames] == [f[:2] for f in ofr]
     for a, b in zip(frames, ofr):
         assert abs(a[2] - b[2]) <= 1e-12
 

@@ -659,7 +659,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
     __shared__ int s_hist[TILE_CELLS];
     __shared__ int s_off[TILE_CELLS];
     __shared__ int s_w[4];
-    __shared__ int s_n, s_c1, s_cmax;
+    __shared__ int s_c1, s_cmax;
     // one LDS region, three uses: (a) creation index, arrival order and cell of
     // a small tile; (b) 64-bit keys of a small tile; (c) keys + storage index of
     // a round
@@ -673,14 +673,20 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
     const int nt = c->n_ptiles, live = c->nslots * c->ntd;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         const int t = ptiles[w];
-        if (threadIdx.x == 0) {
-            s_n = tile_count[t];
-            s_cmax = 0;
+        const int n = tile_count[t], b = tile_begin[t]; // n >= 1: listed tiles hold particles
+        // the first group of bucket reads and their key and creation-index
+        // reads touch no LDS: issued before the barrier, they overlap it
+        int sv0[ES_GROUP], kc0[ES_GROUP], pd0[ES_GROUP];
+#pragma unroll
+        for (int j = 0; j < ES_GROUP; ++j) sv0[j] = tmp[b + min((int)threadIdx.x + j * WG, n - 1)];
+#pragma unroll
+        for (int j = 0; j < ES_GROUP; ++j) {
+            kc0[j] = key[sv0[j]];
+            pd0[j] = pid[sv0[j]];
         }
+        if (threadIdx.x == 0) s_cmax = 0;
         s_hist[threadIdx.x] = 0;
         __syncthreads();
-        const int n = s_n;
-        const int b = tile_begin[t];
         int* cb = cell_begin + (size_t)w * CELL_STRIDE;
         if (t >= live) { // parking tile
             for (int r = threadIdx.x; r < n; r += WG) idx[b + r] = tmp[b + r];
@@ -705,12 +711,22 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
             for (int k0 = 0; k0 < ES_CAP / WG; k0 += ES_GROUP) {
                 if (k0 * WG < n) {
                     int sv[ES_GROUP], kc[ES_GROUP], pd[ES_GROUP];
+                    if (k0 == 0) {
 #pragma unroll
-                    for (int j = 0; j < ES_GROUP; ++j) sv[j] = tmp[b + min((int)threadIdx.x + (k0 + j) * WG, n - 1)];
+                        for (int j = 0; j < ES_GROUP; ++j) {
+                            sv[j] = sv0[j];
+                            kc[j] = kc0[j];
+                            pd[j] = pd0[j];
+                        }
+                    } else {
 #pragma unroll
-                    for (int j = 0; j < ES_GROUP; ++j) {
-                        kc[j] = key[sv[j]];
-                        pd[j] = pid[sv[j]];
+                        for (int j = 0; j < ES_GROUP; ++j)
+                            sv[j] = tmp[b + min((int)threadIdx.x + (k0 + j) * WG, n - 1)];
+#pragma unroll
+                        for (int j = 0; j < ES_GROUP; ++j) {
+                            kc[j] = key[sv[j]];
+                            pd[j] = pid[sv[j]];
+                        }
                     }
 #pragma unroll
                     for (int j = 0; j < ES_GROUP; ++j) {
@@ -731,8 +747,13 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
         } else {
             for (int r0 = threadIdx.x; r0 < n; r0 += WG * ES_GROUP) {
                 int kc[ES_GROUP];
+                if (r0 == (int)threadIdx.x) { // the first group was read before the barrier
 #pragma unroll
-                for (int j = 0; j < ES_GROUP; ++j) kc[j] = key[tmp[b + min(r0 + j * WG, n - 1)]];
+                    for (int j = 0; j < ES_GROUP; ++j) kc[j] = kc0[j];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < ES_GROUP; ++j) kc[j] = key[tmp[b + min(r0 + j * WG, n - 1)]];
+                }
 #pragma unroll
                 for (int j = 0; j < ES_GROUP; ++j)
                     if (r0 + j * WG < n) atomicAdd(&s_hist[kc[j] & 255], 1);
@@ -931,6 +952,18 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         // P_z only feeds the grid's vz output, written on output steps
         const bool need_vz = slots[slot].out_step != 0;
         const int tx = local / nty, ty = local - tx * nty;
+        // this thread's particle range and its first particle's loads do not
+        // depend on the LDS staging below: issued first, they overlap it
+        const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
+        const int r0 = cb[cid], r1 = cb[cid + 1];
+        float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
+        if (r0 < r1) {
+            const int sn = idx[r0];
+            const float2 po = P.o[sn], pv = P.v[sn];
+            nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
+            nvx = pv.x; nvy = pv.y;
+            if (need_vz) nvz = P.vz[sn];
+        }
         for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
         // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged once
         for (int q = threadIdx.x; q < TCELLS * TCELLS; q += WG) {
@@ -953,17 +986,7 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         for (int k = 0; k < 9; ++k)
 #pragma unroll
             for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
-        const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
-        const int r0 = cb[cid], r1 = cb[cid + 1];
         // the next particle's loads are issued before the current one is summed
-        float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
-        if (r0 < r1) {
-            const int sn = idx[r0];
-            const float2 po = P.o[sn], pv = P.v[sn];
-            nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
-            nvx = pv.x; nvy = pv.y;
-            if (need_vz) nvz = P.vz[sn];
-        }
         for (int r = r0; r < r1; ++r) {
             const float ox = nox, oy = noy, mp = nmp, hp = nhp, pvx = nvx, pvy = nvy, pvz = nvz;
             if (r + 1 < r1) {

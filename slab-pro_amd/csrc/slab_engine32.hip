@@ -611,10 +611,12 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
 // Per particle tile: counting sort by cell, then each cell's run sorted by
 // creation index: canonical order (slot, tile, cell, creation index).
 //  * Tiles of at most ES_CAP particles stage (cell, creation index) in LDS with
-//    one coalesced pass.  If no cell holds more than ES_INSERT particles, each
-//    cell's run is insertion-sorted by its thread; otherwise the whole tile is
-//    bitonic-sorted on 64-bit keys (cell, creation index, arrival position),
-//    whose cost does not depend on how disordered the arrivals are.
+//    one coalesced pass.  If no cell holds more than ES_RANK particles, each
+//    particle's place in its cell is its rank there: the number of the cell's
+//    particles with a smaller creation index (unique within a slot), counted
+//    by the particle's own thread, so no cell is sorted serially; otherwise
+//    the whole tile is bitonic-sorted on 64-bit keys (cell, creation index,
+//    arrival position).
 //  * Larger tiles place their particles by cell in global memory, then sort
 //    the cells in rounds: each round bitonic-sorts a contiguous range of whole
 //    cells (at most ES_ROUND particles) in LDS; only a single cell of more than
@@ -622,7 +624,7 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
 //  * Parking tiles keep their bucket order (their particles are only carried).
 constexpr int ES_CAP = 4096;
 constexpr int ES_ROUND = 2048;
-constexpr int ES_INSERT = 32;
+constexpr int ES_RANK = 256;
 constexpr int ES_GROUP = 4; // particles per thread whose global reads are issued together
 constexpr int ES_LDS_BYTES = ES_CAP * 8;
 
@@ -761,7 +763,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
         }
         __syncthreads();
         const int cnt = s_hist[threadIdx.x];
-        if (cnt > ES_INSERT) atomicMax(&s_cmax, cnt);
+        if (cnt > ES_RANK) atomicMax(&s_cmax, cnt);
         {
             const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
             int inc = cnt;
@@ -784,30 +786,23 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
         __syncthreads();
         const int off = s_off[threadIdx.x];
         s_hist[threadIdx.x] = off; // cursor
-        const bool dense = s_cmax > ES_INSERT;
+        const bool dense = s_cmax > ES_RANK;
         __syncthreads();
         if (small && !dense) {
             for (int r = threadIdx.x; r < n; r += WG) s_ord[atomicAdd(&s_hist[s_cell[r]], 1)] = (unsigned short)r;
             __syncthreads();
-            for (int q = off + 1; q < off + cnt; ++q) {
-                const unsigned short rv = s_ord[q];
-                const int pv = s_pid[rv];
-                int k = q - 1;
-                while (k >= off && s_pid[s_ord[k]] > pv) {
-                    s_ord[k + 1] = s_ord[k];
-                    --k;
-                }
-                s_ord[k + 1] = rv;
-            }
-            __syncthreads();
-            // final position of each arrival (over the creation indices, no longer read)
-            unsigned short* s_pos = reinterpret_cast<unsigned short*>(s_mem);
-            for (int q = threadIdx.x; q < n; q += WG) s_pos[s_ord[q]] = (unsigned short)q;
-            __syncthreads();
+            // s_off[c] .. s_hist[c] now lists cell c's arrivals; each arrival
+            // is placed from its registers at the cell start plus its rank
 #pragma unroll
             for (int k = 0; k < ES_CAP / WG; ++k) {
                 const int r = threadIdx.x + k * WG;
-                if (r < n) idx[b + s_pos[r]] = svr[k];
+                if (r < n) {
+                    const int cr = s_cell[r], pr = s_pid[r];
+                    const int lo = s_off[cr], hi = s_hist[cr];
+                    int rank = 0;
+                    for (int q = lo; q < hi; ++q) rank += s_pid[s_ord[q]] < pr ? 1 : 0;
+                    idx[b + lo + rank] = svr[k];
+                }
             }
         } else if (small) {
             // keys (cell << 44 | creation index << 13 | arrival position), built

@@ -250,14 +250,16 @@ def _dense(n_per_axis, cells, x0, y0, h=1.0):
     return configs.finish_particles(px, py, hh, area * h, 300.0)
 
 
-@pytest.mark.parametrize("n_per_axis,cells,corner", [(6, 6, 68.0), (6, 14, 60.0), (20, 4, 66.0), (5, 16, 64.0)])
+@pytest.mark.parametrize("n_per_axis,cells,corner",
+                         [(6, 6, 68.0), (17, 3, 66.0), (6, 14, 60.0), (20, 4, 66.0), (5, 16, 64.0)])
 def test_dense_tiles_sort_canonically(n_per_axis, cells, corner):
-    """A tile within the in-LDS sort capacity whose cells hold 36 particles
-    (bitonic sort of the whole tile), and tiles beyond it (4,761 particles in
-    cells of 36; 6,400 in cells of ~100; ~6,000 in cells of at most 25: LDS
-    rounds of whole cells) are ordered
-    canonically: results are bitwise the same alone, in a mix and with another
-    storage re-sort period, and stay close to the FP64 form."""
+    """Tiles within the in-LDS sort capacity whose cells hold 36 particles
+    (each placed by its rank in its cell) and 289 particles (bitonic sort of
+    the whole tile), and tiles beyond it (4,761 particles in cells of 36;
+    6,400 in cells of ~100; ~6,000 in cells of at most 25: LDS rounds of whole
+    cells) are ordered canonically: results are bitwise the same alone, in a
+    mix and with another storage re-sort period, and stay close to the FP64
+    form."""
     from slabpro import abi
     dense = (_dense(n_per_axis, cells, corner, corner), FRICS[1])
     other = _scenario(21, FRICS[0])

@@ -365,9 +365,9 @@ __global__ void __launch_bounds__(WG) k_scatter(Ctl* c, const int* __restrict__ 
 // Writes idx (sorted position -> storage slot) and cell_begin, and clears the
 // tile's count (for the next step's g2p) and its scatter cursor.
 // Tiles of at most TS_CAP particles are staged in LDS (cell, creation index,
-// storage index), counted and placed by cell there, and each cell's run is
-// insertion-sorted by creation index in LDS; larger tiles do the same steps
-// in global memory.  The dependent global reads (bucket, then key and creation
+// storage index), counted and placed by cell there, and each particle is
+// written at its cell's start plus its rank by creation index in the cell;
+// larger tiles place by cell and insertion-sort each cell in global memory.  The dependent global reads (bucket, then key and creation
 // index) are issued in groups of TS_GROUP per thread.
 constexpr int TS_CAP = 2048;
 constexpr int TS_GROUP = 4;
@@ -447,20 +447,16 @@ __global__ void __launch_bounds__(WG) k_tile_sort(Ctl* c, const int32_t* __restr
         if (small) {
             for (int r = threadIdx.x; r < n; r += WG) s_ord[atomicAdd(&s_hist[s_cell[r]], 1)] = (unsigned short)r;
             __syncthreads();
-            // insertion sort of each cell's run by creation index
-            const int lo = s_off[threadIdx.x], hi = s_hist[threadIdx.x];
-            for (int q = lo + 1; q < hi; ++q) {
-                const unsigned short rv = s_ord[q];
-                const int pv = s_pid[rv];
-                int k = q - 1;
-                while (k >= lo && s_pid[s_ord[k]] > pv) {
-                    s_ord[k + 1] = s_ord[k];
-                    --k;
-                }
-                s_ord[k + 1] = rv;
+            // s_off[c] .. s_hist[c] lists cell c's arrivals; each arrival goes
+            // to the cell start plus its rank there by creation index (unique),
+            // counted by its own thread: no cell is sorted serially
+            for (int r = threadIdx.x; r < n; r += WG) {
+                const int cr = s_cell[r], pr = s_pid[r];
+                const int lo = s_off[cr], hi = s_hist[cr];
+                int rank = 0;
+                for (int q = lo; q < hi; ++q) rank += s_pid[s_ord[q]] < pr ? 1 : 0;
+                idx[b + lo + rank] = s_sv[r];
             }
-            __syncthreads();
-            for (int q = threadIdx.x; q < n; q += WG) idx[b + q] = s_sv[s_ord[q]];
         } else {
             for (int r0 = threadIdx.x; r0 < n; r0 += WG * TS_GROUP) {
                 int sv[TS_GROUP], kc[TS_GROUP];

@@ -7,6 +7,7 @@
 #include "slab_kernels.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -53,6 +54,8 @@ struct slab_run {
     Ctl* d_ctl = nullptr;
     Ctl* h_ctl = nullptr; // pinned mirror
     int nx = 0, ny = 0, ntx = 0, nty = 0, ntiles = 0, nblk = 0;
+    // up to this many scan blocks, k_tile_offsets forms the block offsets itself
+    int scan_merge_max = 2048;
     size_t nn = 0;
     double dx = 0, inv_dx = 0;
     bool f32 = false;
@@ -170,13 +173,20 @@ void enqueue_keys(slab_run* r) {
 
 // Tile scan + bucketing + canonical sort + P2G terms: the particle order that
 // stands in for remesh + sort_particles_by_grid_block (src/simulation.cpp:584-724).
-void enqueue_sort(slab_run* r, int step) {
+// with_begin: the tile count reduction also starts the step (one extra
+// block); the prime passes 0.
+void enqueue_sort(slab_run* r, int step, int with_begin) {
     hipStream_t s = r->stream;
-    timed(r, 1, step, [&] { k_tile_reduce<<<r->nblk, WG, 0, s>>>(r->d_ctl, r->tile_count, r->ntiles, r->bsum); });
-    timed(r, 2, step, [&] { k_tile_scan<<<1, WG, 0, s>>>(r->d_ctl, r->bsum, r->nblk, r->boff); });
+    timed(r, 1, step, [&] {
+        k_tile_reduce<<<r->nblk + (with_begin ? 1 : 0), WG, 0, s>>>(r->d_ctl, r->tile_count, r->ntiles, r->bsum,
+                                                                    r->part, with_begin);
+    });
+    const int merged = r->nblk <= r->scan_merge_max;
+    if (!merged) timed(r, 2, step, [&] { k_tile_scan<<<1, WG, 0, s>>>(r->d_ctl, r->bsum, r->nblk, r->boff); });
     timed(r, 3, step, [&] {
-        k_tile_offsets<<<r->nblk, WG, 0, s>>>(r->d_ctl, r->tile_count, r->ntiles, r->boff, r->tile_begin, r->ptiles,
-                                               r->gtiles, r->pstamp, r->gstamp, r->tile_pos);
+        k_tile_offsets<<<r->nblk, WG, 0, s>>>(r->d_ctl, r->tile_count, r->ntiles, r->boff, r->bsum, r->nblk, merged,
+                                               r->tile_begin, r->ptiles, r->gtiles, r->pstamp, r->gstamp,
+                                               r->tile_pos);
     });
     timed(r, 4, step, [&] {
         const unsigned nb = (unsigned)((r->np + WG * kScatterPerThread - 1) / (WG * kScatterPerThread));
@@ -204,8 +214,7 @@ void enqueue_p2g(slab_run* r, int step) {
 
 void enqueue_step(slab_run* r, int step) {
     hipStream_t s = r->stream;
-    timed(r, 0, step, [&] { k_ctl_begin<<<1, WG, 0, s>>>(r->d_ctl, r->part); });
-    enqueue_sort(r, step);
+    enqueue_sort(r, step, 1);
     enqueue_p2g(r, step);
     timed(r, 8, step, [&] {
         k_g2p<<<pblocks(r), WG, 0, s>>>(r->d_ctl, r->A, r->B, r->idx, r->vf, r->tz, r->tgx, r->tgy, r->key,
@@ -388,6 +397,7 @@ slab_status slab_run_create(const slab_terrain* terrain, const slab_params* para
     r->nty = (ny + TILE - 1) / TILE;
     r->ntiles = r->ntx * r->nty;
     r->nblk = (r->ntiles + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    if (const char* v = std::getenv("SLAB_SCAN_MERGE_MAX_BLOCKS")) r->scan_merge_max = std::atoi(v);
     auto cleanup = [&](slab_status s) {
         slab_run_destroy(r);
         return s;
@@ -570,7 +580,7 @@ slab_status slab_run_prime(slab_run* r) {
     k_ctl_prime<<<1, WG, 0, s>>>(r->d_ctl, r->part);
     bool tm = r->timing;
     r->timing = false;
-    enqueue_sort(r, 0);
+    enqueue_sort(r, 0, 0);
     enqueue_p2g(r, 0);
     r->timing = tm;
     // keys again for step 1 (the tile sort consumed the counts)

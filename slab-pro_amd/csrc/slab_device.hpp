@@ -132,4 +132,29 @@ __device__ __forceinline__ void wave_tile_count(int t, bool valid, int* __restri
     if (head) atomicAdd(&tile_count[t], end - lane);
 }
 
+// Exclusive prefix over the scan blocks before this one, and the total, of
+// the three block sums, summed by this workgroup from bsum directly (used
+// when there are few scan blocks: it saves the separate scan launch).
+__device__ __forceinline__ void block_prefix3(const int* __restrict__ bsum, int nblk, int pre[3], int tot[3]) {
+    __shared__ int s_r[4][6];
+    int v[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += WG)
+        for (int k = 0; k < 3; ++k) {
+            const int x = bsum[3 * b + k];
+            v[k] += b < (int)blockIdx.x ? x : 0;
+            v[3 + k] += x;
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 6; ++k) v[k] += __shfl_xor(v[k], o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 6; ++k) s_r[wv][k] = v[k];
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) {
+        pre[k] = s_r[0][k] + s_r[1][k] + s_r[2][k] + s_r[3][k];
+        tot[k] = s_r[0][3 + k] + s_r[1][3 + k] + s_r[2][3 + k] + s_r[3][3 + k];
+    }
+    __syncthreads();
+}
+
 } // namespace slab

@@ -488,31 +488,6 @@ __global__ void __launch_bounds__(WG) k_escan(ECtl* c, const int* __restrict__ b
     if (threadIdx.x == 0) scan_totals(c, run);
 }
 
-// Exclusive prefix over the scan blocks before this one, and the total, of
-// the three block sums, summed by this workgroup from bsum directly (used
-// when there are few scan blocks: it saves the k_escan launch).
-__device__ __forceinline__ void block_prefix3(const int* __restrict__ bsum, int nblk, int pre[3], int tot[3]) {
-    __shared__ int s_r[4][6];
-    int v[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblk; b += WG)
-        for (int k = 0; k < 3; ++k) {
-            const int x = bsum[3 * b + k];
-            v[k] += b < (int)blockIdx.x ? x : 0;
-            v[3 + k] += x;
-        }
-    for (int o = 32; o > 0; o >>= 1)
-        for (int k = 0; k < 6; ++k) v[k] += __shfl_xor(v[k], o, 64);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int k = 0; k < 6; ++k) s_r[wv][k] = v[k];
-    __syncthreads();
-    for (int k = 0; k < 3; ++k) {
-        pre[k] = s_r[0][k] + s_r[1][k] + s_r[2][k] + s_r[3][k];
-        tot[k] = s_r[0][3 + k] + s_r[1][3 + k] + s_r[2][3 + k] + s_r[3][3 + k];
-    }
-    __syncthreads();
-}
-
 // Per scan block: tile begin offsets and the particle / grid tile lists.  The
 // block's starting offsets come from k_escan (boff), or, with few scan blocks
 // (merged != 0), from block_prefix3, block 0 then writing the totals.
@@ -524,6 +499,8 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
                                                  int* __restrict__ pstamp, int* __restrict__ gstamp,
                                                  int* __restrict__ tile_pos) {
     if (c->stop) return;
+    const bool idle = scan_block_idle(c, slots, ntiles);
+    if (idle && blockIdx.x != 0) return; // block 0 writes the totals even when idle
     int base[3];
     if (merged) {
         int tot[3];
@@ -533,7 +510,7 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
     } else {
         for (int q = 0; q < 3; ++q) base[q] = boff[3 * blockIdx.x + q];
     }
-    if (scan_block_idle(c, slots, ntiles)) return;
+    if (idle) return;
     const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
     int f[SCAN_ITEMS][3];
     int acc[3] = {0, 0, 0};

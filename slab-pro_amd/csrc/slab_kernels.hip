@@ -84,8 +84,9 @@ __global__ void k_ctl_clear(Ctl* c) {
 // of k_keys after upload) to max |v_p|^2 and the bounding box of the positions
 // this step's P2G uses; a step that proceeds resets the shards for its g2p
 // (a stopped one keeps them for diagnostics).
-__global__ void __launch_bounds__(WG) k_ctl_begin(Ctl* c, unsigned long long* __restrict__ part) {
-    if (c->stop) return;
+// Run by one workgroup: the extra block of k_tile_reduce (the tile counting
+// does not read what this writes: its counts come from the previous g2p).
+__device__ void ctl_begin_block(Ctl* c, unsigned long long* __restrict__ part) {
     __shared__ unsigned long long s_out[5];
     __shared__ int s_go;
     reduce_partials(part, s_out);
@@ -244,9 +245,16 @@ __device__ __forceinline__ void tile_flags(const Ctl* c, const int* __restrict__
     f[2] = g;
 }
 
-// Per-block sums of (particles, particle tiles, grid tiles).
-__global__ void k_tile_reduce(Ctl* c, const int* __restrict__ tile_count, int ntiles, int* __restrict__ bsum) {
+// Per-block sums of (particles, particle tiles, grid tiles).  With
+// with_begin != 0 the last block starts the step instead (ctl_begin_block).
+__global__ void __launch_bounds__(WG) k_tile_reduce(Ctl* c, const int* __restrict__ tile_count, int ntiles,
+                                                    int* __restrict__ bsum, unsigned long long* __restrict__ part,
+                                                    int with_begin) {
     if (c->stop) return;
+    if (with_begin && blockIdx.x == gridDim.x - 1) {
+        ctl_begin_block(c, part);
+        return;
+    }
     int acc[3] = {0, 0, 0};
     const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
     for (int k = 0; k < SCAN_ITEMS; ++k) {
@@ -283,11 +291,28 @@ __global__ void k_tile_scan(Ctl* c, const int* __restrict__ bsum, int nblk, int*
     }
 }
 
-// tile_begin, particle-tile list, grid-tile list (both in tile-id order) and stamps.
-__global__ void k_tile_offsets(Ctl* c, const int* __restrict__ tile_count, int ntiles, const int* __restrict__ boff,
-                               int* __restrict__ tile_begin, int* __restrict__ ptiles, int* __restrict__ gtiles,
-                               int* __restrict__ pstamp, int* __restrict__ gstamp, int* __restrict__ tile_pos) {
+// tile_begin, particle-tile list, grid-tile list (both in tile-id order) and
+// stamps.  The block's starting offsets come from k_tile_scan (boff), or, with
+// few scan blocks (merged != 0), from block_prefix3, block 0 then writing the
+// list lengths.
+__global__ void __launch_bounds__(WG) k_tile_offsets(Ctl* c, const int* __restrict__ tile_count, int ntiles,
+                                                     const int* __restrict__ boff, const int* __restrict__ bsum,
+                                                     int nblk, int merged, int* __restrict__ tile_begin,
+                                                     int* __restrict__ ptiles, int* __restrict__ gtiles,
+                                                     int* __restrict__ pstamp, int* __restrict__ gstamp,
+                                                     int* __restrict__ tile_pos) {
     if (c->stop) return;
+    int base[3];
+    if (merged) {
+        int tot[3];
+        block_prefix3(bsum, nblk, base, tot);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            c->n_ptiles = tot[1];
+            c->n_gtiles = tot[2];
+        }
+    } else {
+        for (int q = 0; q < 3; ++q) base[q] = boff[3 * blockIdx.x + q];
+    }
     const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
     int f[SCAN_ITEMS][3];
     int acc[3] = {0, 0, 0};
@@ -298,7 +323,7 @@ __global__ void k_tile_offsets(Ctl* c, const int* __restrict__ tile_count, int n
     int ex[3], tot[3];
     block_exscan3(acc, ex, tot);
     int run[3];
-    for (int q = 0; q < 3; ++q) run[q] = boff[3 * blockIdx.x + q] + ex[q];
+    for (int q = 0; q < 3; ++q) run[q] = base[q] + ex[q];
     const int ep = c->epoch;
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         const int t = t0 + k;

@@ -12,17 +12,18 @@ struct PBuf64 {
 };
 
 __global__ void k_ctl_clear(Ctl* c);
-__global__ void k_ctl_begin(Ctl* c, unsigned long long* part);
 __global__ void k_ctl_end(Ctl* c);
 __global__ void k_prime_pre(Ctl* c);
 __global__ void k_ctl_prime(Ctl* c, const unsigned long long* part);
 __global__ void k_part_init(unsigned long long* part);
 __global__ void k_keys(Ctl* c, PBuf64 A, PBuf64 B, int* key, int* tile_count, unsigned long long* part);
 __global__ void k_init_z(Ctl* c, PBuf64 A, const double* tz);
-__global__ void k_tile_reduce(Ctl* c, const int* tile_count, int ntiles, int* bsum);
+__global__ void k_tile_reduce(Ctl* c, const int* tile_count, int ntiles, int* bsum, unsigned long long* part,
+                              int with_begin);
 __global__ void k_tile_scan(Ctl* c, const int* bsum, int nblk, int* boff);
-__global__ void k_tile_offsets(Ctl* c, const int* tile_count, int ntiles, const int* boff, int* tile_begin,
-                               int* ptiles, int* gtiles, int* pstamp, int* gstamp, int* tile_pos);
+__global__ void k_tile_offsets(Ctl* c, const int* tile_count, int ntiles, const int* boff, const int* bsum,
+                               int nblk, int merged, int* tile_begin, int* ptiles, int* gtiles, int* pstamp,
+                               int* gstamp, int* tile_pos);
 __global__ void k_scatter(Ctl* c, const int* key, const int* tile_begin, int* cursor, int* tmp);
 constexpr int kScatterPerThread = 8; // particles per thread of k_scatter
 __global__ void k_tile_sort(Ctl* c, const int32_t* pidA, const int32_t* pidB, const int* key, const int* ptiles,

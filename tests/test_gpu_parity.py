@@ -178,6 +178,30 @@ def test_run_twice_bitwise_identical():
     assert np.array_equal(m1[0], m2[0]) and np.array_equal(m1[1], m2[1])
 
 
+def test_separate_tile_scan_launch_matches_merged(monkeypatch):
+    """FP64 block offsets of the tile scan from a separate scan launch (the
+    form used beyond 2,048 scan blocks, forced with
+    SLAB_SCAN_MERGE_MAX_BLOCKS=0) give bitwise the results of the offsets
+    formed inside the offsets kernel."""
+    cfg = _cfgs().tiny(fric={"type": 0, "mu": 0.3})
+    outs = []
+    for split in (False, True):
+        if split:
+            monkeypatch.setenv("SLAB_SCAN_MERGE_MAX_BLOCKS", "0")
+        run = _make(cfg)
+        run.prime()
+        info = run.step(40)
+        outs.append((run.particles(), run.grid(), run.max_maps(), info))
+        run.close()
+    (p1, g1, m1, i1), (p2, g2, m2, i2) = outs
+    assert i1["step"] == i2["step"] and i1["time"] == i2["time"]
+    for k in p1:
+        assert np.array_equal(p1[k], p2[k]), k
+    for k in ("mass", "vx", "vy", "h"):
+        assert np.array_equal(g1[k], g2[k]), k
+    assert np.array_equal(m1[0], m2[0]) and np.array_equal(m1[1], m2[1])
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_frames_and_stop_semantics(precision):
     """The run returns at every output frame with the oracle's frame and step

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, where it is the checker / the timed CPU baseline.
 The product library never loads it.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -34,15 +35,38 @@ def build(force=False):
     return LIB_PATH
 
 
+def build_mutant(n):
+    """Compile the oracle with planted mistake number n (ORC_MUTANT in
+    slab_oracle.c); returns the library path.  For the mutation tests only."""
+    path = os.path.join(_HERE, "_build", "liboracle_mut%d.so" % int(n))
+    subprocess.check_call(["make", "-s", "-C", _HERE, os.path.join("_build", os.path.basename(path))])
+    return path
+
+
 _lib = None
+_lib_path = None
+
+
+@contextlib.contextmanager
+def using_library(path):
+    """Routes every oracle call inside the block to the library at `path`
+    (a mutant build); the ordinary library is restored afterwards."""
+    global _lib, _lib_path
+    saved = (_lib, _lib_path)
+    _lib, _lib_path = None, path
+    try:
+        yield
+    finally:
+        _lib, _lib_path = saved
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = _lib_path or LIB_PATH
+        if path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         dp = ctypes.POINTER(ctypes.c_double)
         L.orc_N.restype = ctypes.c_double
         L.orc_N.argtypes = [ctypes.c_double]
@@ -105,7 +129,7 @@ class OracleSim:
         self.nx, self.ny = z.shape
         self.dx = float(dx)
         self._z = z
-        L = lib()
+        L = self._L = lib()  # the library this simulation was created on
         self.h = L.orc_create(self.nx, self.ny, self.dx, _p(z), ctypes.byref(prm))
         if not self.h:
             raise ValueError("orc_create rejected the terrain")
@@ -116,7 +140,7 @@ class OracleSim:
 
     def close(self):
         if self.h:
-            lib().orc_destroy(self.h)
+            self._L.orc_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -126,59 +150,59 @@ class OracleSim:
             pass
 
     def prime(self):
-        lib().orc_prime(self.h)
+        self._L.orc_prime(self.h)
 
     def step(self):
-        return lib().orc_step(self.h)
+        return self._L.orc_step(self.h)
 
     def run(self, max_steps=1 << 62):
-        return lib().orc_run(self.h, max_steps)
+        return self._L.orc_run(self.h, max_steps)
 
     def set_sort(self, on):
-        lib().orc_set_sort(self.h, 1 if on else 0)
+        self._L.orc_set_sort(self.h, 1 if on else 0)
 
     def set_time(self, t, frame):
-        lib().orc_set_time(self.h, t, frame)
+        self._L.orc_set_time(self.h, t, frame)
 
     def scalars(self):
         out = np.zeros(10)
-        lib().orc_scalars(self.h, _p(out))
+        self._L.orc_scalars(self.h, _p(out))
         return {"time": out[0], "dt": out[1], "frame": int(out[2]), "step": int(out[3]),
                 "phases": dict(zip(["updateDt", "remesh", "sort", "P2G", "grid", "G2P"], out[4:10]))}
 
     def window(self):
         w = (ctypes.c_int32 * 4)()
-        lib().orc_window(self.h, w)
+        self._L.orc_window(self.h, w)
         return tuple(w)
 
     def grid(self):
         iL, jB, gnx, gny = self.window()
         names = ["mass", "vx", "vy", "vz", "h", "flipx", "flipy", "Fx", "Fy"]
         arrs = {k: np.zeros((gnx, gny)) for k in names}
-        lib().orc_get_grid(self.h, *[_p(arrs[k]) for k in names])
+        self._L.orc_get_grid(self.h, *[_p(arrs[k]) for k in names])
         arrs["window"] = (iL, jB, gnx, gny)
         return arrs
 
     def particles(self):
         names = ["x", "y", "vx", "vy", "vz", "z", "h", "A", "m"]
         arrs = {k: np.zeros(self.np) for k in names}
-        lib().orc_get_particles(self.h, *[_p(arrs[k]) for k in names])
+        self._L.orc_get_particles(self.h, *[_p(arrs[k]) for k in names])
         return arrs
 
     def set_velocity(self, vx, vy):
-        lib().orc_set_velocity(self.h, _p(_f64(vx)), _p(_f64(vy)))
+        self._L.orc_set_velocity(self.h, _p(_f64(vx)), _p(_f64(vy)))
 
     def max_maps(self):
         hm = np.zeros((self.nx, self.ny))
         vm = np.zeros((self.nx, self.ny))
-        lib().orc_get_max(self.h, _p(hm), _p(vm))
+        self._L.orc_get_max(self.h, _p(hm), _p(vm))
         return hm, vm
 
     def terrain(self):
         gx = np.zeros((self.nx, self.ny))
         gy = np.zeros((self.nx, self.ny))
         th = np.zeros((self.nx, self.ny))
-        lib().orc_get_terrain(self.h, _p(gx), _p(gy), _p(th))
+        self._L.orc_get_terrain(self.h, _p(gx), _p(gy), _p(th))
         return gx, gy, th
 
 

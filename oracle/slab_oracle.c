@@ -13,6 +13,18 @@
 #include <string.h>
 #include <time.h>
 
+/* Planted mistakes, for the tests that show each closed-form check fails
+ * when the term it covers is wrong (tests/test_oracle_mutants.py builds one
+ * library per value; the ordinary build leaves ORC_MUTANT at 0):
+ *   1 strain trace without the y term        2 strain trace without dt
+ *   3 h multiplied by (1 + tr)               4 FLIP/PIC weights r and 1-r swapped
+ *   5 FLIP term dropped from the blend       6 pressure without the 1/2
+ *   7 pressure with cos instead of cos^2     8 pressure without h
+ *   9 pressure with the opposite sign */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 struct orc_sim {
     /* terrain (Grid "terrain", data_arrays.hpp:49-93) */
     int32_t nx, ny;
@@ -309,7 +321,18 @@ static void p2g(orc_sim* s) {
         const double c = cos(s->th[p]);
         const double coeff = -mp * g * c * c;
         const double mbx = coeff * Gx, mby = coeff * Gy;
+#if ORC_MUTANT == 6
+        const double s00 = -1.0 * rho * g * c * c * hp, s01 = 0.0, s10 = 0.0, s11 = s00;
+#elif ORC_MUTANT == 7
+        const double s00 = -0.5 * rho * g * c * hp, s01 = 0.0, s10 = 0.0, s11 = s00;
+#elif ORC_MUTANT == 8
+        const double s00 = -0.5 * rho * g * c * c, s01 = 0.0, s10 = 0.0, s11 = s00;
+#elif ORC_MUTANT == 9
+        const double s00 = 0.5 * rho * g * c * c * hp, s01 = 0.0, s10 = 0.0, s11 = s00;
+#else
+        /* simulation.cpp:753 */
         const double s00 = -0.5 * rho * g * c * c * hp, s01 = 0.0, s10 = 0.0, s11 = s00;
+#endif
         double wx[3], wy[3], dwx[3], dwy[3];
         weights3(xp, ib + s->iL, dx, inv, wx, dwx);
         weights3(yp, jb + s->jB, dx, inv, wy, dwy);
@@ -433,19 +456,38 @@ static void g2p(orc_sim* s) {
                 flx += s->gflx[k] * w;
                 fly += s->gfly[k] * w;
                 zp += s->gz[k] * w;
-                tr += gwx * s->gvx[k] + gwy * s->gvy[k];
+#if ORC_MUTANT == 1
+                tr += gwx * s->gvx[k];
+#else
+                tr += gwx * s->gvx[k] + gwy * s->gvy[k]; /* simulation.cpp:924-928 */
+#endif
             }
         }
+#if ORC_MUTANT != 2
         tr *= dt;
+#endif
         s->pz[p] = zp;
+#if ORC_MUTANT == 4
+        const double nvx = (1. - r) * (s->vx[p] + flx) + r * picx;
+        const double nvy = (1. - r) * (s->vy[p] + fly) + r * picy;
+#elif ORC_MUTANT == 5
+        const double nvx = r * s->vx[p] + (1. - r) * picx;
+        const double nvy = r * s->vy[p] + (1. - r) * picy;
+#else
+        /* simulation.cpp:932 */
         const double nvx = r * (s->vx[p] + flx) + (1. - r) * picx;
         const double nvy = r * (s->vy[p] + fly) + (1. - r) * picy;
+#endif
         s->vx[p] = nvx;
         s->vy[p] = nvy;
         s->vz[p] = nvx * Gx + nvy * Gy;
         s->x[p] = xp + dt * picx;
         s->y[p] = yp + dt * picy;
-        s->h[p] /= 1. + tr;
+#if ORC_MUTANT == 3
+        s->h[p] *= 1. + tr;
+#else
+        s->h[p] /= 1. + tr; /* simulation.cpp:937 */
+#endif
         s->A[p] = s->m[p] / (s->h[p] * rho);
     }
 }

@@ -19,8 +19,16 @@
  *                               brute-force dense evaluation on tiny inputs
  *   grid update                 static Coulomb, sliding-block s(t), Voellmy and
  *                               mu(I) terminal speeds
+ *   P2G pressure term           momentum balance of a frictionless dam break
+ *                               (1/2 rho g cos^2 h0^2 t per unit width; flat and
+ *                               45-degree cross-slope)
  *   G2P                         partition of unity (uniform field), mass invariance,
- *                               sum h_p A_p = sum m_p / rho
+ *                               sum h_p A_p = sum m_p / rho, height update
+ *                               h/(1+(a+b)dt) under a linear velocity field,
+ *                               FLIP/PIC weights from velocities alternating
+ *                               about an exactly reproduced mean
+ *   each of the last three      fails when its term is planted wrong
+ *                               (ORC_MUTANT builds, tests/test_oracle_mutants.py)
  *   updateDt / frame snapping   closed form
  *   remesh window               closed form (C0 t=0 window i 34..66, j 54..96)
  *   sort                        permutation + grid-sum invariance

@@ -10,7 +10,10 @@ term, a wrong sign or index, a transposed operand) fails it:
     acceleration g(sin - mu cos), static Coulomb, Voellmy terminal speed
     s^2 = xi h cos (sin - mu cos), mu(I) steady state mu(I*) = tan(theta)
   - G2P: rigid translation of a uniform block interior, mass invariance,
-    sum h_p A_p = sum m_p / rho
+    sum h_p A_p = sum m_p / rho; the height update under a linear velocity
+    field (h / (1 + (a+b) dt)); the FLIP/PIC weights (U + r delta)
+  - the depth-averaged pressure: momentum balance of a frictionless dam break
+    (flat ground and a 45-degree cross-slope)
   - updateDt / frame snapping; sort = permutation with invariant grid sums
 """
 import math
@@ -348,3 +351,122 @@ def test_sort_is_permutation_with_invariant_sums():
         assert np.allclose(pa[k], pb[k], rtol=1e-12, atol=1e-12 * np.abs(pb[k]).max())
     ga, gb = a.grid(), b.grid()
     assert abs(ga["mass"].sum() - gb["mass"].sum()) <= 1e-14 * gb["mass"].sum()
+
+
+# ---------------------------------------------------------------- G2P height update, FLIP/PIC blend, pressure
+
+def _flat_block(vel, dx=1.0, n=60, half=20.0, h0=1.0, rho=300.0, dt_max=1e-2, flip_ratio=0.95):
+    """UNIFORM 2x2 block of side 2*half on flat ground with no friction;
+    vel(px, py) -> (vx, vy) sets the initial particle velocities."""
+    from slabpro import configs
+    z = np.full((n, n), 50.0)
+    c = n * dx / 2
+    px, py, area = configs.rectangle_release(dx, c, c, 2 * half, 2 * half, 0.0)
+    part = configs.finish_particles(px, py, np.full(len(px), h0), area * h0, rho)
+    cfg = {"z": z, "dx": dx, "params": dict(rho=rho, gravity=9.81, dt_max=dt_max, output_dt=1e9, total_time=1e9,
+                                            C_cfl=0.5, flip_ratio=flip_ratio),
+           "friction": {"type": 0, "mu": 0.0}, **part}
+    s = oracle_sim(cfg)
+    vx, vy = vel(px, py)
+    s.set_velocity(vx, vy)
+    inner = (np.abs(px - c) < half - 4 * dx) & (np.abs(py - c) < half - 4 * dx)
+    return s, cfg, inner, vx, vy
+
+
+def test_height_update_follows_the_strain_trace():
+    # v = (a (x - xc), b (y - yc)) on flat frictionless ground.  The quadratic
+    # B-spline reproduces linear fields, so interior grid velocities are the
+    # same linear field, sum_i grad w_i . v_i = a + b, and G2P's height update
+    # (simulation.cpp:924-937) must give h / (1 + (a + b) dt) exactly.
+    a, b = 2.0, -0.5
+    s, cfg, inner, vx, vy = _flat_block(lambda px, py: (a * (px - 30.0), b * (py - 30.0)))
+    s.prime()
+    s.step()
+    dt = s.scalars()["dt"]
+    assert dt == 0.01  # CFL 0.5 dx / 41 m/s > dt_max
+    p = s.particles()
+    expect = 1.0 / (1.0 + (a + b) * dt)
+    assert np.max(np.abs(p["h"][inner] / expect - 1.0)) < 1e-12
+    assert np.allclose(p["A"][inner], p["m"][inner] / (p["h"][inner] * 300.0), rtol=1e-15)
+    # velocities are reproduced and positions advect with them
+    assert np.allclose(p["vx"][inner], vx[inner], rtol=0, atol=1e-12)
+    assert np.allclose(p["x"][inner], np.asarray(cfg["x"])[inner] + dt * vx[inner], rtol=0, atol=1e-12)
+
+
+def test_flip_pic_blend_weights():
+    # Velocities alternate by +-delta around a mean U with the particle's
+    # in-cell offset (0.25 vs 0.75 of a cell).  The B-spline weights of the
+    # two offsets balance at every node (also after the w > 3e-3 cutoff), so
+    # the grid velocity is exactly U, nothing accelerates (flip = 0) and
+    # simulation.cpp:932 gives v_p = r (v_p + 0) + (1 - r) U = U + r delta_p.
+    U = (1.5, -0.7)
+    dlt = (0.4, 0.3)
+    r = 0.8
+
+    def vel(px, py):
+        sx = np.where(np.isclose(np.mod(px, 1.0), 0.25), 1.0, -1.0)
+        sy = np.where(np.isclose(np.mod(py, 1.0), 0.25), 1.0, -1.0)
+        return U[0] + dlt[0] * sx, U[1] + dlt[1] * sy
+
+    s, cfg, inner, vx, vy = _flat_block(vel, flip_ratio=r, dt_max=1e-3)
+    s.prime()
+    g = s.grid()
+    act = g["mass"] > 1e-6
+    iL, jB = g["window"][:2]
+    gi = np.arange(g["mass"].shape[0])[:, None] + iL
+    gj = np.arange(g["mass"].shape[1])[None, :] + jB
+    core = act & (np.abs(gi - 30) < 16) & (np.abs(gj - 30) < 16)
+    assert np.allclose(g["vx"][core], U[0], rtol=0, atol=1e-13)
+    assert np.allclose(g["vy"][core], U[1], rtol=0, atol=1e-13)
+    s.step()
+    p = s.particles()
+    assert np.allclose(p["vx"][inner] - U[0], r * (vx[inner] - U[0]), rtol=0, atol=1e-12)
+    assert np.allclose(p["vy"][inner] - U[1], r * (vy[inner] - U[1]), rtol=0, atol=1e-12)
+    dt = s.scalars()["dt"]
+    assert np.allclose(p["x"][inner], np.asarray(cfg["x"])[inner] + dt * U[0], rtol=0, atol=1e-12)
+    assert np.allclose(p["h"][inner], 1.0, rtol=0, atol=1e-12)
+
+
+def _dam_break(theta_deg, h0, T=1.0, dt_max=2e-3, rho=300.0):
+    """Frictionless dam break across y: reservoir y in [10, 40] (dam at y = 40),
+    x in [5, 45], depth h0, on a plane tilted by theta along x.  Returns the
+    y-momentum per unit width and per rho of the particles that start beyond
+    the section y = 25 in the columns x in [12, 38], at times 0.5 and T."""
+    from slabpro import configs
+    dx = 1.0
+    t = math.tan(math.radians(theta_deg))
+    nx, ny = 56, 60
+    z = np.repeat((-t * np.arange(nx) * dx)[:, None], ny, axis=1) + 100.0
+    px, py = configs.uniform_samples(dx, 5.0, 45.0 - 1e-9, 10.0, 40.0 - 1e-9)
+    part = configs.finish_particles(px, py, np.full(len(px), h0), 40.0 * 30.0 * h0, rho)
+    cfg = {"z": z, "dx": dx, "params": dict(rho=rho, gravity=9.81, dt_max=dt_max, output_dt=1e9, total_time=1e9,
+                                            C_cfl=0.5, flip_ratio=0.95),
+           "friction": {"type": 0, "mu": 0.0}, **part}
+    s = oracle_sim(cfg)
+    s.prime()
+    sel = (px >= 12.0) & (px <= 38.0) & (py > 25.0)
+    width = 26.0
+    out = []
+    for t_end in (0.5, T):
+        while s.scalars()["time"] < t_end - 1e-9:
+            s.step()
+        p = s.particles()
+        out.append((s.scalars()["time"], (p["m"][sel] * p["vy"][sel]).sum() / rho / width))
+    return out
+
+
+@pytest.mark.parametrize("theta_deg,tol", [(0.0, 0.01), (45.0, 0.03)])
+def test_pressure_term_momentum_balance_of_a_dam_break(theta_deg, tol):
+    # Shallow-water momentum balance behind a dam break (Ritter's problem):
+    # until the rarefaction from the dam reaches a section inside the resting
+    # reservoir, the only force on the fluid beyond that section is the
+    # hydrostatic thrust there, 1/2 rho g cos^2(theta) h0^2 per unit width
+    # (the depth-averaged pressure sigma = -1/2 rho g cos^2 h of
+    # simulation.cpp:753 and its divergence at :792).  So the fluid beyond it
+    # gains y-momentum 1/2 g cos^2 h0^2 t per unit width and per rho.  On the
+    # tilted plane the fluid also slides along x, which does not enter y.
+    h0 = 2.0
+    c2 = math.cos(math.radians(theta_deg)) ** 2
+    for t, P in _dam_break(theta_deg, h0):
+        expect = 0.5 * 9.81 * c2 * h0 * h0 * t
+        assert abs(P / expect - 1.0) < tol, (t, P, expect)

@@ -109,16 +109,22 @@ def cpu_baseline(cfg, steps, warm):
 
 
 def copy_probe():
-    """A device copy of known size: FETCH_SIZE / WRITE_SIZE per dispatch of its
-    kernel give the factors that turn those counters into HBM bytes."""
-    import torch
-    n = 1 << 28  # 256 Mi float32 = 1 GiB
-    a = torch.ones(n, device="cuda", dtype=torch.float32)
-    b = torch.empty_like(a)
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    log(f"copied 5 x {a.numel() * 4} bytes")
+    """Streaming kernels of known size written for gfx950 (slab_stream_probe):
+    the practical HBM ceiling for a read+write copy, a read, a write, and
+    G2P's particle streams at the 8M-particle workload's count.  Their
+    FETCH_SIZE / WRITE_SIZE per dispatch also give the factors that turn those
+    counters into HBM bytes.  Prints one JSON line."""
+    from slabpro import abi
+    gib = 1 << 30
+    out = {}
+    for name, kind, n, nbytes in (("copy_1GiB", abi.PROBE_COPY, gib, 2 * gib),
+                                  ("read_1GiB", abi.PROBE_READ, gib, gib),
+                                  ("write_1GiB", abi.PROBE_WRITE, gib, gib),
+                                  ("particles_8147024", abi.PROBE_PARTICLES, 8147024, 60 * 8147024)):
+        ms = abi.stream_probe(kind, n, 20)
+        out[name] = {"ms": ms, "bytes": nbytes, "GB_s": nbytes / ms / 1e6, "frac_of_8TBs": nbytes / ms / 1e6 / 8000.0}
+        log(f"{name}: {ms:.4f} ms, {nbytes / ms / 1e6:.0f} GB/s")
+    print(json.dumps({"stream_probe": out}), flush=True)
 
 
 def main():

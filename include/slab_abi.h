@@ -152,6 +152,14 @@ void slab_run_destroy(slab_run* run);
 /* Library / device facts: out[0]=device id, out[1]=CU count, out[2]=is gfx950. */
 slab_status slab_device_info(int32_t out[3]);
 
+/* Streaming kernels of known size on the current device, the practical HBM
+ * ceiling for the step's access patterns.  kind 0: copy n bytes (float4 reads
+ * and writes); 1: read n bytes; 2: write n bytes; 3: the particle streams of
+ * G2P for n particles (28 B read, 32 B written in place, one per thread).
+ * *ms = mean kernel time of iters launches (HIP events).  No reference
+ * counterpart: measurement only. */
+slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters, double* ms);
+
 /* ---------------------------------------------------------------------------
  * Batches: many scenarios on one shared DEM, advanced together (FP32 form).
  * No reference counterpart -- the reference runs one scenario per process
@@ -183,7 +191,12 @@ slab_status slab_batch_admit(slab_batch* b, const slab_params* params, const sla
                              const double* x, const double* y, const double* vx, const double* vy, const double* h,
                              const double* A, const double* m, int32_t* slot);
 /* Runs up to max_steps batch steps (every active slot advances one of its own
- * steps per batch step); returns early when no slot is active. */
+ * steps per batch step); returns early when no slot is active.
+ * SLAB_EDOMAIN when the particle tiles of the active slots exceed the batch's
+ * active-tile capacity: that step is left half started, so from then on
+ * slab_batch_step, slab_batch_admit and slab_batch_release return SLAB_ESTATE
+ * (downloads still read the last completed step); destroy the batch.  The
+ * same holds for slab_run_step of an FP32 run. */
 slab_status slab_batch_step(slab_batch* b, int32_t max_steps, int32_t* n_active);
 slab_status slab_batch_slot_info(slab_batch* b, int32_t slot, slab_slot_info* out);
 /* status[max_slots] and time[max_slots] of every slot (either may be NULL). */
@@ -191,7 +204,10 @@ slab_status slab_batch_statuses(slab_batch* b, int32_t* status, double* time);
 slab_status slab_batch_download_max(slab_batch* b, int32_t slot, double* h_max, double* v_max2);
 slab_status slab_batch_download_particles(slab_batch* b, int32_t slot, double* x, double* y, double* vx, double* vy,
                                           double* vz, double* z, double* h, double* A, double* m);
-/* Frees a slot; its particles leave the device state at the next step. */
+/* Frees a slot at once, whatever its status (a running scenario is
+ * cancelled): its particles are dropped from the device state by one
+ * compaction pass inside this call, so their capacity and the slot are free
+ * for the next admission.  The other slots are not advanced. */
 slab_status slab_batch_release(slab_batch* b, int32_t slot);
 /* out[0] = device bytes per particle, out[1] = device bytes per slot,
  * out[2] = free particle capacity, out[3] = free slots (admission by footprint). */

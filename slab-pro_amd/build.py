@@ -33,6 +33,7 @@ SOURCES = {
     "slab_kernels.hip": ["-ffp-contract=off"],
     "slab_engine32.hip": ["-ffp-contract=fast"],
     "slab_abi.hip": ["-ffp-contract=off"],
+    "slab_probe.hip": [],
 }
 
 COMMON = [

@@ -298,7 +298,13 @@ Slot slot_params(const slab_params* p, const slab_friction* f) {
     return S;
 }
 
-slab_status eng_fail(const std::string& msg, slab_status st = SLAB_EDEVICE) { return fail(st, msg); }
+// Engine failures: an overflow of the active tiles is SLAB_EDOMAIN, every
+// later call on that engine SLAB_ESTATE, anything else a device error.
+slab_status eng_fail(const std::string& msg, slab_status st = SLAB_EDEVICE) {
+    if (msg == kStoppedByError) st = SLAB_ESTATE;
+    else if (msg == "active-tile capacity exceeded") st = SLAB_EDOMAIN;
+    return fail(st, msg);
+}
 
 // Non-finite inputs are rejected before they reach the device.
 bool all_finite(const double* a, int64_t n) {
@@ -874,6 +880,7 @@ slab_status slab_batch_admit(slab_batch* b, const slab_params* params, const sla
     const int s = engine_admit(b->eng, slot_params(params, friction), np, x, y, vx, vy, nullptr, h, A, m, &why);
     if (s < 0) {
         *slot = -1;
+        if (why == kStoppedByError) return fail(SLAB_ESTATE, why);
         return fail((why == "no free slot" || why == "particle capacity exhausted") ? SLAB_ENOMEM : SLAB_EDEVICE, why);
     }
     *slot = s;
@@ -884,8 +891,10 @@ slab_status slab_batch_step(slab_batch* b, int32_t max_steps, int32_t* n_active)
     if (!b) return fail(SLAB_EINVAL, "null batch");
     if (max_steps < 0) return fail(SLAB_EINVAL, "max_steps < 0");
     std::string why;
-    if (!engine_step(b->eng, max_steps, &why))
+    if (!engine_step(b->eng, max_steps, &why)) {
+        if (why == kStoppedByError) return fail(SLAB_ESTATE, why);
         return fail(why.find("capacity") != std::string::npos ? SLAB_EDOMAIN : SLAB_EDEVICE, why);
+    }
     if (n_active) {
         int a = 0;
         for (int s = 0; s < b->nslots; ++s) {
@@ -941,7 +950,8 @@ slab_status slab_batch_download_particles(slab_batch* b, int32_t slot, double* x
 slab_status slab_batch_release(slab_batch* b, int32_t slot) {
     if (!b) return fail(SLAB_EINVAL, "null batch");
     std::string why;
-    return engine_release(b->eng, slot, &why) ? SLAB_OK : fail(SLAB_EINVAL, why);
+    if (engine_release(b->eng, slot, &why)) return SLAB_OK;
+    return fail(why == kStoppedByError ? SLAB_ESTATE : SLAB_EINVAL, why);
 }
 
 slab_status slab_batch_footprint(slab_batch* b, int64_t out[4]) {

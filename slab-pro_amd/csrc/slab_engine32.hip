@@ -1667,6 +1667,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     if (const char* v = std::getenv("SLAB_SCAN_MERGE_MAX_BLOCKS")) e->scan_merge_max = std::atoi(v);
     e->cap = max_particles;
     e->tile_cap = (int)std::min<int64_t>(e->ntiles, std::max<int64_t>(65536, max_particles / 32));
+    // a smaller active-tile capacity (tests of the overflow path)
+    if (const char* v = std::getenv("SLAB_TILE_CAP")) e->tile_cap = std::max(1, std::min(e->tile_cap, std::atoi(v)));
     std::string why;
     auto fail = [&](const std::string& m) -> Engine32* {
         if (err) *err = m;
@@ -1779,10 +1781,21 @@ const Slot& engine_slot(const Engine32* e, int s) { return e->h_slots[s]; }
 
 int64_t engine_free_particles(const Engine32* e) { return e->cap - e->h_c->n_store; }
 
+// A step that stopped on an error (the active tiles exceeded the tile
+// capacity) leaves a step half started (slot dt and step counters advanced,
+// its end pending, tile lists unwritten): the engine accepts no further step,
+// admission or release; downloads still read the last completed step.
+bool engine_stopped(const Engine32* e, std::string* err) {
+    if (e->h_c->stop != STOP_ERROR) return false;
+    if (err) *err = kStoppedByError;
+    return true;
+}
+
 int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, const double* y, const double* vx,
                  const double* vy, const double* vz, const double* h, const double* A, const double* m,
                  std::string* err) {
     if (!engine_pull(e, err)) return -1;
+    if (engine_stopped(e, err)) return -1;
     int slot = -1;
     for (int s = 0; s < e->nslots; ++s)
         if (e->h_slots[s].status == SLOT_FREE) {
@@ -1874,6 +1887,7 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
 }
 
 bool engine_step(Engine32* e, int max_steps, std::string* err) {
+    if (engine_stopped(e, err)) return false;
     hipStream_t s = e->stream;
     size_particle_grids(e);
     k_eclear<<<1, WG, 0, s>>>(e->d_c, e->d_slots);
@@ -1906,6 +1920,7 @@ bool engine_step(Engine32* e, int max_steps, std::string* err) {
 
 bool engine_release(Engine32* e, int slot, std::string* err) {
     if (!engine_pull(e, err)) return false;
+    if (engine_stopped(e, err)) return false;
     if (slot < 0 || slot >= e->nslots) {
         if (err) *err = "bad slot";
         return false;

@@ -64,6 +64,9 @@ struct ECtl {
 
 struct Engine32;
 
+// Message of every call refused after a step stopped on an error.
+constexpr const char* kStoppedByError = "stopped by an earlier error (active-tile capacity exceeded); destroy it";
+
 // Host-side engine API used by the C ABI (slab_run_* with SLAB_FP32, slab_batch_*).
 struct EngineResult {
     int ok;

@@ -84,6 +84,7 @@ SIGNATURES = {
     "slab_last_error": (ctypes.c_char_p, []),
     "slab_run_destroy": (None, [_VP]),
     "slab_device_info": (ctypes.c_int, [_IP]),
+    "slab_stream_probe": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _DP]),
     "slab_batch_create": (ctypes.c_int, [ctypes.POINTER(Terrain), ctypes.c_int32, ctypes.c_int64,
                                          ctypes.POINTER(ctypes.c_void_p)]),
     "slab_batch_admit": (ctypes.c_int, [_VP, ctypes.POINTER(Params), ctypes.POINTER(Friction), ctypes.c_int64]
@@ -167,6 +168,18 @@ def device_info():
     out = (ctypes.c_int32 * 3)()
     _check(lib().slab_device_info(out))
     return {"device": out[0], "cus": out[1], "gfx950": bool(out[2])}
+
+
+PROBE_COPY, PROBE_READ, PROBE_WRITE, PROBE_PARTICLES = range(4)
+
+
+def stream_probe(kind, n, iters=20):
+    """Mean kernel time (ms) of a streaming kernel of known size: PROBE_COPY /
+    _READ / _WRITE over n bytes, PROBE_PARTICLES over n particles (28 B read,
+    32 B written in place)."""
+    ms = ctypes.c_double(0.0)
+    _check(lib().slab_stream_probe(int(kind), int(n), int(iters), ctypes.byref(ms)))
+    return ms.value
 
 
 def kernel_names():

@@ -85,12 +85,14 @@ def run_batched(dem, dx, scenarios, max_slots=256, capacity=None, steps_per_call
             "particle_updates_per_s": pupd / wall, "results": done}
 
 
-def run_one_at_a_time(dem, dx, scenarios, precision=abi.FP64, progress=None):
+def run_one_at_a_time(dem, dx, scenarios, precision=abi.FP64, progress=None, results=False):
     """The plain path: each scenario its own run (create, upload, prime, run to
     total_time, download max maps), one after another.  progress(msg) is
-    called after each scenario."""
+    called after each scenario; with results, the max maps (h_max, v_max2) of
+    each scenario are returned in "results"."""
     t0 = time.perf_counter()
     per = []
+    maps = []
     pupd = 0
     for s in scenarios:
         t1 = time.perf_counter()
@@ -98,7 +100,9 @@ def run_one_at_a_time(dem, dx, scenarios, precision=abi.FP64, progress=None):
         run.upload(s["x"], s["y"], s["h"], s["A"], s["m"])
         run.prime()
         _, info = run.run()
-        run.max_maps()
+        mm = run.max_maps()
+        if results:
+            maps.append(mm)
         run.close()
         pupd += len(s["x"]) * info["step"]
         per.append(time.perf_counter() - t1)
@@ -108,15 +112,24 @@ def run_one_at_a_time(dem, dx, scenarios, precision=abi.FP64, progress=None):
     wall = time.perf_counter() - t0
     return {"wall_s": wall, "scenarios": len(scenarios), "per_scenario_s": per,
             "scenarios_per_hour": len(scenarios) / wall * 3600.0, "particle_updates": pupd,
-            "particle_updates_per_s": pupd / wall}
+            "particle_updates_per_s": pupd / wall, "results": maps}
 
 
 def serve(dem, dx, requests, arrivals, max_slots=64, capacity=8_000_000, steps_per_call=8, graph=True,
-          progress=None):
+          progress=None, collect=None):
     """Continuous batching.  requests[k] (a scenario dict) arrives at wall
     time arrivals[k] (s, open loop).  Each loop: admit arrived requests in FIFO
     order while a slot and particle capacity are free; run a few batch steps;
-    collect and release finished slots.  Latency = completion - arrival."""
+    collect and release finished slots.  Latency = completion - arrival.
+
+    A request that an empty batch cannot hold (more particles than the
+    capacity) is rejected: it is counted in "rejected", gets no latency, and
+    the queue moves on.  collect(k, batch, slot), if given, is called for each
+    finished request before its slot is released; its return value is kept in
+    "results"[k].  "co_running"[k] is the number of requests that were on the
+    device when request k was admitted (-1 if it never was); "admit_call"[k]
+    and "finish_call"[k] count the batch step calls made before request k was
+    admitted and before it was seen finished."""
     b = abi.Batch(dem, dx, max_slots, capacity)
     b.use_graph(graph)
     n = len(requests)
@@ -125,6 +138,12 @@ def serve(dem, dx, requests, arrivals, max_slots=64, capacity=8_000_000, steps_p
     queue = collections.deque()
     slot_of = {}
     lat = np.full(n, np.nan)
+    co_running = np.full(n, -1)
+    admit_call = np.full(n, -1)
+    finish_call = np.full(n, -1)
+    calls = 0
+    results = {}
+    rejected = []
     busy_particles = []
     t0 = time.perf_counter()
     last_report = t0
@@ -138,16 +157,26 @@ def serve(dem, dx, requests, arrivals, max_slots=64, capacity=8_000_000, steps_p
             s = requests[k]
             slot = b.admit(_params(s, abi.FP32), _fric(s), s["x"], s["y"], s["h"], s["A"], s["m"])
             if slot is None:
+                if not slot_of:  # refused by an empty batch: it can never fit
+                    rejected.append(int(k))
+                    queue.popleft()
+                    continue
                 break
+            co_running[k] = len(slot_of)
+            admit_call[k] = calls
             slot_of[slot] = k
             queue.popleft()
         if slot_of:
             b.step(steps_per_call)
+            calls += 1
             st, _ = b.statuses()
             fin = time.perf_counter() - t0
             for slot, k in list(slot_of.items()):
                 if st[slot] in (abi.SLOT_DONE, abi.SLOT_FAILED):
                     lat[k] = fin - arrivals[k]
+                    finish_call[k] = calls
+                    if collect is not None:
+                        results[int(k)] = collect(k, b, slot)
                     b.release(slot)
                     del slot_of[slot]
             busy_particles.append(sum(len(requests[k]["x"]) for k in slot_of.values()))
@@ -158,7 +187,12 @@ def serve(dem, dx, requests, arrivals, max_slots=64, capacity=8_000_000, steps_p
             progress(f"serve: {int(np.sum(np.isfinite(lat)))}/{n} done, queue {len(queue)}, active {len(slot_of)}")
     wall = time.perf_counter() - t0
     b.close()
-    return {"wall_s": wall, "requests": n, "p50_s": float(np.nanpercentile(lat, 50)),
-            "p95_s": float(np.nanpercentile(lat, 95)), "p99_s": float(np.nanpercentile(lat, 99)),
-            "mean_s": float(np.nanmean(lat)), "scenarios_per_hour": n / wall * 3600.0,
+    served = int(np.sum(np.isfinite(lat)))
+    pct = (lambda q: float(np.nanpercentile(lat, q))) if served else (lambda q: float("nan"))
+    return {"wall_s": wall, "requests": n, "served": served, "rejected": rejected,
+            "p50_s": pct(50), "p95_s": pct(95), "p99_s": pct(99),
+            "mean_s": float(np.nanmean(lat)) if served else float("nan"),
+            "scenarios_per_hour": served / wall * 3600.0, "latency_s": lat, "co_running": co_running,
+            "admit_call": admit_call, "finish_call": finish_call,
+            "results": results,
             "mean_active_particles": float(np.mean(busy_particles)) if busy_particles else 0.0}

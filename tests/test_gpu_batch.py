@@ -283,3 +283,38 @@ def test_dense_tiles_sort_canonically(n_per_axis, cells, corner):
     r.close()
     assert np.max(np.abs(pa["x"] - q["x"])) < 1e-3
     assert np.max(np.abs(pa["vx"] - q["vx"])) < 1e-2 * max(1.0, np.abs(q["vx"]).max())
+
+
+def test_active_tile_overflow_stops_the_batch(monkeypatch):
+    # An active-tile capacity of one tile (SLAB_TILE_CAP, read when the batch
+    # is created) against a release that spans several tiles: the step is
+    # refused with SLAB_EDOMAIN, after which the half-started step must not be
+    # resumed -- step, admit and release are refused with SLAB_ESTATE -- while
+    # downloads still return the last completed state.
+    from slabpro import abi
+    z, dx = _terrain()
+    part, fr = _scenario(51, FRICS[0], n_extra=10)
+    cells = {(int(x // dx) // 16, int(y // dx) // 16) for x, y in zip(part["x"], part["y"])}
+    assert len(cells) >= 2
+    monkeypatch.setenv("SLAB_TILE_CAP", "1")
+    b = abi.Batch(z, dx, 2, 100000)
+    monkeypatch.delenv("SLAB_TILE_CAP")
+    s = b.admit(_params(), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
+    assert s is not None
+    with pytest.raises(abi.SlabError) as ei:
+        b.step(4)
+    assert ei.value.status == abi.EDOMAIN
+    with pytest.raises(abi.SlabError) as ei:
+        b.step(1)
+    assert ei.value.status == abi.ESTATE
+    with pytest.raises(abi.SlabError) as ei:
+        b.admit(_params(), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
+    assert ei.value.status == abi.ESTATE
+    with pytest.raises(abi.SlabError) as ei:
+        b.release(s)
+    assert ei.value.status == abi.ESTATE
+    p = b.particles(s)
+    assert np.max(np.abs(p["x"] - part["x"])) < 1e-3 * dx  # the admitted state (FP32 offsets)
+    hm, _ = b.max_maps(s)
+    assert np.all(hm == 0.0)  # no step completed
+    b.close()

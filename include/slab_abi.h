@@ -204,6 +204,20 @@ slab_status slab_batch_statuses(slab_batch* b, int32_t* status, double* time);
 slab_status slab_batch_download_max(slab_batch* b, int32_t slot, double* h_max, double* v_max2);
 slab_status slab_batch_download_particles(slab_batch* b, int32_t slot, double* x, double* y, double* vx, double* vy,
                                           double* vz, double* z, double* h, double* A, double* m);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, code between graph replays, harnesses that forbid
+ * hipMalloc): the same outputs as slab_batch_download_particles, staged
+ * through a fixed scratch array of the library (a counter plus 9 FP64 fields
+ * of SLAB_NOALLOC_MAX_PARTICLES particles, about 18.9 MB, held by every GPU the
+ * library runs on until the process ends) instead of per-call allocations;
+ * no hipMalloc / hipFree is made.  SLAB_EDOMAIN for a slot of more than
+ * SLAB_NOALLOC_MAX_PARTICLES particles.  All calls on one GPU share that GPU's
+ * scratch and each overwrites it: calls that overlap on one GPU (different
+ * host threads or batches) race. */
+#define SLAB_NOALLOC_MAX_PARTICLES 262144
+slab_status slab_batch_download_particles_noalloc(slab_batch* b, int32_t slot, double* x, double* y, double* vx,
+                                                  double* vy, double* vz, double* z, double* h, double* A,
+                                                  double* m);
 /* Frees a slot at once, whatever its status (a running scenario is
  * cancelled): its particles are dropped from the device state by one
  * compaction pass inside this call, so their capacity and the slot are free

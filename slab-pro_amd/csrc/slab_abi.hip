@@ -947,6 +947,17 @@ slab_status slab_batch_download_particles(slab_batch* b, int32_t slot, double* x
     return engine_particles(b->eng, slot, dst, &why) ? SLAB_OK : eng_fail(why);
 }
 
+slab_status slab_batch_download_particles_noalloc(slab_batch* b, int32_t slot, double* x, double* y, double* vx,
+                                                  double* vy, double* vz, double* z, double* h, double* A,
+                                                  double* m) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    if (slot < 0 || slot >= b->nslots) return fail(SLAB_EINVAL, "bad slot");
+    double* const dst[9] = {x, y, vx, vy, vz, z, h, A, m};
+    std::string why;
+    if (engine_particles_noalloc(b->eng, slot, dst, &why)) return SLAB_OK;
+    return fail(why.find("the download without allocation") != std::string::npos ? SLAB_EDOMAIN : SLAB_EDEVICE, why);
+}
+
 slab_status slab_batch_release(slab_batch* b, int32_t slot) {
     if (!b) return fail(SLAB_EINVAL, "null batch");
     std::string why;

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <thread>
 
 namespace slab {
@@ -1288,6 +1289,50 @@ __global__ void k_ecollect(ECtl* c, PB32 A, PB32 B, int slot, int* __restrict__ 
     f[7] = P.m[s];
 }
 
+// --- the download that allocates nothing --------------------------------------
+
+// Fixed scratch of slab_batch_download_particles_noalloc: a counter, then the
+// nine FP64 fields of at most NOALLOC_MAX particles in creation order.  Each
+// GPU the library runs on holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (scratch_address), never through the name.
+constexpr int NOALLOC_MAX = SLAB_NOALLOC_MAX_PARTICLES;
+constexpr size_t NOALLOC_HEAD = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_HEAD + 9 * (size_t)NOALLOC_MAX * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+__global__ void k_noalloc_clear(int* __restrict__ counter) {
+    if (threadIdx.x == 0) *counter = 0;
+}
+
+// One slot's particles of the current buffer as the FP64 fields of
+// slab_batch_download_particles, written at their creation index:
+// out[f * NOALLOC_MAX + pid] for f = x, y, vx, vy, vz, z, h, A, m (the same
+// operations as the host side of engine_particles, so the same values).
+__global__ void __launch_bounds__(WG) k_ecollect_pid(const ECtl* c, PB32 A, PB32 B, int slot, int n, double rho,
+                                                     int* __restrict__ counter, double* __restrict__ out) {
+    const PB32 P = ecur(c, A, B);
+    const int s = blockIdx.x * WG + threadIdx.x;
+    if (s >= c->n_store) return;
+    if ((P.slot[s] & SLOT_MASK) != slot) return;
+    const int q = P.pid[s];
+    if (q < 0 || q >= n) return;
+    atomicAdd(counter, 1);
+    const int cell = P.cell[s];
+    const float2 po = P.o[s], pv = P.v[s];
+    const float hp = P.h[s], mp = P.m[s];
+    const double dx = c->dx;
+    out[q] = ((double)(cell >> 16) + (double)po.x) * dx;
+    out[(size_t)NOALLOC_MAX + q] = ((double)(cell & 0xffff) + (double)po.y) * dx;
+    out[2 * (size_t)NOALLOC_MAX + q] = pv.x;
+    out[3 * (size_t)NOALLOC_MAX + q] = pv.y;
+    out[4 * (size_t)NOALLOC_MAX + q] = P.vz[s];
+    out[5 * (size_t)NOALLOC_MAX + q] = P.z[s];
+    out[6 * (size_t)NOALLOC_MAX + q] = hp;
+    out[7 * (size_t)NOALLOC_MAX + q] = (double)mp / ((double)hp * rho);
+    out[8 * (size_t)NOALLOC_MAX + q] = mp;
+}
+
 // Bounding box of one slot's particles in the buffer its last P2G read
 // (for the reference's remesh window, src/simulation.cpp:594-606).
 __global__ void k_ebbox(ECtl* c, PB32 A, PB32 B, int slot, int n, unsigned long long* __restrict__ bb) {
@@ -1444,6 +1489,8 @@ struct Engine32 {
     int persist = kPersistE; // workgroups of the tile-list kernels (sort + P2G, grid)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    int device = 0;
+    unsigned char* scratch_ptr = nullptr; // this device's address of `scratch`
 };
 
 namespace {
@@ -1636,6 +1683,31 @@ int64_t engine_slot_bytes(const Engine32* e) {
     return (int64_t)e->nn * (16 + 8 + (e->single ? 12 : 0)) + VMAX_SHARDS * 8 + (int64_t)sizeof(Slot);
 }
 
+// Device address of `scratch` on each device, taken once per device with that
+// device current.
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_addr[64] = {};
+
+bool scratch_address(int dev, unsigned char** out, std::string* err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= 64) {
+        if (err) *err = "device id out of range for the download scratch";
+        return false;
+    }
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (r != hipSuccess || !p) {
+            (void)hipGetLastError();
+            if (err) *err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(r);
+            return false;
+        }
+        g_scratch_addr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_addr[dev];
+    return true;
+}
+
 Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, int64_t max_particles, bool single,
                         std::string* err) {
     if (nx < 4 || ny < 4 || nx > 65535 || ny > 65535) {
@@ -1676,6 +1748,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
         return nullptr;
     };
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed");
+    if (hipGetDevice(&e->device) != hipSuccess) return fail("hipGetDevice failed");
+    if (!scratch_address(e->device, &e->scratch_ptr, &why)) return fail(why);
     if (hipHostMalloc((void**)&e->h_c, sizeof(ECtl), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&e->h_slots, sizeof(Slot) * nslots, hipHostMallocDefault) != hipSuccess)
         return fail("hipHostMalloc failed");
@@ -1998,6 +2072,37 @@ bool engine_particles(Engine32* e, int slot, double* const dst[9], std::string* 
         if (dst[6]) dst[6][q] = g[6];
         if (dst[7]) dst[7][q] = (double)g[7] / ((double)g[6] * rho);
         if (dst[8]) dst[8][q] = g[7];
+    }
+    return true;
+}
+
+bool engine_particles_noalloc(Engine32* e, int slot, double* const dst[9], std::string* err) {
+    if (!engine_pull(e, err)) return false;
+    const int n = e->h_slots[slot].np;
+    if (n > NOALLOC_MAX) {
+        if (err)
+            *err = "slot holds " + std::to_string(n) + " particles, more than the " + std::to_string(NOALLOC_MAX) +
+                   " the download without allocation stages";
+        return false;
+    }
+    if (n <= 0) return true;
+    hipStream_t s = e->stream;
+    int* counter = reinterpret_cast<int*>(e->scratch_ptr);
+    double* out = reinterpret_cast<double*>(e->scratch_ptr + NOALLOC_HEAD);
+    k_noalloc_clear<<<1, 64, 0, s>>>(counter);
+    const unsigned nb = (unsigned)std::max(1, (e->h_c->n_store + WG - 1) / WG);
+    k_ecollect_pid<<<nb, WG, 0, s>>>(e->d_c, e->A, e->B, slot, n, e->h_slots[slot].rho, counter, out);
+    int got = -1;
+    bool ok = hipGetLastError() == hipSuccess &&
+              hipMemcpyAsync(&got, counter, sizeof(int), hipMemcpyDeviceToHost, s) == hipSuccess;
+    for (int k = 0; k < 9; ++k)
+        if (dst[k])
+            ok = ok && hipMemcpyAsync(dst[k], out + (size_t)k * NOALLOC_MAX, (size_t)n * sizeof(double),
+                                      hipMemcpyDeviceToHost, s) == hipSuccess;
+    ok = ok && hipStreamSynchronize(s) == hipSuccess;
+    if (!ok || got != n) {
+        if (err) *err = ok ? "particle count mismatch while collecting a slot" : "collect failed";
+        return false;
     }
     return true;
 }

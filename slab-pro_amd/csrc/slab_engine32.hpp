@@ -8,6 +8,7 @@
 // shared sort / scan machinery orders particles by (slot, tile, cell,
 // creation index) and one P2G / grid / G2P launch covers every slot.
 #pragma once
+#include "slab_abi.h"
 #include "slab_common.hpp"
 
 #include <string>
@@ -87,6 +88,9 @@ const ECtl& engine_ctl(const Engine32* e);
 const Slot& engine_slot(const Engine32* e, int s);
 bool engine_release(Engine32* e, int slot, std::string* err); // frees a slot (its particles drop out)
 bool engine_particles(Engine32* e, int slot, double* const dst[9], std::string* err);
+// The same download staged through the library's fixed per-device scratch
+// instead of allocations (slots of at most SLAB_NOALLOC_MAX_PARTICLES).
+bool engine_particles_noalloc(Engine32* e, int slot, double* const dst[9], std::string* err);
 bool engine_max_maps(Engine32* e, int slot, double* h_max, double* v_max2, std::string* err);
 bool engine_window(Engine32* e, int slot, int* i0, int* j0, int* wnx, int* wny, double* const dst[5],
                    std::string* err);

@@ -94,6 +94,7 @@ SIGNATURES = {
     "slab_batch_statuses": (ctypes.c_int, [_VP, _IP, _DP]),
     "slab_batch_download_max": (ctypes.c_int, [_VP, ctypes.c_int32, _DP, _DP]),
     "slab_batch_download_particles": (ctypes.c_int, [_VP, ctypes.c_int32] + [_DP] * 9),
+    "slab_batch_download_particles_noalloc": (ctypes.c_int, [_VP, ctypes.c_int32] + [_DP] * 9),
     "slab_batch_release": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "slab_batch_footprint": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int64)]),
     "slab_batch_use_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
@@ -372,6 +373,14 @@ class Batch:
         names = ["x", "y", "vx", "vy", "vz", "z", "h", "A", "m"]
         arrs = {k: np.zeros(n) for k in names}
         _check(lib().slab_batch_download_particles(self.h, int(slot), *[_p(arrs[k]) for k in names]))
+        return arrs
+
+    def particles_noalloc(self, slot):
+        """As particles(), through the library's fixed scratch (no device allocation)."""
+        n = self.slot_info(slot)["np"]
+        names = ["x", "y", "vx", "vy", "vz", "z", "h", "A", "m"]
+        arrs = {k: np.zeros(n) for k in names}
+        _check(lib().slab_batch_download_particles_noalloc(self.h, int(slot), *[_p(arrs[k]) for k in names]))
         return arrs
 
     def release(self, slot):

@@ -13,9 +13,14 @@ Prints ONE JSON line.  roofline: dominant kernel's algorithmic bytes per launch
 / its mean HIP-event duration vs the 8 TB/s HBM peak; roofline.traffic: that
 kernel's HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
 WRITE_SIZE summary of the same workload (profiles/step_<form>_<workload>_*.md),
-null when there is none.  cpu_baseline: the FP64 CPU oracle (single-threaded
-restatement of the reference step) timed on this host on a bounded number of
-steps of the same workload.
+null when there is none.  stream_ceiling: GB/s of hand-written streaming
+kernels on the same GPU (the practical ceiling; frac_of_copy_ceiling reads the
+dominant kernel against it).  cpu_baseline: the FP64 CPU oracle
+(single-threaded restatement of the reference step) timed on this host, 10
+untimed + 50 timed steps from the t = 0 prime and from the run's state at
+t = 60 s, with the host's core count and CPU model.  ensemble: the
+256-scenario ensemble (FP32 batched scenarios/h, the FP64 one-at-a-time rate
+on a stratified sample and each sampled scenario's max-height L1 error).
 """
 import argparse
 import json
@@ -86,26 +91,124 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg, steps, warm):
-    """Time the single-threaded FP64 oracle on `steps` steps of the workload."""
+def host_facts():
+    """nproc, usable cores and the CPU model of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "cpu_model": model}
+
+
+def cpu_baseline(cfg, steps, warm, state=None, time0=0.0, frame0=0):
+    """Time the single-threaded FP64 oracle on `steps` steps (after `warm`
+    untimed ones) of the workload: from the t = 0 prime, or from `state`
+    (particle arrays x, y, vx, vy, h, A, m of a run at time0)."""
     from oracle import oracle as orc
     t0 = time.time()
     pr = dict(cfg["params"])
     fr = dict(cfg["friction"])
     typ = fr.pop("type")
-    s = orc.OracleSim(cfg["z"], cfg["dx"], orc.params(fric_type=typ, **pr, **fr), cfg["x"], cfg["y"], cfg["h"],
-                      cfg["A"], cfg["m"])
-    s.prime()
+    src = state if state is not None else cfg
+    s = orc.OracleSim(cfg["z"], cfg["dx"], orc.params(fric_type=typ, **pr, **fr), src["x"], src["y"], src["h"],
+                      src["A"], src["m"])
+    if state is None:
+        s.prime()
+    else:
+        s.set_velocity(state["vx"], state["vy"])
+        s.set_time(time0, frame0)
     for _ in range(warm):
         s.step()
     t1 = time.time()
     for _ in range(steps):
         s.step()
     el = time.time() - t1
-    np_ = len(cfg["x"])
-    log(f"cpu oracle: setup {t1 - t0:.1f}s, {steps} steps {el:.2f}s; phases {s.scalars()['phases']}")
+    np_ = len(src["x"])
+    log(f"cpu oracle ({'t=0' if state is None else f't={time0:g} s'}): setup {t1 - t0:.1f}s, {warm}+{steps} steps, "
+        f"timed {el:.2f}s; phases {s.scalars()['phases']}")
     s.close()
     return np_ * steps / el, el
+
+
+def mid_run_state(cfg, t_mid):
+    """The workload's particle state at simulated time t_mid, from the FP64
+    form on the GPU (the state the CPU baseline's mid-run sample starts from)."""
+    import slabpro
+    from slabpro import abi
+    r = slabpro.make_run(cfg, precision=abi.FP64)
+    r.prime()
+    info = None
+    while True:
+        info = r.step(1 << 20)
+        if info["time"] >= t_mid - 1e-9 or info["finished"]:
+            break
+    p = r.particles()
+    r.close()
+    return p, info["time"], info["frame"]
+
+
+def stream_ceiling():
+    """Hand-written streaming kernels (slab_stream_probe): GB/s of a 1 GiB
+    read+write copy and of G2P's particle streams for 8,147,024 particles."""
+    from slabpro import abi
+    gib = 1 << 30
+    cp = abi.stream_probe(abi.PROBE_COPY, gib, 10)
+    pt = abi.stream_probe(abi.PROBE_PARTICLES, 8147024, 10)
+    return {"copy_1GiB_GBs": 2 * gib / cp / 1e6, "g2p_streams_8147024_GBs": 60 * 8147024 / pt / 1e6,
+            "source": "slab_stream_probe (slab-pro_amd/csrc/slab_probe.hip)"}
+
+
+def ensemble_numbers(scenarios=256, fp64_sample=16):
+    """The 256-scenario ensemble on the 2000x2000 DEM at 5 m: FP32 batched
+    scenarios/h, the plain FP64 one-at-a-time rate on a size-stratified sample
+    (one scenario per size stratum), and each sampled scenario's max-height
+    L1 error of the batched FP32 result against its FP64 run."""
+    import numpy as np
+    from slabpro import abi, configs, ensemble
+    t0 = time.time()
+    dem = configs.c1_dem()
+    scen = [configs.c3_scenario(i) for i in range(scenarios)]
+    sizes = np.array([len(x["x"]) for x in scen])
+    fr = [x["friction"]["type"] for x in scen]
+    log(f"ensemble: {len(scen)} scenarios, {sizes.sum()} particles (min {sizes.min()}, max {sizes.max()}), "
+        f"generated in {time.time() - t0:.1f}s")
+    k = max(1, min(fp64_sample, len(scen)))
+    rank = np.argsort(sizes, kind="stable")
+    sample = [int(rank[int((q + 0.5) * len(scen) / k)]) for q in range(k)]
+    bat = ensemble.run_batched(dem, 5.0, scen, max_slots=len(scen), progress=log, results=set(sample))
+    log(f"fp32 batched: {bat['wall_s']:.1f}s, {bat['scenarios_per_hour']:.0f} scen/h, "
+        f"{bat['particle_updates_per_s']:.3g} pu/s, failed {bat['failed']}")
+    one64 = ensemble.run_one_at_a_time(dem, 5.0, [scen[i] for i in sample], abi.FP64, progress=log, results=True)
+    log(f"fp64 one-at-a-time on {k} scenarios: {one64['wall_s']:.1f}s, {one64['scenarios_per_hour']:.0f} scen/h")
+    errs = []
+    for q, i in enumerate(sample):
+        h32 = bat["results"][i][1][0]
+        h64 = one64["results"][q][0]
+        errs.append(float(np.abs(h32 - h64).sum() / np.abs(h64).sum()))
+    errs = np.array(errs)
+    return {
+        "workload": f"c3: {len(scen)} scenarios on the 2000x2000 DEM at 5 m, 1-10 ellipses each, log-uniform "
+                    f"2k-1M particles, 50% Voellmy / 30% Coulomb / 20% mu(I), 120 s each",
+        "particles_total": int(sizes.sum()),
+        "friction_counts": {"coulomb": fr.count(0), "voellmy": fr.count(1), "mu_I": fr.count(2)},
+        "fp32_batched_scenarios_per_hour": bat["scenarios_per_hour"],
+        "fp32_batched_wall_s": bat["wall_s"], "fp32_batched_particle_updates_per_s": bat["particle_updates_per_s"],
+        "failed": bat["failed"], "autotuned": bat["autotuned"],
+        "fp64_one_at_a_time_scenarios_per_hour": one64["scenarios_per_hour"],
+        "fp64_sample": f"{k} scenarios, one per size stratum (indices {sample})",
+        "speedup_vs_fp64_one_at_a_time": bat["scenarios_per_hour"] / one64["scenarios_per_hour"],
+        "hmax_L1_vs_fp64": {"max": float(errs.max()), "p95": float(np.percentile(errs, 95)),
+                            "mean": float(errs.mean()), "per_sampled_scenario": [float(e) for e in errs]},
+    }
 
 
 def copy_probe():
@@ -134,8 +237,13 @@ def main():
                     help="timed steps (default: 10,000 of the C2 run's ~12,000, i.e. most of its 120 s)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["c0", "c1", "c2"])
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=50, help="timed oracle steps per CPU sample")
+    ap.add_argument("--cpu-warmup", type=int, default=10, help="untimed oracle steps before each CPU sample")
+    ap.add_argument("--cpu-mid", type=float, default=60.0,
+                    help="simulated time (s) of the CPU baseline's mid-run sample (0: t = 0 sample only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 256-scenario ensemble numbers in the default line")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (FP32 form)")
     ap.add_argument("--form", default="fp32", choices=["fp32", "fp64"],
                     help="fp32: the FP32 form (int cell + FP32 offsets, FP32 bulk, FP64 time); fp64: plain form")
@@ -230,12 +338,29 @@ def main():
     traffic, traffic_src = measured_traffic(args.form, args.workload, dom)
     log("kernel times (ms total, launches): " + ", ".join(f"{k} {v[0]:.2f}/{v[1]}" for k, v in tim.items()))
 
+    run.close()
+    ceiling = stream_ceiling()
+    log(f"stream ceiling: {ceiling}")
+
     cpu = None
     if not args.no_cpu:
-        cpu_rate, cpu_el = cpu_baseline(cfg, args.cpu_steps, 1)
-        cpu = {"value": cpu_rate, "unit": "particle-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps (after prime + 1 warm-up step) of the same {args.workload} "
-                         f"workload, {npart} particles, FP64 oracle, single thread, {cpu_el:.1f} s"}
+        rate0, el0 = cpu_baseline(cfg, args.cpu_steps, args.cpu_warmup)
+        samples = {"t0": {"value": rate0, "timed_s": el0}}
+        if args.cpu_mid > 0:
+            st, tmid, fmid = mid_run_state(cfg, args.cpu_mid)
+            rate1, el1 = cpu_baseline(cfg, args.cpu_steps, args.cpu_warmup, st, tmid, fmid)
+            samples[f"t{tmid:g}s"] = {"value": rate1, "timed_s": el1}
+        rates = [v["value"] for v in samples.values()]
+        cpu = {"value": sum(rates) / len(rates), "unit": "particle-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_warmup} untimed + {args.cpu_steps} timed steps of the same {args.workload} "
+                         f"workload ({npart} particles), FP64 oracle, single thread, from the t = 0 prime and "
+                         f"from the state at t = {args.cpu_mid:g} s (FP64 form on the GPU); value = mean of "
+                         f"the samples",
+               "samples": samples, "host": host_facts()}
+
+    ens = None
+    if not args.no_ensemble and args.workload == "c2" and args.form == "fp32":
+        ens = ensemble_numbers()
 
     out = {
         "metric": "particle-updates/s at 8M particles",
@@ -263,48 +388,23 @@ def main():
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBs": step_alg / (ms_step * 1e-3) / 1e9,
                           "frac": step_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "n_active_nodes": n_act},
         "kernels_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in tim.items()},
+        "stream_ceiling": ceiling,
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
-    run.close()
+    out["roofline"]["frac_of_copy_ceiling"] = achieved / ceiling["copy_1GiB_GBs"]
     print(json.dumps(out), flush=True)
 
 
 def ensemble_main(args):
     """C3: 256 scenarios on the 2000x2000 DEM at 5 m; FP32 batched (all) vs the
-    plain FP64 one-at-a-time GPU path (a size-stratified sample, stated)."""
-    import numpy as np
-    from slabpro import abi, configs, ensemble
-    t0 = time.time()
-    dem = configs.c1_dem()
-    scen = [configs.c3_scenario(i) for i in range(args.scenarios)]
-    sizes = np.array([len(s["x"]) for s in scen])
-    log(f"ensemble: {len(scen)} scenarios, {sizes.sum()} particles (min {sizes.min()}, max {sizes.max()}), "
-        f"generated in {time.time() - t0:.1f}s")
-    fr = [s["friction"]["type"] for s in scen]
-    bat = ensemble.run_batched(dem, 5.0, scen, max_slots=len(scen), progress=log)
-    log(f"fp32 batched: {bat['wall_s']:.1f}s, {bat['scenarios_per_hour']:.0f} scen/h, "
-        f"{bat['particle_updates_per_s']:.3g} pu/s, failed {bat['failed']}")
-    k = max(1, min(args.fp64_sample, len(scen)))
-    rank = np.argsort(sizes, kind="stable")
-    sample = [int(rank[int((q + 0.5) * len(scen) / k)]) for q in range(k)]
-    one64 = ensemble.run_one_at_a_time(dem, 5.0, [scen[i] for i in sample], abi.FP64, progress=log)
-    log(f"fp64 one-at-a-time on {k} scenarios: {one64['wall_s']:.1f}s, {one64['scenarios_per_hour']:.0f} scen/h")
-    one32 = ensemble.run_one_at_a_time(dem, 5.0, [scen[i] for i in sample], abi.FP32, progress=log)
-    log(f"fp32 one-at-a-time on the same {k}: {one32['wall_s']:.1f}s, {one32['scenarios_per_hour']:.0f} scen/h")
-    out = {
-        "metric": "scenarios/h on the 256-scenario ensemble",
-        "value": bat["scenarios_per_hour"], "unit": "scenarios/h", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f32", "data": "synthetic (seeded DEM + ellipse releases)",
-        "config": {"workload": f"c3: {len(scen)} scenarios on the 2000x2000 DEM at 5 m, 1-10 ellipses each, "
-                               f"log-uniform 2k-1M particles, 50% Voellmy / 30% Coulomb / 20% mu(I), 120 s each",
-                   "particles_total": int(sizes.sum()),
-                   "friction_counts": {"coulomb": fr.count(0), "voellmy": fr.count(1), "mu_I": fr.count(2)}},
-        "fp32_batched": {k2: v for k2, v in bat.items() if k2 != "results"},
-        "fp64_one_at_a_time": {"sample": f"{k} scenarios, one per size stratum (ranks {sample})",
-                               **{k2: v for k2, v in one64.items() if k2 != "per_scenario_s"}},
-        "fp32_one_at_a_time": {k2: v for k2, v in one32.items() if k2 != "per_scenario_s"},
-        "speedup_vs_fp64_one_at_a_time": bat["scenarios_per_hour"] / one64["scenarios_per_hour"],
-    }
+    plain FP64 one-at-a-time GPU path (a size-stratified sample, stated), with
+    the sampled scenarios' max-height L1 error beside the throughput."""
+    e = ensemble_numbers(args.scenarios, args.fp64_sample)
+    out = {"metric": "scenarios/h on the 256-scenario ensemble", "value": e["fp32_batched_scenarios_per_hour"],
+           "unit": "scenarios/h", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+           "data": "synthetic (seeded DEM + ellipse releases)", "config": {"workload": e["workload"]},
+           "ensemble": e}
     print(json.dumps(out), flush=True)
 
 

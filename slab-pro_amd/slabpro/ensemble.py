@@ -31,9 +31,12 @@ def run_batched(dem, dx, scenarios, max_slots=256, capacity=None, steps_per_call
                 progress=None, every_s=15.0, autotune=True):
     """Advance every scenario in one FP32 batch.  Scenarios are admitted in
     order whenever a slot and particle capacity are free; finished slots are
-    collected (max maps) and released.  Returns wall time and counts.
-    progress(msg) is called about every every_s seconds with the state of the
-    batch (finished scenarios, active slots, simulated-time range)."""
+    collected and released.  Returns wall time and counts; "results"[i] =
+    (slot info, max maps or None): the max maps (h_max, v_max2) are downloaded
+    for every scenario when results is True, or for the indices in results
+    when it is a set.  progress(msg) is called about every every_s seconds
+    with the state of the batch (finished scenarios, active slots,
+    simulated-time range)."""
     n = len(scenarios)
     cap = capacity or sum(_footprint(s) for s in scenarios)
     t0 = time.perf_counter()
@@ -74,7 +77,8 @@ def run_batched(dem, dx, scenarios, max_slots=256, capacity=None, steps_per_call
             if st[slot] in (abi.SLOT_DONE, abi.SLOT_FAILED):
                 info = b.slot_info(slot)
                 pupd += info["np"] * info["step"]
-                done[idx] = (info, b.max_maps(slot) if results else None)
+                want = (idx in results) if isinstance(results, (set, frozenset)) else bool(results)
+                done[idx] = (info, b.max_maps(slot) if want else None)
                 b.release(slot)
                 del slot_of[slot]
     wall = time.perf_counter() - t0

@@ -185,8 +185,9 @@ slab_status slab_batch_create(const slab_terrain* terrain, int32_t max_slots, in
                               slab_batch** out);
 /* Admits one scenario (particles as initializeParticles leaves them; vx, vy
  * and m may be NULL); it is primed (dt = 0 pass, simulation.cpp:493-495) by
- * the next step.  SLAB_ENOMEM when no slot or particle capacity is free,
- * SLAB_EINVAL for non-finite inputs or parameters; *slot = -1 then. */
+ * the next step.  SLAB_ENOMEM when no slot, particle capacity or active-tile
+ * capacity (see slab_batch_tiles) is free, SLAB_EINVAL for non-finite inputs
+ * or parameters; *slot = -1 then. */
 slab_status slab_batch_admit(slab_batch* b, const slab_params* params, const slab_friction* friction, int64_t np,
                              const double* x, const double* y, const double* vx, const double* vy, const double* h,
                              const double* A, const double* m, int32_t* slot);
@@ -223,9 +224,26 @@ slab_status slab_batch_download_particles_noalloc(slab_batch* b, int32_t slot, d
  * compaction pass inside this call, so their capacity and the slot are free
  * for the next admission.  The other slots are not advanced. */
 slab_status slab_batch_release(slab_batch* b, int32_t slot);
-/* out[0] = device bytes per particle, out[1] = device bytes per slot,
- * out[2] = free particle capacity, out[3] = free slots (admission by footprint). */
+/* out[0] = device bytes per particle, out[1] = device bytes per slot (its
+ * h_max / v_max2 maps), out[2] = free particle capacity, out[3] = free slots.
+ * With slab_batch_tiles this is the batch's device footprint: particles +
+ * slots + active tiles. */
 slab_status slab_batch_footprint(slab_batch* b, int64_t out[4]);
+/* Active tiles of the batch.  The grid velocity of a step is kept only for its
+ * grid tiles (16 x 16 nodes holding particles, and their right / upper /
+ * diagonal neighbours), in a pool of fixed capacity; P2G keeps a partial
+ * region per particle tile.  out[0] = grid-tile capacity, out[1] = grid tiles
+ * of the last step, out[2] = particle-tile capacity, out[3] = particle tiles of
+ * the last step, out[4] = device bytes per grid tile, out[5] = device bytes
+ * per particle tile.  No reference counterpart (the reference sizes one
+ * bounding-box window per run, src/simulation.cpp:584-642). */
+slab_status slab_batch_tiles(slab_batch* b, int64_t out[6]);
+/* Admission reserves `factor` times the particle and grid tiles of a release
+ * (default 4) for its flow to spread; slab_batch_admit refuses a scenario
+ * whose reservation does not fit beside the live ones.  factor = 0 admits
+ * without a tile check (a step whose tiles then overflow the capacity returns
+ * SLAB_EDOMAIN, see slab_batch_step). */
+slab_status slab_batch_set_tile_growth(slab_batch* b, int32_t factor);
 /* Replay the step's launch sequence from a captured hipGraph (on = 1). */
 slab_status slab_batch_use_graph(slab_batch* b, int32_t on);
 /* Storage re-sort period of the FP32 form (default 16): every `steps`-th step

@@ -881,7 +881,10 @@ slab_status slab_batch_admit(slab_batch* b, const slab_params* params, const sla
     if (s < 0) {
         *slot = -1;
         if (why == kStoppedByError) return fail(SLAB_ESTATE, why);
-        return fail((why == "no free slot" || why == "particle capacity exhausted") ? SLAB_ENOMEM : SLAB_EDEVICE, why);
+        return fail((why == "no free slot" || why == "particle capacity exhausted" || why == "tile capacity exhausted")
+                        ? SLAB_ENOMEM
+                        : SLAB_EDEVICE,
+                    why);
     }
     *slot = s;
     return SLAB_OK;
@@ -976,6 +979,20 @@ slab_status slab_batch_footprint(slab_batch* b, int64_t out[4]) {
     out[2] = engine_free_particles(b->eng);
     out[3] = free_slots;
     return SLAB_OK;
+}
+
+slab_status slab_batch_tiles(slab_batch* b, int64_t out[6]) {
+    if (!b || !out) return fail(SLAB_EINVAL, "null argument");
+    std::string why;
+    if (!engine_pull(b->eng, &why)) return eng_fail(why);
+    engine_tiles(b->eng, out);
+    return SLAB_OK;
+}
+
+slab_status slab_batch_set_tile_growth(slab_batch* b, int32_t factor) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    std::string why;
+    return engine_set_tile_growth(b->eng, factor, &why) ? SLAB_OK : fail(SLAB_EINVAL, why);
 }
 
 slab_status slab_batch_use_graph(slab_batch* b, int32_t on) {

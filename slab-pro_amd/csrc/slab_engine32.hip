@@ -465,7 +465,7 @@ __device__ __forceinline__ void scan_totals(ECtl* c, const int tot[3]) {
     c->n_sorted = tot[0];
     c->n_ptiles = tot[1];
     c->n_gtiles = tot[2];
-    if (tot[1] > c->tile_cap) {
+    if (tot[1] > c->tile_cap || tot[2] > c->gtile_cap) {
         c->err = 7;
         c->stop = STOP_ERROR;
     }
@@ -498,7 +498,7 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
                                                  int nblk, int merged, int* __restrict__ tile_begin,
                                                  int* __restrict__ ptiles, int* __restrict__ gtiles,
                                                  int* __restrict__ pstamp, int* __restrict__ gstamp,
-                                                 int* __restrict__ tile_pos) {
+                                                 int* __restrict__ tile_pos, int* __restrict__ gtile_pos) {
     if (c->stop) return;
     const bool idle = scan_block_idle(c, slots, ntiles);
     if (idle && blockIdx.x != 0) return; // block 0 writes the totals even when idle
@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
         int tot[3];
         block_prefix3(bsum, nblk, base, tot);
         if (blockIdx.x == 0 && threadIdx.x == 0) scan_totals(c, tot);
-        if (tot[1] > c->tile_cap) return; // every block sees the same totals
+        if (tot[1] > c->tile_cap || tot[2] > c->gtile_cap) return; // every block sees the same totals
     } else {
         for (int q = 0; q < 3; ++q) base[q] = boff[3 * blockIdx.x + q];
     }
@@ -535,6 +535,7 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
         }
         if (f[k][2]) {
             gtiles[run[2]] = t;
+            gtile_pos[t] = run[2];
             gstamp[t] = ep;
         }
         for (int q = 0; q < 3; ++q) run[q] += f[k][q];
@@ -1030,6 +1031,13 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
 // workgroup, one thread per node, after merging the node's partial sums from
 // the particle tiles of the same slot that reach it (own, left, below,
 // diagonal; fixed order).
+// The grid velocity / FLIP field {vx, vy, flipx, flipy} it writes lives for
+// one step.  A single run keeps it dense over the terrain (slot * nn + i*ny +
+// j).  A batch (POOL) keeps it in a pool of the step's grid tiles only: tile
+// w of the grid-tile list holds its 256 nodes at w*256 + (i&15)*16 + (j&15),
+// and gtile_pos maps a tile id to its list position, so the batch's memory
+// grows with its active tiles instead of slots x terrain.
+template <bool POOL>
 __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ slots,
                                               const int* __restrict__ gtiles, const int* __restrict__ pstamp,
                                               const int* __restrict__ tile_pos, const float* __restrict__ tpart,
@@ -1038,6 +1046,9 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
                                               float* __restrict__ v_max2, float4* __restrict__ vf,
                                               float* __restrict__ out_m, float* __restrict__ out_h,
                                               float* __restrict__ out_vz) {
+    // multiply-adds fused per source expression, so the two layouts (which
+    // differ only in addressing) round the same way
+#pragma clang fp contract(on)
     if (c->stop || c->compact) return;
     const int nt = c->n_gtiles, ep = c->epoch, nx = c->nx, ny = c->ny, nty = c->nty, ntd = c->ntd;
     const size_t nn = (size_t)nx * ny;
@@ -1131,7 +1142,7 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
             const float v2 = nvx * nvx + nvy * nvy;
             if (v_max2[k] < v2) v_max2[k] = v2;
         }
-        vf[k] = make_float4(nvx, nvy, flx, fly);
+        vf[POOL ? (size_t)w * TILE_CELLS + threadIdx.x : k] = make_float4(nvx, nvy, flx, fly);
         if (out_m && S.out_step) {
             out_m[k] = am;
             out_h[k] = hh;
@@ -1148,11 +1159,16 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
 // constant mass and creation index are not rewritten).  Also writes the next
 // keys and tile counts (keys indexed like the storage) and the per-slot max
 // |v|^2.
+template <bool POOL>
 __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                              const int* __restrict__ idx, const float4* __restrict__ vf,
+                                             const int* __restrict__ gtile_pos,
                                              const float* __restrict__ tz, const float* __restrict__ tgx,
                                              const float* __restrict__ tgy, int* __restrict__ key,
                                              int* __restrict__ tile_count, unsigned long long* __restrict__ vmx) {
+    // multiply-adds fused per source expression, so the two layouts (which
+    // differ only in addressing) round the same way
+#pragma clang fp contract(on)
     if (c->stop) return;
     const PB32 P = ecur(c, A, B);
     const bool reo = c->reorder != 0;
@@ -1195,14 +1211,34 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
             weights3f(fx, inv, wx, dwx);
             weights3f(fy, inv, wy, dwy);
             float picx = 0.f, picy = 0.f, flx = 0.f, fly = 0.f, z = 0.f, tr = 0.f;
-            const float4* vfs = vf + (size_t)sl * nn;
+            // a stencil spans at most 2 x 2 grid tiles: their list positions
+            // (pool) are read once (a tile the stencil does not reach reads
+            // its base tile's entry)
+            int gp[2][2] = {{0, 0}, {0, 0}};
+            if (POOL) {
+                const int t0 = sl * c->ntd + (ib >> 4) * c->nty + (jb >> 4);
+                const int ox1 = ((ib + 2) >> 4) != (ib >> 4) ? c->nty : 0;
+                const int oy1 = ((jb + 2) >> 4) != (jb >> 4) ? 1 : 0;
+                gp[0][0] = gtile_pos[t0];
+                gp[1][0] = gtile_pos[t0 + ox1];
+                gp[0][1] = gtile_pos[t0 + oy1];
+                gp[1][1] = gtile_pos[t0 + ox1 + oy1];
+            }
 #pragma unroll
             for (int di = 0; di < 3; ++di) {
 #pragma unroll
                 for (int dj = 0; dj < 3; ++dj) {
                     const size_t k = (size_t)(ib + di) * ny + (jb + dj);
                     const float w = wx[di] * wy[dj];
-                    const float4 q = vfs[k];
+                    size_t kv;
+                    if (POOL) {
+                        const int i = ib + di, j = jb + dj;
+                        const int qx = (i >> 4) - (ib >> 4), qy = (j >> 4) - (jb >> 4);
+                        kv = (size_t)gp[qx][qy] * TILE_CELLS + (((i & 15) << 4) | (j & 15));
+                    } else {
+                        kv = (size_t)sl * nn + k;
+                    }
+                    const float4 q = vf[kv];
                     picx += q.x * w;
                     picy += q.y * w;
                     flx += q.z * w;
@@ -1491,6 +1527,15 @@ struct Engine32 {
     hipGraphExec_t gexec = nullptr;
     int device = 0;
     unsigned char* scratch_ptr = nullptr; // this device's address of `scratch`
+    // batches keep the grid velocity in a pool of gtile_cap grid tiles
+    // (single runs: dense over the terrain)
+    bool pool_vf = false;
+    int gtile_cap = 0;
+    int* gtile_pos = nullptr;
+    // admission assumes a scenario's active tiles may grow to this many
+    // times those of its release
+    int tile_growth = 4;
+    std::vector<int64_t> res_ptiles, res_gtiles; // per slot: tiles reserved at admission
 };
 
 namespace {
@@ -1563,7 +1608,8 @@ void enqueue_estep(Engine32* e, int step) {
     if (!merged) tk(e, 2, step, [&] { k_escan<<<1, WG, 0, s>>>(c, e->bsum, e->nblk, e->boff); });
     tk(e, 3, step, [&] {
         k_eoffsets<<<e->nblk, WG, 0, s>>>(c, e->d_slots, e->tile_count, e->ntiles, e->boff, e->bsum, e->nblk, merged,
-                                           e->tile_begin, e->ptiles, e->gtiles, e->pstamp, e->gstamp, e->tile_pos);
+                                           e->tile_begin, e->ptiles, e->gtiles, e->pstamp, e->gstamp, e->tile_pos,
+                                           e->gtile_pos);
     });
     tk(e, 4, step, [&] {
         const unsigned nb = (unsigned)std::max<int64_t>(1, (e->launch_n + WG * SC_PER - 1) / (WG * SC_PER));
@@ -1578,12 +1624,21 @@ void enqueue_estep(Engine32* e, int step) {
                                  e->tpart);
     });
     tk(e, 7, step, [&] {
-        k_egrid<<<tb, WG, 0, s>>>(c, e->d_slots, e->gtiles, e->pstamp, e->tile_pos, e->tpart, e->tgx, e->tgy, e->tcth,
-                                  e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz);
+        if (e->pool_vf)
+            k_egrid<true><<<tb, WG, 0, s>>>(c, e->d_slots, e->gtiles, e->pstamp, e->tile_pos, e->tpart, e->tgx, e->tgy,
+                                            e->tcth, e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz);
+        else
+            k_egrid<false><<<tb, WG, 0, s>>>(c, e->d_slots, e->gtiles, e->pstamp, e->tile_pos, e->tpart, e->tgx, e->tgy,
+                                             e->tcth, e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz);
     });
     tk(e, 8, step, [&] {
-        k_eg2p<<<xcd_grid((e->launch_n + WG - 1) / WG), WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->tz, e->tgx, e->tgy, e->key,
-                                           e->tile_count, e->vmx);
+        const unsigned gb = xcd_grid((e->launch_n + WG - 1) / WG);
+        if (e->pool_vf)
+            k_eg2p<true><<<gb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->gtile_pos, e->tz, e->tgx, e->tgy,
+                                           e->key, e->tile_count, e->vmx);
+        else
+            k_eg2p<false><<<gb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->gtile_pos, e->tz, e->tgx,
+                                            e->tgy, e->key, e->tile_count, e->vmx);
     });
 }
 
@@ -1680,7 +1735,27 @@ int64_t engine_particle_bytes() {
 }
 
 int64_t engine_slot_bytes(const Engine32* e) {
-    return (int64_t)e->nn * (16 + 8 + (e->single ? 12 : 0)) + VMAX_SHARDS * 8 + (int64_t)sizeof(Slot);
+    // max maps (h_max, v_max2) per node; a single run also keeps its grid
+    // velocity and output fields dense
+    return (int64_t)e->nn * (8 + (e->single ? 16 + 12 : 0)) + VMAX_SHARDS * 8 + (int64_t)sizeof(Slot);
+}
+
+bool engine_set_tile_growth(Engine32* e, int factor, std::string* err) {
+    if (factor < 0 || factor > 1024) {
+        if (err) *err = "tile growth factor must be in [0, 1024]";
+        return false;
+    }
+    e->tile_growth = factor;
+    return true;
+}
+
+void engine_tiles(const Engine32* e, int64_t out[6]) {
+    out[0] = e->gtile_cap;
+    out[1] = e->h_c->n_gtiles;
+    out[2] = e->tile_cap;
+    out[3] = e->h_c->n_ptiles;
+    out[4] = e->pool_vf ? (int64_t)TILE_CELLS * (int64_t)sizeof(float4) : 0;
+    out[5] = (int64_t)CELL_STRIDE * 4 + (int64_t)P2G_FIELDS * REGION_NODES * 4;
 }
 
 // Device address of `scratch` on each device, taken once per device with that
@@ -1739,8 +1814,15 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     if (const char* v = std::getenv("SLAB_SCAN_MERGE_MAX_BLOCKS")) e->scan_merge_max = std::atoi(v);
     e->cap = max_particles;
     e->tile_cap = (int)std::min<int64_t>(e->ntiles, std::max<int64_t>(65536, max_particles / 32));
-    // a smaller active-tile capacity (tests of the overflow path)
-    if (const char* v = std::getenv("SLAB_TILE_CAP")) e->tile_cap = std::max(1, std::min(e->tile_cap, std::atoi(v)));
+    // a batch's grid tiles are its particle tiles plus right / upper /
+    // diagonal neighbours: at most 4x, in practice well under 2x
+    e->pool_vf = !single;
+    e->gtile_cap = single ? e->ntiles : (int)std::min<int64_t>(e->ntiles, 2 * (int64_t)e->tile_cap);
+    // smaller active-tile capacities (tests of admission and the overflow path)
+    if (const char* v = std::getenv("SLAB_TILE_CAP")) {
+        e->tile_cap = std::max(1, std::min(e->tile_cap, std::atoi(v)));
+        if (!single) e->gtile_cap = std::max(1, std::min(e->gtile_cap, 2 * e->tile_cap));
+    }
     std::string why;
     auto fail = [&](const std::string& m) -> Engine32* {
         if (err) *err = m;
@@ -1760,7 +1842,9 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
               ealloc(e->allocs, &e->tz, e->nn, &why) && ealloc(e->allocs, &e->tgx, e->nn, &why) &&
               ealloc(e->allocs, &e->tgy, e->nn, &why) && ealloc(e->allocs, &e->tcth, e->nn, &why) &&
               ealloc(e->allocs, &e->hmax, sn, &why) && ealloc(e->allocs, &e->vmax2, sn, &why) &&
-              ealloc(e->allocs, &e->vf, sn, &why) && ealloc(e->allocs, &e->vmx, (size_t)nslots * VMAX_SHARDS, &why) &&
+              ealloc(e->allocs, &e->vf, e->pool_vf ? (size_t)e->gtile_cap * TILE_CELLS : sn, &why) &&
+              ealloc(e->allocs, &e->gtile_pos, e->ntiles, &why) &&
+              ealloc(e->allocs, &e->vmx, (size_t)nslots * VMAX_SHARDS, &why) &&
               ealloc(e->allocs, &e->tile_count, e->ntiles, &why) && ealloc(e->allocs, &e->cursor, e->ntiles, &why) &&
               ealloc(e->allocs, &e->tile_begin, e->ntiles, &why) &&
               ealloc(e->allocs, &e->pstamp, e->ntiles, &why) && ealloc(e->allocs, &e->gstamp, e->ntiles, &why) &&
@@ -1792,7 +1876,9 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
          hipMemcpyAsync(e->tgy, gy.data(), e->nn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(e->tcth, cth.data(), e->nn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemsetAsync(e->hmax, 0, sn * 4, s) == hipSuccess && hipMemsetAsync(e->vmax2, 0, sn * 4, s) == hipSuccess &&
-         hipMemsetAsync(e->vf, 0, sn * sizeof(float4), s) == hipSuccess &&
+         hipMemsetAsync(e->vf, 0, (e->pool_vf ? (size_t)e->gtile_cap * TILE_CELLS : sn) * sizeof(float4), s) ==
+             hipSuccess &&
+         hipMemsetAsync(e->gtile_pos, 0, (size_t)e->ntiles * 4, s) == hipSuccess &&
          hipMemsetAsync(e->vmx, 0, (size_t)nslots * VMAX_SHARDS * 8, s) == hipSuccess &&
          hipMemsetAsync(e->tile_count, 0, (size_t)e->ntiles * 4, s) == hipSuccess &&
          hipMemsetAsync(e->cursor, 0, (size_t)e->ntiles * 4, s) == hipSuccess &&
@@ -1813,10 +1899,13 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     c.single = single ? 1 : 0;
     c.has_out = single ? 1 : 0;
     c.tile_cap = e->tile_cap;
+    c.gtile_cap = e->gtile_cap;
     c.reorder_every = kReorderEvery;
     c.dx = dx;
     c.inv_dx = e->inv;
     e->admit_base.assign(nslots, -1);
+    e->res_ptiles.assign(nslots, 0);
+    e->res_gtiles.assign(nslots, 0);
     if (!push_all(e, err)) {
         engine_destroy(e);
         return nullptr;
@@ -1917,6 +2006,47 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
         fm[p] = (float)(m ? m[p] : prm.rho * h[p] * A[p]); // simulation.cpp:452
         pid[p] = (int32_t)p;
     }
+    // admission by active tiles: the release's particle tiles and grid tiles
+    // (its particle tiles plus right / upper / diagonal neighbours), reserved
+    // tile_growth times over for the flow to spread, must fit beside the
+    // larger of the live slots' reservations and the last step's counts
+    int64_t res_p = 0, res_g = 0;
+    if (!e->single && e->tile_growth > 0) {
+        std::vector<int32_t> pt, gt;
+        pt.reserve(n);
+        for (size_t p = 0; p < n; ++p) {
+            if (sl[p] & SLOT_OUT) continue;
+            const int ii = cell[p] >> 16, jj = cell[p] & 0xffff;
+            const int ib = std::min(std::max(ii - (fo[p].x < 0.5f ? 1 : 0), 0), e->nx - 3);
+            const int jb = std::min(std::max(jj - (fo[p].y < 0.5f ? 1 : 0), 0), e->ny - 3);
+            pt.push_back((ib >> 4) * e->nty + (jb >> 4));
+        }
+        std::sort(pt.begin(), pt.end());
+        pt.erase(std::unique(pt.begin(), pt.end()), pt.end());
+        gt.reserve(4 * pt.size());
+        for (int t : pt) {
+            const int tx = t / e->nty, ty = t - tx * e->nty;
+            for (int dx = 0; dx < 2; ++dx)
+                for (int dy = 0; dy < 2; ++dy)
+                    if (tx + dx < e->ntx && ty + dy < e->nty) gt.push_back((tx + dx) * e->nty + ty + dy);
+        }
+        std::sort(gt.begin(), gt.end());
+        gt.erase(std::unique(gt.begin(), gt.end()), gt.end());
+        // a scenario never holds more than the terrain's tiles
+        res_p = std::min<int64_t>((int64_t)e->tile_growth * (int64_t)pt.size(), e->ntd);
+        res_g = std::min<int64_t>((int64_t)e->tile_growth * (int64_t)gt.size(), e->ntd);
+        int64_t sum_p = 0, sum_g = 0;
+        for (int q = 0; q < e->nslots; ++q) {
+            sum_p += e->res_ptiles[q];
+            sum_g += e->res_gtiles[q];
+        }
+        const int64_t used_p = std::max<int64_t>(sum_p, e->h_c->n_ptiles);
+        const int64_t used_g = std::max<int64_t>(sum_g, e->h_c->n_gtiles);
+        if (used_p + res_p > e->tile_cap || used_g + res_g > e->gtile_cap) {
+            if (err) *err = "tile capacity exhausted";
+            return -1;
+        }
+    }
     const PB32& P = e->h_c->parity ? e->B : e->A;
     hipStream_t s = e->stream;
     bool ok = true;
@@ -1957,6 +2087,8 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
     k_ekeys<<<nb, WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, (int)base, (int)(base + np), e->key, e->tile_count);
     if (!hip_ok(hipGetLastError(), "admit kernels", err) || !hip_ok(hipStreamSynchronize(s), "admit", err)) return -1;
     e->admit_base[slot] = base;
+    e->res_ptiles[slot] = res_p;
+    e->res_gtiles[slot] = res_g;
     return slot;
 }
 
@@ -2001,6 +2133,8 @@ bool engine_release(Engine32* e, int slot, std::string* err) {
     }
     e->h_slots[slot].status = SLOT_FREE;
     e->h_slots[slot].np = 0;
+    e->res_ptiles[slot] = 0;
+    e->res_gtiles[slot] = 0;
     if (!push_all(e, err)) return false;
     // Re-key every stored particle with the new statuses (the released slot's
     // particles get no key), then run one re-sort pass that only permutes:

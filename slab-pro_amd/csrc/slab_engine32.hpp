@@ -60,6 +60,8 @@ struct ECtl {
     int32_t reorder;       // this step's G2P writes the other buffer in sorted order
     int32_t reorder_every; // storage re-sort period in steps (other steps update in place)
     int32_t pend_end;      // a started step still has to be ended (time, frames, buffer flip)
+    int32_t gtile_cap;     // grid tiles the grid-velocity pool holds (batches; all tiles for a single run)
+    int32_t pad_;
     double dx, inv_dx;
 };
 
@@ -100,7 +102,13 @@ bool engine_set_time(Engine32* e, int slot, double time, int frame, std::string*
 bool engine_keep_grid(Engine32* e, int slot, int on, std::string* err);
 void engine_timing(Engine32* e, int enable, double* ms, int64_t* counts, int n);
 int64_t engine_particle_bytes();                      // device bytes per particle of capacity
-int64_t engine_slot_bytes(const Engine32* e);         // device bytes per slot (grid + max maps)
+int64_t engine_slot_bytes(const Engine32* e);         // device bytes per slot (max maps; dense grid of a single run)
+// Active-tile capacity and use: out = {grid-tile capacity, grid tiles of the
+// last step, particle-tile capacity, particle tiles of the last step,
+// device bytes per grid tile, device bytes per particle tile}.
+void engine_tiles(const Engine32* e, int64_t out[6]);
+// Admission reserves this many times a release's tiles for its flow to spread.
+bool engine_set_tile_growth(Engine32* e, int factor, std::string* err);
 int64_t engine_free_particles(const Engine32* e);     // capacity left for admissions
 int engine_use_graph(Engine32* e, int on);             // capture the step sequence in a hipGraph
 // Storage re-sort period: every k-th step G2P writes the particles in sorted

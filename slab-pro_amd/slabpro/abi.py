@@ -98,6 +98,8 @@ SIGNATURES = {
     "slab_batch_release": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "slab_batch_footprint": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int64)]),
     "slab_batch_use_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "slab_batch_tiles": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int64)]),
+    "slab_batch_set_tile_growth": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "slab_batch_timing": (ctypes.c_int, [_VP, ctypes.c_int32, _DP, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
     "slab_batch_destroy": (None, [_VP]),
     "slab_batch_set_resort": (ctypes.c_int, [_VP, ctypes.c_int32]),
@@ -391,6 +393,15 @@ class Batch:
         _check(lib().slab_batch_footprint(self.h, out))
         return {"bytes_per_particle": out[0], "bytes_per_slot": out[1], "free_particles": out[2],
                 "free_slots": out[3]}
+
+    def tiles(self):
+        out = (ctypes.c_int64 * 6)()
+        _check(lib().slab_batch_tiles(self.h, out))
+        return {"grid_tile_capacity": out[0], "grid_tiles": out[1], "particle_tile_capacity": out[2],
+                "particle_tiles": out[3], "bytes_per_grid_tile": out[4], "bytes_per_particle_tile": out[5]}
+
+    def set_tile_growth(self, factor):
+        _check(lib().slab_batch_set_tile_growth(self.h, int(factor)))
 
     def use_graph(self, on=True):
         _check(lib().slab_batch_use_graph(self.h, 1 if on else 0))

@@ -299,6 +299,7 @@ def test_active_tile_overflow_stops_the_batch(monkeypatch):
     monkeypatch.setenv("SLAB_TILE_CAP", "1")
     b = abi.Batch(z, dx, 2, 100000)
     monkeypatch.delenv("SLAB_TILE_CAP")
+    b.set_tile_growth(0)  # no tile check at admission: the step meets the overflow
     s = b.admit(_params(), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
     assert s is not None
     with pytest.raises(abi.SlabError) as ei:
@@ -317,4 +318,48 @@ def test_active_tile_overflow_stops_the_batch(monkeypatch):
     assert np.max(np.abs(p["x"] - part["x"])) < 1e-3 * dx  # the admitted state (FP32 offsets)
     hm, _ = b.max_maps(s)
     assert np.all(hm == 0.0)  # no step completed
+    b.close()
+
+
+def test_admission_by_active_tiles(monkeypatch):
+    # A tile capacity that holds one release's reserved tiles but not two:
+    # the second scenario is refused (admit -> None, SLAB_ENOMEM) although a
+    # slot and particle capacity are free, and admitted once the first is
+    # released.  The grid velocity lives in the batch's grid-tile pool; the
+    # scenario's results equal its run alone in a batch with room to spare.
+    from slabpro import abi
+    z, dx = _terrain()
+    part, fr = _scenario(61, FRICS[1])
+    ib = np.floor(np.asarray(part["x"]) / dx - 0.5).astype(int)
+    jb = np.floor(np.asarray(part["y"]) / dx - 0.5).astype(int)
+    pt = {(i >> 4, j >> 4) for i, j in zip(ib, jb)}
+    gt = {(a + u, b + v) for a, b in pt for u in (0, 1) for v in (0, 1)}
+    growth = 4
+    cap = growth * max(len(pt), (len(gt) + 1) // 2)
+    monkeypatch.setenv("SLAB_TILE_CAP", str(cap))
+    b = abi.Batch(z, dx, 4, 200000)
+    monkeypatch.delenv("SLAB_TILE_CAP")
+    b.set_tile_growth(growth)
+    t = b.tiles()
+    print("tiles", t, "release tiles", len(pt), len(gt), flush=True)
+    assert t["particle_tile_capacity"] == cap and t["grid_tile_capacity"] == 2 * cap
+    assert t["bytes_per_grid_tile"] == 256 * 16
+    args = (_params(), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
+    s0 = b.admit(*args)
+    assert s0 is not None
+    assert b.admit(*args) is None  # tiles, not slots or particles, are exhausted
+    fp = b.footprint()
+    assert fp["free_slots"] == 3 and fp["free_particles"] >= len(part["x"])
+    while b.step(64) > 0:
+        pass
+    first = (b.max_maps(s0), b.particles(s0))
+    b.release(s0)
+    s1 = b.admit(*args)
+    assert s1 is not None
+    while b.step(64) > 0:
+        pass
+    (ref,) = _run_batch([(part, fr)])
+    for (hm, vm), p in (first, (b.max_maps(s1), b.particles(s1))):
+        assert np.array_equal(hm, ref[0][0]) and np.array_equal(vm, ref[0][1])
+        assert np.array_equal(p["x"], ref[1]["x"]) and np.array_equal(p["h"], ref[1]["h"])
     b.close()

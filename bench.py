@@ -161,10 +161,10 @@ def stream_ceiling():
     read+write copy and of G2P's particle streams for 8,147,024 particles."""
     from slabpro import abi
     gib = 1 << 30
-    cp = abi.stream_probe(abi.PROBE_COPY, gib, 10)
+    cp = min(abi.stream_probe(abi.PROBE_COPY, gib, 10), abi.stream_probe(abi.PROBE_COPY_NT, gib, 10))
     pt = abi.stream_probe(abi.PROBE_PARTICLES, 8147024, 10)
     return {"copy_1GiB_GBs": 2 * gib / cp / 1e6, "g2p_streams_8147024_GBs": 60 * 8147024 / pt / 1e6,
-            "source": "slab_stream_probe (slab-pro_amd/csrc/slab_probe.hip)"}
+            "source": "slab_stream_probe (slab-pro_amd/csrc/slab_probe.hip), faster of the two copy kernels"}
 
 
 def ensemble_numbers(scenarios=256, fp64_sample=16):
@@ -221,8 +221,10 @@ def copy_probe():
     gib = 1 << 30
     out = {}
     for name, kind, n, nbytes in (("copy_1GiB", abi.PROBE_COPY, gib, 2 * gib),
+                                  ("copy_nt_1GiB", abi.PROBE_COPY_NT, gib, 2 * gib),
                                   ("read_1GiB", abi.PROBE_READ, gib, gib),
                                   ("write_1GiB", abi.PROBE_WRITE, gib, gib),
+                                  ("write_nt_1GiB", abi.PROBE_WRITE_NT, gib, gib),
                                   ("particles_8147024", abi.PROBE_PARTICLES, 8147024, 60 * 8147024)):
         ms = abi.stream_probe(kind, n, 20)
         out[name] = {"ms": ms, "bytes": nbytes, "GB_s": nbytes / ms / 1e6, "frac_of_8TBs": nbytes / ms / 1e6 / 8000.0}
@@ -255,9 +257,11 @@ def main():
     ap.add_argument("--accuracy", action="store_true",
                     help="full run of --workload in both forms: FP32 error vs FP64 (max maps, runout, deposit)")
     ap.add_argument("--rates", default="2,5,10")
-    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--slots", type=int, default=128, help="serving: batch slots")
     ap.add_argument("--capacity", type=int, default=24_000_000, help="serving: device particle capacity")
+    ap.add_argument("--conditioning", default="",
+                    help="comma-separated ensemble scenario indices: FP32 error vs FP64 sensitivity of h_max")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 5 plain 1 GiB device copies (calibration of the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -279,6 +283,8 @@ def main():
         return serve_main(args)
     if args.accuracy:
         return accuracy_main(args)
+    if args.conditioning:
+        return conditioning_main(args)
 
     t0 = time.time()
     cfg = getattr(configs, args.workload)()
@@ -425,30 +431,79 @@ def accuracy_main(args):
     print(json.dumps(out), flush=True)
 
 
+def conditioning_main(args):
+    """Conditioning of the max-height map for chosen ensemble scenarios: the
+    FP32 form's h_max L1 error against the FP64 form beside the change of the
+    FP64 result when every initial position moves by one FP32 offset ulp of a
+    cell (dx * 2^-24, random sign per particle and axis)."""
+    import numpy as np
+    import slabpro
+    from slabpro import abi, configs
+    dem = configs.c1_dem()
+    rows = []
+    for i in [int(v) for v in args.conditioning.split(",")]:
+        s = configs.c3_scenario(i)
+        cfg = dict(s)
+        cfg["z"] = dem
+        maps = {}
+        for tag, prec, pert in (("fp64", abi.FP64, False), ("fp64_moved", abi.FP64, True), ("fp32", abi.FP32, False)):
+            c = dict(cfg)
+            if pert:
+                g = np.random.default_rng(7)
+                u = cfg["dx"] * 2.0 ** -24
+                c["x"] = np.asarray(cfg["x"]) + g.choice([-1.0, 1.0], len(cfg["x"])) * u
+                c["y"] = np.asarray(cfg["y"]) + g.choice([-1.0, 1.0], len(cfg["y"])) * u
+            r = slabpro.make_run(c, precision=prec)
+            r.prime()
+            r.run()
+            maps[tag] = r.max_maps()[0]
+            r.close()
+        l1 = lambda a, b: float(np.abs(a - b).sum() / np.abs(b).sum())
+        row = {"scenario": i, "particles": len(s["x"]), "friction": s["friction"],
+               "fp32_vs_fp64": l1(maps["fp32"], maps["fp64"]), "fp64_moved_vs_fp64": l1(maps["fp64_moved"], maps["fp64"])}
+        log(row)
+        rows.append(row)
+    print(json.dumps({"metric": "h_max L1: FP32 error and FP64 sensitivity to one FP32 offset ulp", "rows": rows}),
+          flush=True)
+
+
 def serve_main(args):
     """C4: open-loop Poisson arrivals drawn from the ensemble distribution on the
-    2000x2000 DEM; continuous batching with admission by device footprint."""
+    2000x2000 DEM; continuous batching with admission by device footprint
+    (free slots, particle capacity, reserved active tiles), beside the plain
+    FP64 one-at-a-time rate on a size-stratified sample of the same requests."""
     import numpy as np
-    from slabpro import configs, ensemble
+    from slabpro import abi, configs, ensemble
     dem = configs.c1_dem()
     res = {}
     n = args.requests
     t0 = time.time()
     reqs = [configs.c3_scenario(1000 + k) for k in range(n)]
-    log(f"serve: {n} requests generated in {time.time() - t0:.1f}s, {sum(len(r['x']) for r in reqs)} particles")
+    sizes = np.array([len(r["x"]) for r in reqs])
+    log(f"serve: {n} requests generated in {time.time() - t0:.1f}s, {sizes.sum()} particles")
     for lam in [float(v) for v in args.rates.split(",")]:
         arr, _ = configs.c4_requests(n, lam, seed_index=int(lam * 10))
         r = ensemble.serve(dem, 5.0, reqs, arr, max_slots=args.slots, capacity=args.capacity, progress=log)
-        res[f"lambda_{lam:g}"] = r
+        res[f"lambda_{lam:g}"] = {k: v for k, v in r.items()
+                                  if k not in ("latency_s", "co_running", "admit_call", "finish_call", "results")}
         log(f"lambda {lam:g}/s: p50 {r['p50_s']:.2f}s p95 {r['p95_s']:.2f}s p99 {r['p99_s']:.2f}s, "
-            f"{r['scenarios_per_hour']:.0f} scen/h over {r['wall_s']:.1f}s")
+            f"{r['scenarios_per_hour']:.0f} scen/h over {r['wall_s']:.1f}s, rejected {len(r['rejected'])}")
+    k = max(1, min(args.fp64_sample, n))
+    rank = np.argsort(sizes, kind="stable")
+    sample = [int(rank[int((q + 0.5) * n / k)]) for q in range(k)]
+    one64 = ensemble.run_one_at_a_time(dem, 5.0, [reqs[i] for i in sample], abi.FP64, progress=log)
+    log(f"fp64 one-at-a-time on {k} requests: {one64['scenarios_per_hour']:.0f} scen/h")
+    for v in res.values():
+        v["vs_fp64_one_at_a_time"] = v["scenarios_per_hour"] / one64["scenarios_per_hour"]
     first = res[next(iter(res))]
     out = {"metric": "p99 completion latency (serving)", "value": first["p99_s"], "unit": "s", "n_gpus": 1,
            "higher_is_better": False, "dtype": "f32", "data": "synthetic (seeded requests, Poisson arrivals)",
-           "config": {"workload": f"c4: {n} of the 2,000 seeded requests per rate, rates {args.rates}/s, "
-                                  f"c3 scenario distribution on the 2000x2000 DEM, {args.slots} slots, "
+           "config": {"workload": f"c4: {n} seeded requests per rate, rates {args.rates}/s, c3 scenario "
+                                  f"distribution on the 2000x2000 DEM, {args.slots} slots, "
                                   f"{args.capacity / 1e6:g}M-particle capacity"},
-           "per_rate": res}
+           "per_rate": res,
+           "fp64_one_at_a_time": {"scenarios_per_hour": one64["scenarios_per_hour"],
+                                  "sample": f"{k} requests, one per size stratum (indices {sample})"}}
     print(json.dumps(out), flush=True)
 
 

@@ -155,7 +155,9 @@ slab_status slab_device_info(int32_t out[3]);
 /* Streaming kernels of known size on the current device, the practical HBM
  * ceiling for the step's access patterns.  kind 0: copy n bytes (float4 reads
  * and writes); 1: read n bytes; 2: write n bytes; 3: the particle streams of
- * G2P for n particles (28 B read, 32 B written in place, one per thread).
+ * G2P for n particles (28 B read, 32 B written in place, one per thread);
+ * 4 and 5: the copy and the write with non-temporal accesses and 128 B in
+ * flight per thread.
  * *ms = mean kernel time of iters launches (HIP events).  No reference
  * counterpart: measurement only. */
 slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters, double* ms);

@@ -60,6 +60,38 @@ __global__ void __launch_bounds__(WG) k_probe_write(float4* __restrict__ b, size
     }
 }
 
+// The same copy and write with non-temporal stores (the written lines are
+// not kept in the caches) and PVN float4 per thread.
+constexpr int PVN = 8;
+typedef float f4v __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(WG) k_probe_copy_nt(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
+    const f4v* __restrict__ av = reinterpret_cast<const f4v*>(a);
+    f4v* __restrict__ bv = reinterpret_cast<f4v*>(b);
+    const size_t base = ((size_t)blockIdx.x * WG * PVN) + threadIdx.x;
+    f4v v[PVN];
+#pragma unroll
+    for (int k = 0; k < PVN; ++k) {
+        const size_t i = base + (size_t)k * WG;
+        v[k] = i < n4 ? __builtin_nontemporal_load(&av[i]) : (f4v){0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int k = 0; k < PVN; ++k) {
+        const size_t i = base + (size_t)k * WG;
+        if (i < n4) __builtin_nontemporal_store(v[k], &bv[i]);
+    }
+}
+
+__global__ void __launch_bounds__(WG) k_probe_write_nt(float4* __restrict__ b, size_t n4, float val) {
+    f4v* __restrict__ bv = reinterpret_cast<f4v*>(b);
+    const size_t base = ((size_t)blockIdx.x * WG * PVN) + threadIdx.x;
+    const f4v q = {val, val, val, val};
+#pragma unroll
+    for (int k = 0; k < PVN; ++k) {
+        const size_t i = base + (size_t)k * WG;
+        if (i < n4) __builtin_nontemporal_store(q, &bv[i]);
+    }
+}
+
 // G2P's particle streams without its grid gathers: read cell, (ox, oy),
 // (vx, vy), h, slot (28 B); write cell, (ox, oy), (vx, vy), vz, h, z in place
 // (32 B).  The arithmetic is a stand-in that keeps every value live.
@@ -96,8 +128,10 @@ bool palloc(std::vector<void*>& l, T** p, size_t n) {
 using namespace slab;
 
 extern "C" slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters, double* ms_out) {
-    if (kind < 0 || kind > 3 || n < 1 || iters < 1 || !ms_out) return set_error(SLAB_EINVAL, "bad probe arguments");
-    if (kind < 3 && (n % 16 != 0 || n > ((int64_t)1 << 34))) return set_error(SLAB_EINVAL, "bytes: multiple of 16, <= 16 GiB");
+    if (kind < 0 || kind > 5 || n < 1 || iters < 1 || !ms_out) return set_error(SLAB_EINVAL, "bad probe arguments");
+    const bool bytes_kind = kind != 3;
+    if (bytes_kind && (n % 16 != 0 || n > ((int64_t)1 << 34)))
+        return set_error(SLAB_EINVAL, "bytes: multiple of 16, <= 16 GiB");
     if (kind == 3 && n > ((int64_t)1 << 30)) return set_error(SLAB_EINVAL, "particles: <= 2^30");
     std::vector<void*> l;
     hipStream_t st = nullptr;
@@ -116,13 +150,14 @@ extern "C" slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters,
         int32_t *cell = nullptr, *slot = nullptr;
         float2 *o = nullptr, *v = nullptr;
         float *h = nullptr, *vz = nullptr, *z = nullptr;
-        const size_t n4 = kind < 3 ? (size_t)n / 16 : 0;
-        const unsigned nb = kind < 3 ? (unsigned)((n4 + (size_t)WG * PV - 1) / ((size_t)WG * PV))
-                                     : (unsigned)((n + WG - 1) / WG);
+        const size_t n4 = bytes_kind ? (size_t)n / 16 : 0;
+        const int pv = kind >= 4 ? PVN : PV;
+        const unsigned nb = bytes_kind ? (unsigned)((n4 + (size_t)WG * pv - 1) / ((size_t)WG * pv))
+                                       : (unsigned)((n + WG - 1) / WG);
         bool ok = true;
-        if (kind == 0) ok = palloc(l, &a, n4) && palloc(l, &b, n4);
+        if (kind == 0 || kind == 4) ok = palloc(l, &a, n4) && palloc(l, &b, n4);
         if (kind == 1) ok = palloc(l, &a, n4) && palloc(l, &red, nb);
-        if (kind == 2) ok = palloc(l, &b, n4);
+        if (kind == 2 || kind == 5) ok = palloc(l, &b, n4);
         if (kind == 3)
             ok = palloc(l, &cell, n) && palloc(l, &slot, n) && palloc(l, &o, n) && palloc(l, &v, n) &&
                  palloc(l, &h, n) && palloc(l, &vz, n) && palloc(l, &z, n);
@@ -148,6 +183,8 @@ extern "C" slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters,
             if (kind == 1) k_probe_read<<<nb, WG, 0, st>>>(a, red, n4);
             if (kind == 2) k_probe_write<<<nb, WG, 0, st>>>(b, n4, 1.0f);
             if (kind == 3) k_probe_particles<<<nb, WG, 0, st>>>(cell, o, v, h, slot, vz, z, (int)n);
+            if (kind == 4) k_probe_copy_nt<<<nb, WG, 0, st>>>(a, b, n4);
+            if (kind == 5) k_probe_write_nt<<<nb, WG, 0, st>>>(b, n4, 1.0f);
         };
         launch(); // warm-up
         double tot = 0.0;

@@ -173,13 +173,13 @@ def device_info():
     return {"device": out[0], "cus": out[1], "gfx950": bool(out[2])}
 
 
-PROBE_COPY, PROBE_READ, PROBE_WRITE, PROBE_PARTICLES = range(4)
+PROBE_COPY, PROBE_READ, PROBE_WRITE, PROBE_PARTICLES, PROBE_COPY_NT, PROBE_WRITE_NT = range(6)
 
 
 def stream_probe(kind, n, iters=20):
     """Mean kernel time (ms) of a streaming kernel of known size: PROBE_COPY /
-    _READ / _WRITE over n bytes, PROBE_PARTICLES over n particles (28 B read,
-    32 B written in place)."""
+    _READ / _WRITE / _COPY_NT / _WRITE_NT over n bytes, PROBE_PARTICLES over n
+    particles (28 B read, 32 B written in place)."""
     ms = ctypes.c_double(0.0)
     _check(lib().slab_stream_probe(int(kind), int(n), int(iters), ctypes.byref(ms)))
     return ms.value

@@ -227,19 +227,24 @@ slab_status slab_batch_download_particles_noalloc(slab_batch* b, int32_t slot, d
  * for the next admission.  The other slots are not advanced. */
 slab_status slab_batch_release(slab_batch* b, int32_t slot);
 /* out[0] = device bytes per particle, out[1] = device bytes per slot (its
- * h_max / v_max2 maps), out[2] = free particle capacity, out[3] = free slots.
+ * control block and tile map; its max maps are counted in tiles),
+ * out[2] = free particle capacity, out[3] = free slots.
  * With slab_batch_tiles this is the batch's device footprint: particles +
  * slots + active tiles. */
 slab_status slab_batch_footprint(slab_batch* b, int64_t out[4]);
 /* Active tiles of the batch.  The grid velocity of a step is kept only for its
  * grid tiles (16 x 16 nodes holding particles, and their right / upper /
  * diagonal neighbours), in a pool of fixed capacity; P2G keeps a partial
- * region per particle tile.  out[0] = grid-tile capacity, out[1] = grid tiles
- * of the last step, out[2] = particle-tile capacity, out[3] = particle tiles of
- * the last step, out[4] = device bytes per grid tile, out[5] = device bytes
- * per particle tile.  No reference counterpart (the reference sizes one
- * bounding-box window per run, src/simulation.cpp:584-642). */
-slab_status slab_batch_tiles(slab_batch* b, int64_t out[6]);
+ * region per particle tile; each slot's h_max / v_max2 maps are kept only for
+ * the tiles its flow has touched, in a third pool (tiles return to it when the
+ * slot is released).  out[0] = grid-tile capacity, out[1] = grid tiles of the
+ * last step, out[2] = particle-tile capacity, out[3] = particle tiles of the
+ * last step, out[4] = device bytes per grid tile, out[5] = device bytes per
+ * particle tile, out[6] = max-map tile capacity, out[7] = max-map tiles in
+ * use, out[8] = device bytes per max-map tile.  No reference counterpart (the
+ * reference sizes one bounding-box window per run, src/simulation.cpp:584-642,
+ * and one full-terrain pair of max maps). */
+slab_status slab_batch_tiles(slab_batch* b, int64_t out[9]);
 /* Admission reserves `factor` times the particle and grid tiles of a release
  * (default 4) for its flow to spread; slab_batch_admit refuses a scenario
  * whose reservation does not fit beside the live ones.  factor = 0 admits

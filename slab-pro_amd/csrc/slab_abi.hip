@@ -302,7 +302,7 @@ Slot slot_params(const slab_params* p, const slab_friction* f) {
 // later call on that engine SLAB_ESTATE, anything else a device error.
 slab_status eng_fail(const std::string& msg, slab_status st = SLAB_EDEVICE) {
     if (msg == kStoppedByError) st = SLAB_ESTATE;
-    else if (msg == "active-tile capacity exceeded") st = SLAB_EDOMAIN;
+    else if (msg.find("capacity exceeded") != std::string::npos) st = SLAB_EDOMAIN;
     return fail(st, msg);
 }
 
@@ -981,7 +981,7 @@ slab_status slab_batch_footprint(slab_batch* b, int64_t out[4]) {
     return SLAB_OK;
 }
 
-slab_status slab_batch_tiles(slab_batch* b, int64_t out[6]) {
+slab_status slab_batch_tiles(slab_batch* b, int64_t out[9]) {
     if (!b || !out) return fail(SLAB_EINVAL, "null argument");
     std::string why;
     if (!engine_pull(b->eng, &why)) return eng_fail(why);

@@ -498,7 +498,8 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
                                                  int nblk, int merged, int* __restrict__ tile_begin,
                                                  int* __restrict__ ptiles, int* __restrict__ gtiles,
                                                  int* __restrict__ pstamp, int* __restrict__ gstamp,
-                                                 int* __restrict__ tile_pos, int* __restrict__ gtile_pos) {
+                                                 int* __restrict__ tile_pos, int* __restrict__ gtile_pos,
+                                                 int* __restrict__ hm_map, const int* __restrict__ hm_stack) {
     if (c->stop) return;
     const bool idle = scan_block_idle(c, slots, ntiles);
     if (idle && blockIdx.x != 0) return; // block 0 writes the totals even when idle
@@ -537,6 +538,17 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
             gtiles[run[2]] = t;
             gtile_pos[t] = run[2];
             gstamp[t] = ep;
+            // a batch's max maps live in a pool of the tiles its slots have
+            // touched: a grid tile takes a (zeroed) pool tile the first time
+            if (hm_map && hm_map[t] < 0) {
+                const int q = atomicAdd(&c->hm_top, 1);
+                if (q < c->hm_cap) {
+                    hm_map[t] = hm_stack[q];
+                } else {
+                    c->err = 8;
+                    c->stop = STOP_ERROR;
+                }
+            }
         }
         for (int q = 0; q < 3; ++q) run[q] += f[k][q];
     }
@@ -1036,7 +1048,9 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
 // j).  A batch (POOL) keeps it in a pool of the step's grid tiles only: tile
 // w of the grid-tile list holds its 256 nodes at w*256 + (i&15)*16 + (j&15),
 // and gtile_pos maps a tile id to its list position, so the batch's memory
-// grows with its active tiles instead of slots x terrain.
+// grows with its active tiles instead of slots x terrain.  A batch's max maps
+// h_max, v_max2 live in a second pool, of the tiles its slots have touched:
+// hm_map maps a tile id to its pool tile (256 nodes, same local order).
 template <bool POOL>
 __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ slots,
                                               const int* __restrict__ gtiles, const int* __restrict__ pstamp,
@@ -1045,7 +1059,7 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
                                               const float* __restrict__ tcth, float* __restrict__ h_max,
                                               float* __restrict__ v_max2, float4* __restrict__ vf,
                                               float* __restrict__ out_m, float* __restrict__ out_h,
-                                              float* __restrict__ out_vz) {
+                                              float* __restrict__ out_vz, const int* __restrict__ hm_map) {
     // multiply-adds fused per source expression, so the two layouts (which
     // differ only in addressing) round the same way
 #pragma clang fp contract(on)
@@ -1138,9 +1152,10 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
             }
             flx = nvx - vx;
             fly = nvy - vy;
-            if (h_max[k] < hh) h_max[k] = hh;
+            const size_t km = POOL ? (size_t)hm_map[t] * TILE_CELLS + threadIdx.x : k;
+            if (h_max[km] < hh) h_max[km] = hh;
             const float v2 = nvx * nvx + nvy * nvy;
-            if (v_max2[k] < v2) v_max2[k] = v2;
+            if (v_max2[km] < v2) v_max2[km] = v2;
         }
         vf[POOL ? (size_t)w * TILE_CELLS + threadIdx.x : k] = make_float4(nvx, nvy, flx, fly);
         if (out_m && S.out_step) {
@@ -1459,6 +1474,44 @@ __global__ void __launch_bounds__(WG) k_ediag(const ECtl* c, PB32 A, PB32 B, int
         for (int q = 0; q < 6; ++q) part[6 * blockIdx.x + q] = sh[q][0];
 }
 
+// Returns a released slot's max-map pool tiles, zeroed, to the free stack
+// (one workgroup per terrain tile of the slot; no allocation runs meanwhile).
+__global__ void __launch_bounds__(WG) k_hm_release(ECtl* c, int slot, int* __restrict__ hm_map,
+                                                   int* __restrict__ hm_stack, float* __restrict__ h_max,
+                                                   float* __restrict__ v_max2) {
+    for (int local = blockIdx.x; local < c->ntd; local += gridDim.x) {
+        const int t = slot * c->ntd + local;
+        const int idx = hm_map[t];
+        if (idx < 0) continue;
+        h_max[(size_t)idx * TILE_CELLS + threadIdx.x] = 0.f;
+        v_max2[(size_t)idx * TILE_CELLS + threadIdx.x] = 0.f;
+        if (threadIdx.x == 0) {
+            const int k = atomicSub(&c->hm_top, 1) - 1;
+            hm_stack[k] = idx;
+            hm_map[t] = -1;
+        }
+    }
+}
+
+// One slot's max maps from the pool as dense terrain arrays (0 where the slot
+// never reached).
+__global__ void k_hm_dense(const ECtl* c, int slot, const int* __restrict__ hm_map, const float* __restrict__ h_max,
+                           const float* __restrict__ v_max2, float* __restrict__ oh, float* __restrict__ ov) {
+    const size_t nn = (size_t)c->nx * c->ny;
+    const size_t q = (size_t)blockIdx.x * WG + threadIdx.x;
+    if (q >= nn) return;
+    const int i = (int)(q / c->ny), j = (int)(q - (size_t)i * c->ny);
+    const int idx = hm_map[slot * c->ntd + (i >> 4) * c->nty + (j >> 4)];
+    float h = 0.f, v = 0.f;
+    if (idx >= 0) {
+        const size_t k = (size_t)idx * TILE_CELLS + (((i & 15) << 4) | (j & 15));
+        h = h_max[k];
+        v = v_max2[k];
+    }
+    oh[q] = h;
+    ov[q] = v;
+}
+
 // ============================================================== host code
 
 namespace {
@@ -1536,6 +1589,9 @@ struct Engine32 {
     // times those of its release
     int tile_growth = 4;
     std::vector<int64_t> res_ptiles, res_gtiles; // per slot: tiles reserved at admission
+    // batches keep the max maps in a pool of touched tiles (single runs: dense)
+    int hm_cap = 0;
+    int *hm_map = nullptr, *hm_stack = nullptr;
 };
 
 namespace {
@@ -1609,7 +1665,7 @@ void enqueue_estep(Engine32* e, int step) {
     tk(e, 3, step, [&] {
         k_eoffsets<<<e->nblk, WG, 0, s>>>(c, e->d_slots, e->tile_count, e->ntiles, e->boff, e->bsum, e->nblk, merged,
                                            e->tile_begin, e->ptiles, e->gtiles, e->pstamp, e->gstamp, e->tile_pos,
-                                           e->gtile_pos);
+                                           e->gtile_pos, e->hm_map, e->hm_stack);
     });
     tk(e, 4, step, [&] {
         const unsigned nb = (unsigned)std::max<int64_t>(1, (e->launch_n + WG * SC_PER - 1) / (WG * SC_PER));
@@ -1626,10 +1682,10 @@ void enqueue_estep(Engine32* e, int step) {
     tk(e, 7, step, [&] {
         if (e->pool_vf)
             k_egrid<true><<<tb, WG, 0, s>>>(c, e->d_slots, e->gtiles, e->pstamp, e->tile_pos, e->tpart, e->tgx, e->tgy,
-                                            e->tcth, e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz);
+                                            e->tcth, e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz, e->hm_map);
         else
             k_egrid<false><<<tb, WG, 0, s>>>(c, e->d_slots, e->gtiles, e->pstamp, e->tile_pos, e->tpart, e->tgx, e->tgy,
-                                             e->tcth, e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz);
+                                             e->tcth, e->hmax, e->vmax2, e->vf, e->om, e->oh, e->ovz, nullptr);
     });
     tk(e, 8, step, [&] {
         const unsigned gb = xcd_grid((e->launch_n + WG - 1) / WG);
@@ -1737,7 +1793,9 @@ int64_t engine_particle_bytes() {
 int64_t engine_slot_bytes(const Engine32* e) {
     // max maps (h_max, v_max2) per node; a single run also keeps its grid
     // velocity and output fields dense
-    return (int64_t)e->nn * (8 + (e->single ? 16 + 12 : 0)) + VMAX_SHARDS * 8 + (int64_t)sizeof(Slot);
+    if (e->single) return (int64_t)e->nn * (8 + 16 + 12) + VMAX_SHARDS * 8 + (int64_t)sizeof(Slot);
+    // a batch slot: its control block, dt shards and tile -> max-map pool index
+    return (int64_t)e->ntd * 4 + VMAX_SHARDS * 8 + (int64_t)sizeof(Slot);
 }
 
 bool engine_set_tile_growth(Engine32* e, int factor, std::string* err) {
@@ -1749,13 +1807,16 @@ bool engine_set_tile_growth(Engine32* e, int factor, std::string* err) {
     return true;
 }
 
-void engine_tiles(const Engine32* e, int64_t out[6]) {
+void engine_tiles(const Engine32* e, int64_t out[9]) {
     out[0] = e->gtile_cap;
     out[1] = e->h_c->n_gtiles;
     out[2] = e->tile_cap;
     out[3] = e->h_c->n_ptiles;
     out[4] = e->pool_vf ? (int64_t)TILE_CELLS * (int64_t)sizeof(float4) : 0;
     out[5] = (int64_t)CELL_STRIDE * 4 + (int64_t)P2G_FIELDS * REGION_NODES * 4;
+    out[6] = e->hm_cap;
+    out[7] = e->single ? 0 : e->h_c->hm_top;
+    out[8] = e->single ? 0 : (int64_t)TILE_CELLS * 8;
 }
 
 // Device address of `scratch` on each device, taken once per device with that
@@ -1823,6 +1884,13 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
         e->tile_cap = std::max(1, std::min(e->tile_cap, std::atoi(v)));
         if (!single) e->gtile_cap = std::max(1, std::min(e->gtile_cap, 2 * e->tile_cap));
     }
+    // a batch's max maps: a pool of touched tiles, sized like the grid-tile
+    // pool but at least 2^20 tiles (2 GB), at most every terrain tile of every slot
+    if (!single) {
+        const int64_t live = (int64_t)nslots * e->ntd;
+        e->hm_cap = (int)std::min<int64_t>(live, std::max<int64_t>(e->gtile_cap, (int64_t)1 << 20));
+        if (const char* v = std::getenv("SLAB_MAXMAP_TILE_CAP")) e->hm_cap = std::max(1, std::min(e->hm_cap, std::atoi(v)));
+    }
     std::string why;
     auto fail = [&](const std::string& m) -> Engine32* {
         if (err) *err = m;
@@ -1841,7 +1909,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     bool ok = ealloc(e->allocs, &e->d_c, 1, &why) && ealloc(e->allocs, &e->d_slots, nslots, &why) &&
               ealloc(e->allocs, &e->tz, e->nn, &why) && ealloc(e->allocs, &e->tgx, e->nn, &why) &&
               ealloc(e->allocs, &e->tgy, e->nn, &why) && ealloc(e->allocs, &e->tcth, e->nn, &why) &&
-              ealloc(e->allocs, &e->hmax, sn, &why) && ealloc(e->allocs, &e->vmax2, sn, &why) &&
+              ealloc(e->allocs, &e->hmax, single ? sn : (size_t)e->hm_cap * TILE_CELLS, &why) &&
+              ealloc(e->allocs, &e->vmax2, single ? sn : (size_t)e->hm_cap * TILE_CELLS, &why) &&
               ealloc(e->allocs, &e->vf, e->pool_vf ? (size_t)e->gtile_cap * TILE_CELLS : sn, &why) &&
               ealloc(e->allocs, &e->gtile_pos, e->ntiles, &why) &&
               ealloc(e->allocs, &e->vmx, (size_t)nslots * VMAX_SHARDS, &why) &&
@@ -1866,7 +1935,18 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     }
     ok = ok && ealloc(e->pallocs, &e->key, max_particles, &why) && ealloc(e->pallocs, &e->idx, max_particles, &why) &&
          ealloc(e->pallocs, &e->tmp, max_particles, &why);
+    if (ok && !single)
+        ok = ealloc(e->allocs, &e->hm_map, (size_t)nslots * e->ntd, &why) &&
+             ealloc(e->allocs, &e->hm_stack, (size_t)e->hm_cap, &why);
     if (!ok) return fail(why);
+    if (!single) {
+        std::vector<int32_t> st(e->hm_cap);
+        for (int q = 0; q < e->hm_cap; ++q) st[q] = q;
+        ok = hipMemsetAsync(e->hm_map, 0xff, (size_t)nslots * e->ntd * 4, e->stream) == hipSuccess &&
+             hipMemcpyAsync(e->hm_stack, st.data(), st.size() * 4, hipMemcpyHostToDevice, e->stream) == hipSuccess &&
+             hipStreamSynchronize(e->stream) == hipSuccess;
+        if (!ok) return fail("max-map pool set-up failed");
+    }
     std::vector<float> gx, gy, cth, zf(e->nn);
     terrain32(z, nx, ny, e->inv, gx, gy, cth);
     for (size_t k = 0; k < e->nn; ++k) zf[k] = (float)z[k];
@@ -1875,7 +1955,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
          hipMemcpyAsync(e->tgx, gx.data(), e->nn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(e->tgy, gy.data(), e->nn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(e->tcth, cth.data(), e->nn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
-         hipMemsetAsync(e->hmax, 0, sn * 4, s) == hipSuccess && hipMemsetAsync(e->vmax2, 0, sn * 4, s) == hipSuccess &&
+         hipMemsetAsync(e->hmax, 0, (single ? sn : (size_t)e->hm_cap * TILE_CELLS) * 4, s) == hipSuccess &&
+         hipMemsetAsync(e->vmax2, 0, (single ? sn : (size_t)e->hm_cap * TILE_CELLS) * 4, s) == hipSuccess &&
          hipMemsetAsync(e->vf, 0, (e->pool_vf ? (size_t)e->gtile_cap * TILE_CELLS : sn) * sizeof(float4), s) ==
              hipSuccess &&
          hipMemsetAsync(e->gtile_pos, 0, (size_t)e->ntiles * 4, s) == hipSuccess &&
@@ -1900,6 +1981,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     c.has_out = single ? 1 : 0;
     c.tile_cap = e->tile_cap;
     c.gtile_cap = e->gtile_cap;
+    c.hm_cap = e->hm_cap;
+    c.hm_top = 0;
     c.reorder_every = kReorderEvery;
     c.dx = dx;
     c.inv_dx = e->inv;
@@ -2042,7 +2125,9 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
         }
         const int64_t used_p = std::max<int64_t>(sum_p, e->h_c->n_ptiles);
         const int64_t used_g = std::max<int64_t>(sum_g, e->h_c->n_gtiles);
-        if (used_p + res_p > e->tile_cap || used_g + res_g > e->gtile_cap) {
+        // touched tiles keep their max maps until release: the same reservation
+        const int64_t used_h = std::max<int64_t>(sum_g, e->h_c->hm_top);
+        if (used_p + res_p > e->tile_cap || used_g + res_g > e->gtile_cap || used_h + res_g > e->hm_cap) {
             if (err) *err = "tile capacity exhausted";
             return -1;
         }
@@ -2063,8 +2148,9 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
     up(P.pid, pid.data());
     up(P.slot, sl.data());
     // per-slot grid state starts empty
-    ok = ok && hipMemsetAsync(e->hmax + (size_t)slot * e->nn, 0, e->nn * 4, s) == hipSuccess &&
-         hipMemsetAsync(e->vmax2 + (size_t)slot * e->nn, 0, e->nn * 4, s) == hipSuccess;
+    if (e->single) // a batch slot's pool tiles were zeroed when they were returned
+        ok = ok && hipMemsetAsync(e->hmax + (size_t)slot * e->nn, 0, e->nn * 4, s) == hipSuccess &&
+             hipMemsetAsync(e->vmax2 + (size_t)slot * e->nn, 0, e->nn * 4, s) == hipSuccess;
     if (!ok || !hip_ok(hipStreamSynchronize(s), "admit upload", err)) {
         if (err && err->empty()) *err = "admit upload failed";
         return -1;
@@ -2118,7 +2204,10 @@ bool engine_step(Engine32* e, int max_steps, std::string* err) {
     }
     for (int q = 0; q < e->nslots; ++q) e->admit_base[q] = -1; // storage was re-sorted
     if (e->h_c->stop == STOP_ERROR) {
-        if (err) *err = e->h_c->err == 7 ? "active-tile capacity exceeded" : "device error";
+        if (err)
+            *err = e->h_c->err == 7   ? "active-tile capacity exceeded"
+                   : e->h_c->err == 8 ? "max-map tile capacity exceeded"
+                                      : "device error";
         return false;
     }
     return true;
@@ -2136,6 +2225,14 @@ bool engine_release(Engine32* e, int slot, std::string* err) {
     e->res_ptiles[slot] = 0;
     e->res_gtiles[slot] = 0;
     if (!push_all(e, err)) return false;
+    if (!e->single) {
+        k_hm_release<<<(unsigned)std::min(e->ntd, 4096), WG, 0, e->stream>>>(e->d_c, slot, e->hm_map, e->hm_stack,
+                                                                            e->hmax, e->vmax2);
+        if (!hip_ok(hipGetLastError(), "release max maps", err)) return false;
+        // the kernel moved the pool's top on the device: refresh the host
+        // copy before the control block is written back below
+        if (!engine_pull(e, err)) return false;
+    }
     // Re-key every stored particle with the new statuses (the released slot's
     // particles get no key), then run one re-sort pass that only permutes:
     // storage is compact again and no slot advances.
@@ -2245,14 +2342,28 @@ bool engine_max_maps(Engine32* e, int slot, double* h_max, double* v_max2, std::
     std::vector<float> f(e->nn);
     double* dst[] = {h_max, v_max2};
     float* src[] = {e->hmax + (size_t)slot * e->nn, e->vmax2 + (size_t)slot * e->nn};
-    for (int k = 0; k < 2; ++k) {
-        if (!dst[k]) continue;
-        if (!hip_ok(hipMemcpyAsync(f.data(), src[k], e->nn * 4, hipMemcpyDeviceToHost, e->stream), "max maps", err) ||
-            !hip_ok(hipStreamSynchronize(e->stream), "sync", err))
+    std::vector<void*> tl;
+    if (!e->single) { // gather the slot's pool tiles into dense terrain arrays
+        float *dh = nullptr, *dv = nullptr;
+        if (!ealloc(tl, &dh, e->nn, err) || !ealloc(tl, &dv, e->nn, err)) {
+            efree(tl);
             return false;
-        for (size_t q = 0; q < e->nn; ++q) dst[k][q] = f[q];
+        }
+        k_hm_dense<<<(unsigned)((e->nn + WG - 1) / WG), WG, 0, e->stream>>>(e->d_c, slot, e->hm_map, e->hmax, e->vmax2,
+                                                                         dh, dv);
+        src[0] = dh;
+        src[1] = dv;
     }
-    return true;
+    bool ok = true;
+    for (int k = 0; k < 2 && ok; ++k) {
+        if (!dst[k]) continue;
+        ok = hip_ok(hipMemcpyAsync(f.data(), src[k], e->nn * 4, hipMemcpyDeviceToHost, e->stream), "max maps", err) &&
+             hip_ok(hipStreamSynchronize(e->stream), "sync", err);
+        if (ok)
+            for (size_t q = 0; q < e->nn; ++q) dst[k][q] = f[q];
+    }
+    efree(tl);
+    return ok;
 }
 
 bool engine_window(Engine32* e, int slot, int* i0, int* j0, int* wnx, int* wny, double* const dst[5],

@@ -61,6 +61,8 @@ struct ECtl {
     int32_t reorder_every; // storage re-sort period in steps (other steps update in place)
     int32_t pend_end;      // a started step still has to be ended (time, frames, buffer flip)
     int32_t gtile_cap;     // grid tiles the grid-velocity pool holds (batches; all tiles for a single run)
+    int32_t hm_cap;        // tiles the max-map pool holds (batches)
+    int32_t hm_top;        // max-map pool tiles in use (the free ones are hm_stack[hm_top..hm_cap))
     int32_t pad_;
     double dx, inv_dx;
 };
@@ -68,7 +70,7 @@ struct ECtl {
 struct Engine32;
 
 // Message of every call refused after a step stopped on an error.
-constexpr const char* kStoppedByError = "stopped by an earlier error (active-tile capacity exceeded); destroy it";
+constexpr const char* kStoppedByError = "stopped by an earlier error (a tile capacity was exceeded); destroy it";
 
 // Host-side engine API used by the C ABI (slab_run_* with SLAB_FP32, slab_batch_*).
 struct EngineResult {
@@ -105,8 +107,9 @@ int64_t engine_particle_bytes();                      // device bytes per partic
 int64_t engine_slot_bytes(const Engine32* e);         // device bytes per slot (max maps; dense grid of a single run)
 // Active-tile capacity and use: out = {grid-tile capacity, grid tiles of the
 // last step, particle-tile capacity, particle tiles of the last step,
-// device bytes per grid tile, device bytes per particle tile}.
-void engine_tiles(const Engine32* e, int64_t out[6]);
+// device bytes per grid tile, device bytes per particle tile, max-map tile
+// capacity, max-map tiles in use, device bytes per max-map tile}.
+void engine_tiles(const Engine32* e, int64_t out[9]);
 // Admission reserves this many times a release's tiles for its flow to spread.
 bool engine_set_tile_growth(Engine32* e, int factor, std::string* err);
 int64_t engine_free_particles(const Engine32* e);     // capacity left for admissions

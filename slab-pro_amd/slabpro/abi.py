@@ -395,10 +395,11 @@ class Batch:
                 "free_slots": out[3]}
 
     def tiles(self):
-        out = (ctypes.c_int64 * 6)()
+        out = (ctypes.c_int64 * 9)()
         _check(lib().slab_batch_tiles(self.h, out))
         return {"grid_tile_capacity": out[0], "grid_tiles": out[1], "particle_tile_capacity": out[2],
-                "particle_tiles": out[3], "bytes_per_grid_tile": out[4], "bytes_per_particle_tile": out[5]}
+                "particle_tiles": out[3], "bytes_per_grid_tile": out[4], "bytes_per_particle_tile": out[5],
+                "max_map_tile_capacity": out[6], "max_map_tiles": out[7], "bytes_per_max_map_tile": out[8]}
 
     def set_tile_growth(self, factor):
         _check(lib().slab_batch_set_tile_growth(self.h, int(factor)))

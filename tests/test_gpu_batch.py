@@ -363,3 +363,39 @@ def test_admission_by_active_tiles(monkeypatch):
         assert np.array_equal(hm, ref[0][0]) and np.array_equal(vm, ref[0][1])
         assert np.array_equal(p["x"], ref[1]["x"]) and np.array_equal(p["h"], ref[1]["h"])
     b.close()
+
+
+def test_max_map_tiles_return_to_the_pool():
+    # A batch keeps each slot's max maps only on the tiles its flow touched.
+    # Releasing a slot (running or done) returns its tiles, zeroed; a scenario
+    # admitted afterwards onto reused tiles gets the same maps as alone.
+    from slabpro import abi
+    z, dx = _terrain()
+    scens = [_scenario(70 + k, FRICS[k % 3]) for k in range(3)]
+    b = abi.Batch(z, dx, 4, 200000)
+    slots = []
+    for part, fr in scens:
+        s = b.admit(_params(), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
+        assert s is not None
+        slots.append(s)
+    b.step(20)
+    t = b.tiles()
+    print("tiles", t, flush=True)
+    assert t["bytes_per_max_map_tile"] == 256 * 8 and 0 < t["max_map_tiles"] <= t["max_map_tile_capacity"]
+    used = t["max_map_tiles"]
+    b.release(slots[0])  # cancelled while running
+    assert b.tiles()["max_map_tiles"] < used
+    while b.step(64) > 0:
+        pass
+    for s in slots[1:]:
+        b.release(s)
+    assert b.tiles()["max_map_tiles"] == 0
+    part, fr = scens[0]
+    s = b.admit(_params(), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
+    while b.step(64) > 0:
+        pass
+    hm, vm = b.max_maps(s)
+    (ref,) = _run_batch([scens[0]])
+    assert np.array_equal(hm, ref[0][0]) and np.array_equal(vm, ref[0][1])
+    assert np.array_equal(b.particles(s)["x"], ref[1]["x"])
+    b.close()

@@ -84,22 +84,27 @@ __device__ __forceinline__ bool cell_out(const ECtl* c, int ci, int cj) {
     return ci < 1 || ci > c->nx - 3 || cj < 1 || cj > c->ny - 3;
 }
 
-// Key of a particle: live (slot, tile, cell), parking tile of its slot (the
-// slot finished, or the particle flowed out), or -1 (released slot: the
-// particle drops out of the next sort).
+// Sort key of a particle: its tile -- live (slot, tile), the parking tile of
+// its slot (the slot finished, or the particle flowed out), or -1 (released
+// slot: the particle drops out of the next sort) -- and, in *cb, its cell
+// in the tile (the stencil base, (ib & 15) << 4 | (jb & 15)).  Tile ids use the
+// whole int32 range, so the number of slots on a terrain is bounded by
+// slots x terrain tiles < 2^31.
 __device__ __forceinline__ int ekey(const ECtl* c, int st, int slot, bool out, int cell, float ox, float oy,
-                                    int* tile) {
+                                    int* tile, int* cb) {
+    *cb = 0;
     if (slot_active(st) && !out) {
         const int ci = cell >> 16, cj = cell & 0xffff;
         const int ib = ebase(ci, ox, c->nx - 3), jb = ebase(cj, oy, c->ny - 3);
         const int t = slot * c->ntd + (ib >> 4) * c->nty + (jb >> 4);
         *tile = t;
-        return t * TILE_CELLS + (((ib & 15) << 4) | (jb & 15));
+        *cb = ((ib & 15) << 4) | (jb & 15);
+        return t;
     }
     if (slot_active(st) || st == SLOT_DONE || st == SLOT_FAILED) {
         const int t = c->nslots * c->ntd + slot;
         *tile = t;
-        return t * TILE_CELLS;
+        return t;
     }
     *tile = 0;
     return -1;
@@ -330,18 +335,22 @@ __global__ void __launch_bounds__(WG) k_eend(ECtl* c, Slot* __restrict__ slots) 
 // --- keys of admitted particles ---------------------------------------------
 
 __global__ void __launch_bounds__(WG) k_ekeys(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B, int s0,
-                                              int s1, int* __restrict__ key, int* __restrict__ tile_count) {
+                                              int s1, int* __restrict__ key, uint8_t* __restrict__ kcell,
+                                              int* __restrict__ tile_count) {
     const PB32 P = ecur(c, A, B);
     const int s = s0 + blockIdx.x * WG + threadIdx.x;
     const bool in = s < s1;
-    int t = 0, k = -1;
+    int t = 0, k = -1, cb = 0;
     if (in) {
         const int raw = P.slot[s], sl = raw & SLOT_MASK;
         const float2 o = P.o[s];
-        k = ekey(c, slots[sl].status, sl, (raw & SLOT_OUT) != 0, P.cell[s], o.x, o.y, &t);
+        k = ekey(c, slots[sl].status, sl, (raw & SLOT_OUT) != 0, P.cell[s], o.x, o.y, &t, &cb);
     }
     wave_count_any(t, in && k >= 0, tile_count);
-    if (in) key[s] = k;
+    if (in) {
+        key[s] = k;
+        kcell[s] = (uint8_t)cb;
+    }
 }
 
 // Initial particle z, src/simulation.cpp:455-465 (4x4 quadratic weights).
@@ -577,7 +586,7 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
         const int s = s0 + 64 * j;
         const int k = s < n ? key[s] : -1;
         valid[j] = k >= 0;
-        t[j] = valid[j] ? k >> 8 : 0;
+        t[j] = valid[j] ? k : 0;
     }
 #pragma unroll
     for (int j = 0; j < SC_PER; ++j) {
@@ -643,7 +652,7 @@ __device__ __forceinline__ int pow2_ceil(int v) {
     return p;
 }
 
-__global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int* __restrict__ key,
+__global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uint8_t* __restrict__ kcell,
                                               const int* __restrict__ ptiles, int* __restrict__ tile_count,
                                               int* __restrict__ cursor, const int* __restrict__ tile_begin,
                                               const int* __restrict__ tmp,
@@ -674,7 +683,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
         for (int j = 0; j < ES_GROUP; ++j) sv0[j] = tmp[b + min((int)threadIdx.x + j * WG, n - 1)];
 #pragma unroll
         for (int j = 0; j < ES_GROUP; ++j) {
-            kc0[j] = key[sv0[j]];
+            kc0[j] = kcell[sv0[j]];
             pd0[j] = pid[sv0[j]];
         }
         if (threadIdx.x == 0) s_cmax = 0;
@@ -717,7 +726,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
                             sv[j] = tmp[b + min((int)threadIdx.x + (k0 + j) * WG, n - 1)];
 #pragma unroll
                         for (int j = 0; j < ES_GROUP; ++j) {
-                            kc[j] = key[sv[j]];
+                            kc[j] = kcell[sv[j]];
                             pd[j] = pid[sv[j]];
                         }
                     }
@@ -726,7 +735,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
                         const int r = threadIdx.x + (k0 + j) * WG;
                         svr[k0 + j] = sv[j];
                         if (r < n) {
-                            const int cc = kc[j] & 255;
+                            const int cc = kc[j];
                             s_cell[r] = (unsigned char)cc;
                             s_pid[r] = pd[j];
                             atomicAdd(&s_hist[cc], 1);
@@ -745,11 +754,11 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
                     for (int j = 0; j < ES_GROUP; ++j) kc[j] = kc0[j];
                 } else {
 #pragma unroll
-                    for (int j = 0; j < ES_GROUP; ++j) kc[j] = key[tmp[b + min(r0 + j * WG, n - 1)]];
+                    for (int j = 0; j < ES_GROUP; ++j) kc[j] = kcell[tmp[b + min(r0 + j * WG, n - 1)]];
                 }
 #pragma unroll
                 for (int j = 0; j < ES_GROUP; ++j)
-                    if (r0 + j * WG < n) atomicAdd(&s_hist[kc[j] & 255], 1);
+                    if (r0 + j * WG < n) atomicAdd(&s_hist[kc[j]], 1);
             }
         }
         __syncthreads();
@@ -841,10 +850,10 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
 #pragma unroll
                 for (int j = 0; j < ES_GROUP; ++j) sv[j] = tmp[b + min(r0 + j * WG, n - 1)];
 #pragma unroll
-                for (int j = 0; j < ES_GROUP; ++j) kc[j] = key[sv[j]];
+                for (int j = 0; j < ES_GROUP; ++j) kc[j] = kcell[sv[j]];
 #pragma unroll
                 for (int j = 0; j < ES_GROUP; ++j)
-                    if (r0 + j * WG < n) idx[b + atomicAdd(&s_hist[kc[j] & 255], 1)] = sv[j];
+                    if (r0 + j * WG < n) idx[b + atomicAdd(&s_hist[kc[j]], 1)] = sv[j];
             }
             __syncthreads();
             for (int c0 = 0; c0 < TILE_CELLS;) {
@@ -866,7 +875,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
                             for (int j = 0; j < ES_GROUP; ++j) sv[j] = idx[b + lo + min(q0 + j * WG, len - 1)];
 #pragma unroll
                             for (int j = 0; j < ES_GROUP; ++j) {
-                                kc[j] = key[sv[j]];
+                                kc[j] = kcell[sv[j]];
                                 pd[j] = pid[sv[j]];
                             }
                         }
@@ -877,7 +886,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const int
                             unsigned long long kq = ~0ull;
                             if (q < len) {
                                 s_sv[q] = sv[j];
-                                kq = ((unsigned long long)(kc[j] & 255) << 44) |
+                                kq = ((unsigned long long)kc[j] << 44) |
                                      ((unsigned long long)(uint32_t)pd[j] << 13) | (unsigned long long)q;
                             }
                             s_key[q] = kq;
@@ -1180,7 +1189,8 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
                                              const int* __restrict__ gtile_pos,
                                              const float* __restrict__ tz, const float* __restrict__ tgx,
                                              const float* __restrict__ tgy, int* __restrict__ key,
-                                             int* __restrict__ tile_count, unsigned long long* __restrict__ vmx) {
+                                             uint8_t* __restrict__ kcell, int* __restrict__ tile_count,
+                                             unsigned long long* __restrict__ vmx) {
     // multiply-adds fused per source expression, so the two layouts (which
     // differ only in addressing) round the same way
 #pragma clang fp contract(on)
@@ -1197,7 +1207,7 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
     const int nwork = reo ? c->n_sorted : c->n_store;
     const int r = xcd_block((nwork + WG - 1) / WG) * WG + threadIdx.x;
     const bool in = r < nwork;
-    int t = 0, kk = -1, sl = 0, o = 0;
+    int t = 0, kk = -1, sl = 0, o = 0, cb = 0;
     float v2 = 0.f;
     bool pub = false;
     if (in) {
@@ -1305,12 +1315,15 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
             Q.pid[o] = P.pid[s];
         }
         if (reo || out != was_out) Q.slot[o] = sl | (out ? SLOT_OUT : 0);
-        kk = ekey(c, st, sl, out, cell, ox, oy, &t);
+        kk = ekey(c, st, sl, out, cell, ox, oy, &t, &cb);
         pub = slot_active(st) && !out;
         v2 = vxp * vxp + vyp * vyp;
     }
     wave_count_any(t, in && kk >= 0, tile_count);
-    if (in) key[o] = kk;
+    if (in) {
+        key[o] = kk;
+        kcell[o] = (uint8_t)cb;
+    }
     publish_vmax(sl, pub, v2, vmx);
 }
 
@@ -1556,6 +1569,7 @@ struct Engine32 {
     float* tpart = nullptr;
     PB32 A{}, B{};
     int *key = nullptr, *idx = nullptr, *tmp = nullptr;
+    uint8_t* kcell = nullptr; // per stored particle: its cell in its tile (the key holds the tile)
     // stored particles the particle-range grids (G2P, scatter) are sized for:
     // the live count rounded up to a power of two, so a lightly filled
     // capacity launches few idle blocks (a captured step is rebuilt when it
@@ -1672,7 +1686,7 @@ void enqueue_estep(Engine32* e, int step) {
         k_escatter<<<nb, WG, 0, s>>>(c, e->key, e->tile_begin, e->cursor, e->tmp);
     });
     tk(e, 5, step, [&] {
-        k_esort<<<pb, WG, 0, s>>>(c, e->A, e->B, e->key, e->ptiles, e->tile_count, e->cursor, e->tile_begin, e->tmp,
+        k_esort<<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->tile_count, e->cursor, e->tile_begin, e->tmp,
                                   e->idx, e->cell_begin);
     });
     tk(e, 6, step, [&] {
@@ -1691,10 +1705,10 @@ void enqueue_estep(Engine32* e, int step) {
         const unsigned gb = xcd_grid((e->launch_n + WG - 1) / WG);
         if (e->pool_vf)
             k_eg2p<true><<<gb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->gtile_pos, e->tz, e->tgx, e->tgy,
-                                           e->key, e->tile_count, e->vmx);
+                                           e->key, e->kcell, e->tile_count, e->vmx);
         else
             k_eg2p<false><<<gb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->gtile_pos, e->tz, e->tgx,
-                                            e->tgy, e->key, e->tile_count, e->vmx);
+                                            e->tgy, e->key, e->kcell, e->tile_count, e->vmx);
     });
 }
 
@@ -1786,8 +1800,8 @@ void terrain32(const double* z, int nx, int ny, double inv, std::vector<float>& 
 } // namespace
 
 int64_t engine_particle_bytes() {
-    // two state buffers (cell, ox, oy, vx, vy, vz, h, m, z, pid, slot = 44 B) + key, idx, tmp
-    return 2 * 44 + 12;
+    // two state buffers (cell, ox, oy, vx, vy, vz, h, m, z, pid, slot = 44 B) + key, cell byte, idx, tmp
+    return 2 * 44 + 13;
 }
 
 int64_t engine_slot_bytes(const Engine32* e) {
@@ -1866,8 +1880,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     e->nty = (ny + TILE - 1) / TILE;
     e->ntd = e->ntx * e->nty;
     e->ntiles = nslots * (e->ntd + 1);
-    if ((double)e->ntiles * TILE_CELLS >= 2147483647.0) {
-        if (err) *err = "FP32 engine: slots x terrain tiles exceed the 31-bit key space";
+    if ((double)e->ntiles >= 2147483647.0) {
+        if (err) *err = "FP32 engine: slots x terrain tiles exceed the 31-bit tile ids";
         delete e;
         return nullptr;
     }
@@ -1933,7 +1947,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
             ok = ok && ealloc(e->pallocs, q, max_particles, &why);
         ok = ok && ealloc(e->pallocs, &b->pid, max_particles, &why) && ealloc(e->pallocs, &b->slot, max_particles, &why);
     }
-    ok = ok && ealloc(e->pallocs, &e->key, max_particles, &why) && ealloc(e->pallocs, &e->idx, max_particles, &why) &&
+    ok = ok && ealloc(e->pallocs, &e->key, max_particles, &why) && ealloc(e->pallocs, &e->kcell, max_particles, &why) &&
+         ealloc(e->pallocs, &e->idx, max_particles, &why) &&
          ealloc(e->pallocs, &e->tmp, max_particles, &why);
     if (ok && !single)
         ok = ealloc(e->allocs, &e->hm_map, (size_t)nslots * e->ntd, &why) &&
@@ -2170,7 +2185,8 @@ int engine_admit(Engine32* e, const Slot& prm, int64_t np, const double* x, cons
     if (!push_all(e, err)) return -1;
     const unsigned nb = (unsigned)((n + WG - 1) / WG);
     k_ez<<<nb, WG, 0, s>>>(e->d_c, e->A, e->B, (int)base, (int)(base + np), e->tz);
-    k_ekeys<<<nb, WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, (int)base, (int)(base + np), e->key, e->tile_count);
+    k_ekeys<<<nb, WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, (int)base, (int)(base + np), e->key, e->kcell,
+                              e->tile_count);
     if (!hip_ok(hipGetLastError(), "admit kernels", err) || !hip_ok(hipStreamSynchronize(s), "admit", err)) return -1;
     e->admit_base[slot] = base;
     e->res_ptiles[slot] = res_p;
@@ -2242,7 +2258,7 @@ bool engine_release(Engine32* e, int slot, std::string* err) {
         if (!hip_ok(hipMemsetAsync(e->tile_count, 0, (size_t)e->ntiles * 4, s), "release", err) ||
             !hip_ok(hipMemsetAsync(e->cursor, 0, (size_t)e->ntiles * 4, s), "release", err))
             return false;
-        k_ekeys<<<(unsigned)((n + WG - 1) / WG), WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, 0, n, e->key,
+        k_ekeys<<<(unsigned)((n + WG - 1) / WG), WG, 0, s>>>(e->d_c, e->d_slots, e->A, e->B, 0, n, e->key, e->kcell,
                                                              e->tile_count);
         size_particle_grids(e); // storage may have grown by admissions since the last step
         e->h_c->compact = 1;

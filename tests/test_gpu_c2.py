@@ -79,3 +79,46 @@ def test_c2_full_run_fp32_error_against_fp64():
     assert rep["vmax_L1"] <= 0.02
     assert rep["runout_diff_cells"] <= 2.0
     assert abs(rep["mass_fp32"] - rep["mass_fp64"]) <= 1e-6 * rep["mass_fp64"]
+
+
+def test_batch_of_256_slots_on_the_8000x8000_dem():
+    # The tile-based batch on the 8M-particle workload's DEM (64M nodes) with
+    # 256 slots: no per-slot terrain-sized array (grid velocity and max maps in
+    # tile pools), tile ids of 256 x 250,001 tiles within 31 bits.  Four small
+    # star releases run 0.5 s; each slot's max maps and particles equal the
+    # same scenario run alone as a single FP32 run (tolerance: exact).
+    import slabpro
+    from slabpro import abi, configs
+    cfg = _c2()
+    z = cfg["z"]
+    b = abi.Batch(z, 1.0, 256, 4_000_000)
+    fp, t = b.footprint(), b.tiles()
+    print("footprint", fp, "tiles", t, flush=True)
+    assert fp["bytes_per_slot"] < 2_000_000  # no per-slot terrain-sized arrays
+    prm = dict(cfg["params"], total_time=0.5, output_dt=0.25)
+    scen = []
+    for k, (cx, cy) in enumerate(((1500.0, 2500.0), (1500.0, 5500.0), (3000.0, 4000.0), (2200.0, 3200.0))):
+        vx, vy = configs.star_polygon(cx, cy, 60.0 + 10.0 * k, 32, 0.2, configs.rng(2, 10 + k))
+        px, py, area = configs.polygon_release(1.0, vx, vy)
+        part = configs.finish_particles(px, py, np.full(len(px), 3.0), area * 3.0, prm["rho"])
+        s = b.admit(abi.params(precision=abi.FP32, **prm), abi.friction(**cfg["friction"]), part["x"], part["y"],
+                    part["h"], part["A"], part["m"])
+        assert s is not None
+        scen.append((s, part))
+    while b.step(64) > 0:
+        pass
+    print("tiles after the run", b.tiles(), flush=True)
+    for s, part in scen:
+        assert b.slot_info(s)["status"] == abi.SLOT_DONE
+        hm, vm = b.max_maps(s)
+        p = b.particles(s)
+        one = {"z": z, "dx": 1.0, "params": prm, "friction": cfg["friction"], **part}
+        r = slabpro.make_run(one, precision=abi.FP32)
+        r.prime()
+        r.run()
+        rh, rv = r.max_maps()
+        rp = r.particles()
+        r.close()
+        assert np.array_equal(hm, rh) and np.array_equal(vm, rv), s
+        assert np.array_equal(p["x"], rp["x"]) and np.array_equal(p["h"], rp["h"]), s
+    b.close()

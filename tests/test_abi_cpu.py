@@ -64,3 +64,18 @@ def test_no_device_fails_loudly():
         return
     if info["gfx950"]:
         pytest.skip("a gfx950 device is present")
+
+
+def test_batch_and_probe_entry_points_reject_bad_arguments():
+    import ctypes
+    L = abi.lib()
+    d = (ctypes.c_double * 4)()
+    out = (ctypes.c_int64 * 9)()
+    ms = ctypes.c_double(0.0)
+    assert L.slab_batch_tiles(None, out) == abi.EINVAL
+    assert L.slab_batch_set_tile_growth(None, 4) == abi.EINVAL
+    assert L.slab_batch_download_particles_noalloc(None, 0, d, d, d, d, d, d, d, d, d) == abi.EINVAL
+    assert L.slab_stream_probe(9, 1 << 20, 1, ctypes.byref(ms)) == abi.EINVAL   # unknown kind
+    assert L.slab_stream_probe(0, 1000, 1, ctypes.byref(ms)) == abi.EINVAL      # not a multiple of 16 bytes
+    assert L.slab_stream_probe(0, 1 << 20, 0, ctypes.byref(ms)) == abi.EINVAL   # no iterations
+    assert L.slab_stream_probe(0, 1 << 20, 1, None) == abi.EINVAL

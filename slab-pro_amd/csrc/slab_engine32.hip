@@ -17,6 +17,7 @@
 #include <cstring>
 #include <mutex>
 #include <thread>
+#include <type_traits>
 
 namespace slab {
 
@@ -571,7 +572,7 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
 // cursor atomics of all groups (one per run of equal tiles) before it waits
 // for any of them.  Arrival order inside a tile is arbitrary: the sort makes
 // it canonical.
-constexpr int SC_PER = 8;
+constexpr int SC_PER = 16;
 __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict__ key,
                                                  const int* __restrict__ tile_begin, int* __restrict__ cursor,
                                                  int* __restrict__ tmp) {
@@ -981,7 +982,10 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         for (int k = 0; k < 9; ++k)
 #pragma unroll
             for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
-        // the next particle's loads are issued before the current one is summed
+        // the next particle's loads are issued before the current one is summed;
+        // the vz momentum (grid vz output only) is summed on output steps only
+        auto sum_particles = [&](auto vz_tag) {
+        constexpr bool VZ = decltype(vz_tag)::value;
         for (int r = r0; r < r1; ++r) {
             const float ox = nox, oy = noy, mp = nmp, hp = nhp, pvx = nvx, pvy = nvy, pvz = nvz;
             if (r + 1 < r1) {
@@ -989,7 +993,7 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
                 const float2 po = P.o[sn], pv = P.v[sn];
                 nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
                 nvx = pv.x; nvy = pv.y;
-                if (need_vz) nvz = P.vz[sn];
+                if (VZ) nvz = P.vz[sn];
             }
             // live particles sit in cells [1, N-3] (outflow parks the others),
             // so the stencil base is never clamped and the floor cell is the
@@ -1024,7 +1028,7 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
                         a[0] = fmaf(mp, wgt, a[0]);
                         a[1] = fmaf(vmx, wgt, a[1]);
                         a[2] = fmaf(vmy, wgt, a[2]);
-                        a[3] = fmaf(vzm, wgt, a[3]);
+                        if (VZ) a[3] = fmaf(vzm, wgt, a[3]);
                         a[4] = fmaf(hm, wgt, a[4]);
                         a[5] = fmaf(-psx[d], wy[e], fmaf(mbx, wgt, a[5]));
                         a[6] = fmaf(-psy[e], wx[d], fmaf(mby, wgt, a[6]));
@@ -1032,6 +1036,11 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
                 }
             }
         }
+        };
+        if (need_vz)
+            sum_particles(std::true_type{});
+        else
+            sum_particles(std::false_type{});
         __syncthreads();
 #pragma unroll
         for (int ph = 0; ph < 9; ++ph) {

@@ -399,3 +399,26 @@ def test_max_map_tiles_return_to_the_pool():
     assert np.array_equal(hm, ref[0][0]) and np.array_equal(vm, ref[0][1])
     assert np.array_equal(b.particles(s)["x"], ref[1]["x"])
     b.close()
+
+
+def test_max_map_pool_overflow_stops_the_batch(monkeypatch):
+    # A max-map pool of one tile (SLAB_MAXMAP_TILE_CAP) and no tile check at
+    # admission: the first step's tile listing runs out of pool tiles, the
+    # step returns SLAB_EDOMAIN and the batch then refuses further steps.
+    from slabpro import abi
+    z, dx = _terrain()
+    part, fr = _scenario(81, FRICS[1], n_extra=10)
+    monkeypatch.setenv("SLAB_MAXMAP_TILE_CAP", "1")
+    b = abi.Batch(z, dx, 2, 100000)
+    monkeypatch.delenv("SLAB_MAXMAP_TILE_CAP")
+    assert b.tiles()["max_map_tile_capacity"] == 1
+    b.set_tile_growth(0)
+    s = b.admit(_params(), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
+    assert s is not None
+    with pytest.raises(abi.SlabError) as ei:
+        b.step(4)
+    assert ei.value.status == abi.EDOMAIN and "max-map" in str(ei.value)
+    with pytest.raises(abi.SlabError) as ei:
+        b.step(1)
+    assert ei.value.status == abi.ESTATE
+    b.close()

@@ -1096,48 +1096,47 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
         float acc[P2G_FIELDS];
 #pragma unroll
         for (int f = 0; f < P2G_FIELDS; ++f) acc[f] = 0.f;
-        int src[4], rn[4];
-        int ns = 0;
-        src[ns] = t;
-        rn[ns++] = a * REGION + b;
-        if (tx > 0 && a < 2) {
-            src[ns] = t - nty;
-            rn[ns++] = (a + TILE) * REGION + b;
-        }
-        if (ty > 0 && b < 2) {
-            src[ns] = t - 1;
-            rn[ns++] = a * REGION + (b + TILE);
-        }
-        if (tx > 0 && ty > 0 && a < 2 && b < 2) {
-            src[ns] = t - nty - 1;
-            rn[ns++] = (a + TILE) * REGION + (b + TILE);
-        }
-        // stamps, then list positions, then partial values of all sources are
-        // read before any is summed (sums in source order)
+        // sources in fixed order: own, left, below, diagonal tile.  The tile
+        // ids depend only on the tile (a missing neighbour reads the own
+        // tile's entries, unused), so their stamps and list positions are
+        // read once per workgroup, together (a stale position is never used);
+        // then the partial values of the stamped sources that reach this node
+        // are read before any is summed (sums in source order).  The node's
+        // terrain and max maps do not depend on them and are read alongside.
+        const int src[4] = {t, tx > 0 ? t - nty : t, ty > 0 ? t - 1 : t, tx > 0 && ty > 0 ? t - nty - 1 : t};
+        const bool hx = tx > 0 && a < 2, hy = ty > 0 && b < 2;
+        const bool has[4] = {true, hx, hy, hx && hy};
+        const int rn[4] = {a * REGION + b, (a + TILE) * REGION + b, a * REGION + (b + TILE),
+                           (a + TILE) * REGION + (b + TILE)};
+        const size_t kt = (size_t)i * ny + j; // terrain node
+        const size_t k = (size_t)slot * nn + kt;
+        const size_t km = POOL ? (size_t)hm_map[t] * TILE_CELLS + threadIdx.x : k;
+        const float Gx = tgx[kt], Gy = tgy[kt], cth = tcth[kt];
+        const float hm_old = h_max[km], vm_old = v_max2[km];
         bool ok[4];
         int pos[4];
         float val[4][P2G_FIELDS];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) ok[k] = k < ns && pstamp[src[k]] == ep;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) pos[k] = ok[k] ? tile_pos[src[k]] : 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (!ok[k]) continue;
-            const float* q = tpart + (size_t)pos[k] * (P2G_FIELDS * REGION_NODES) + rn[k];
-#pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) val[k][f] = q[f * REGION_NODES];
+        for (int q = 0; q < 4; ++q) {
+            const int st = pstamp[src[q]], ps = tile_pos[src[q]];
+            ok[q] = has[q] && st == ep;
+            pos[q] = ps;
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (!ok[k]) continue;
+        for (int q = 0; q < 4; ++q) {
+            if (!ok[q]) continue;
+            const float* pp = tpart + (size_t)pos[q] * (P2G_FIELDS * REGION_NODES) + rn[q];
 #pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) acc[f] += val[k][f];
+            for (int f = 0; f < P2G_FIELDS; ++f) val[q][f] = pp[f * REGION_NODES];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!ok[q]) continue;
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) acc[f] += val[q][f];
         }
         const float g = (float)S.gravity, dt = (float)S.dt;
         const float am = acc[0];
-        const size_t kt = (size_t)i * ny + j; // terrain node
-        const size_t k = (size_t)slot * nn + kt;
         float nvx = 0.f, nvy = 0.f, flx = 0.f, fly = 0.f, hh = 0.f, vz = 0.f;
         if (am > 1e-6f) {
             const float rm = 1.0f / am;
@@ -1147,7 +1146,6 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
             const float vtx = vx + dt * (acc[5] * rm), vty = vy + dt * (acc[6] * rm);
             const float vsq = vtx * vtx + vty * vty;
             if (vsq > 1e-16f) {
-                const float Gx = tgx[kt], Gy = tgy[kt], cth = tcth[kt];
                 const float v_z = vtx * Gx + vty * Gy;
                 const float v3sq = vsq + v_z * v_z;
                 const float vn = sqrtf(v3sq);
@@ -1170,10 +1168,9 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
             }
             flx = nvx - vx;
             fly = nvy - vy;
-            const size_t km = POOL ? (size_t)hm_map[t] * TILE_CELLS + threadIdx.x : k;
-            if (h_max[km] < hh) h_max[km] = hh;
+            if (hm_old < hh) h_max[km] = hh;
             const float v2 = nvx * nvx + nvy * nvy;
-            if (v_max2[km] < v2) v_max2[km] = v2;
+            if (vm_old < v2) v_max2[km] = v2;
         }
         vf[POOL ? (size_t)w * TILE_CELLS + threadIdx.x : k] = make_float4(nvx, nvy, flx, fly);
         if (out_m && S.out_step) {
@@ -1216,6 +1213,16 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
     const int nwork = reo ? c->n_sorted : c->n_store;
     const int r = xcd_block((nwork + WG - 1) / WG) * WG + threadIdx.x;
     const bool in = r < nwork;
+    // A single run (dense layout) has one slot: its status, dt and FLIP ratio
+    // are read once per workgroup, so no particle waits for a per-particle
+    // slot lookup before its stencil reads.
+    int st0 = 0;
+    float dt0 = 0.f, fr0 = 0.f;
+    if (!POOL) {
+        st0 = slots[0].status;
+        dt0 = (float)slots[0].dt;
+        fr0 = (float)slots[0].flip_ratio;
+    }
     int t = 0, kk = -1, sl = 0, o = 0, cb = 0;
     float v2 = 0.f;
     bool pub = false;
@@ -1225,17 +1232,17 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
         const int s = reo ? idx[r] : r;
         o = r; // storage index written
         const int raw = P.slot[s];
-        sl = raw & SLOT_MASK;
+        sl = POOL ? raw & SLOT_MASK : 0;
         bool out = (raw & SLOT_OUT) != 0;
         const bool was_out = out;
-        const Slot& S = slots[sl];
-        const int st = S.status;
+        const int st = POOL ? slots[sl].status : st0;
         int cell = P.cell[s];
         const float2 po = P.o[s], pv = P.v[s];
         float ox = po.x, oy = po.y, vxp = pv.x, vyp = pv.y, hp = P.h[s];
         float vzp, zp; // recomputed by a running particle, carried otherwise
         if (st == SLOT_RUNNING && !c->compact && !out) {
-            const float inv = (float)c->inv_dx, dt = (float)S.dt, fr = (float)S.flip_ratio;
+            const float inv = (float)c->inv_dx;
+            const float dt = POOL ? (float)slots[sl].dt : dt0, fr = POOL ? (float)slots[sl].flip_ratio : fr0;
             const int ci = cell >> 16, cj = cell & 0xffff;
             const size_t fc = (size_t)ci * ny + cj;
             const float Gx = tgx[fc], Gy = tgy[fc];

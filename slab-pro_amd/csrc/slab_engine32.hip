@@ -939,7 +939,32 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
     const float inv = (float)c->inv_dx;
     const int cid = threadIdx.x;
     const int lx = cid >> 4, ly = cid & 15;
+    // this thread's particle range of the workgroup's next tile is read one
+    // tile ahead
+    int nr0 = 0, nr1 = 0;
+    if ((int)blockIdx.x < nt) {
+        const int* cb = cell_begin + (size_t)blockIdx.x * CELL_STRIDE;
+        nr0 = cb[cid];
+        nr1 = cb[cid + 1];
+    }
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        const int r0 = nr0, r1 = nr1;
+        // this thread's first particle's loads depend on its range only: they
+        // are issued before the tile and slot checks and the LDS staging below
+        // (every listed range lies inside the sorted index), and overlap them
+        float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
+        int sn0 = 0;
+        if (r0 < r1) {
+            sn0 = idx[r0];
+            const float2 po = P.o[sn0], pv = P.v[sn0];
+            nox = po.x; noy = po.y; nmp = P.m[sn0]; nhp = P.h[sn0];
+            nvx = pv.x; nvy = pv.y;
+        }
+        if (w + (int)gridDim.x < nt) {
+            const int* cb = cell_begin + (size_t)(w + gridDim.x) * CELL_STRIDE;
+            nr0 = cb[cid];
+            nr1 = cb[cid + 1];
+        }
         const int t = ptiles[w];
         if (t >= live) continue;
         const int slot = t / ntd, local = t - slot * ntd;
@@ -947,19 +972,8 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         const float g = (float)slots[slot].gravity;
         // P_z only feeds the grid's vz output, written on output steps
         const bool need_vz = slots[slot].out_step != 0;
+        if (need_vz && r0 < r1) nvz = P.vz[sn0];
         const int tx = local / nty, ty = local - tx * nty;
-        // this thread's particle range and its first particle's loads do not
-        // depend on the LDS staging below: issued first, they overlap it
-        const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
-        const int r0 = cb[cid], r1 = cb[cid + 1];
-        float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
-        if (r0 < r1) {
-            const int sn = idx[r0];
-            const float2 po = P.o[sn], pv = P.v[sn];
-            nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
-            nvx = pv.x; nvy = pv.y;
-            if (need_vz) nvz = P.vz[sn];
-        }
         for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
         // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged once
         for (int q = threadIdx.x; q < TCELLS * TCELLS; q += WG) {

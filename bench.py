@@ -229,7 +229,20 @@ def copy_probe():
         ms = abi.stream_probe(kind, n, 20)
         out[name] = {"ms": ms, "bytes": nbytes, "GB_s": nbytes / ms / 1e6, "frac_of_8TBs": nbytes / ms / 1e6 / 8000.0}
         log(f"{name}: {ms:.4f} ms, {nbytes / ms / 1e6:.0f} GB/s")
-    print(json.dumps({"stream_probe": out}), flush=True)
+    # kernel boundary against grid barrier: eight dependent copy phases of n
+    # bytes, as eight launches and as one cooperative launch with seven grid
+    # barriers (device-scope release/acquire), same workgroups
+    phases = {}
+    for per_cu, kl, kb in ((4, abi.PROBE_PHASES_LAUNCHES, abi.PROBE_PHASES_BARRIERS),
+                           (1, abi.PROBE_PHASES_LAUNCHES_1, abi.PROBE_PHASES_BARRIERS_1)):
+        for n in (1 << 20, 64 << 20, 256 << 20):
+            t_l = abi.stream_probe(kl, n, 20)
+            t_b = abi.stream_probe(kb, n, 20)
+            phases[f"{per_cu}_per_CU_{n >> 20}MiB"] = {
+                "launches_ms": t_l, "barriers_ms": t_b, "launch_minus_barrier_us_per_boundary": (t_l - t_b) / 7 * 1e3}
+            log(f"{per_cu} workgroup(s) per CU, 8 phases of {n >> 20} MiB: 8 launches {t_l:.4f} ms, "
+                f"1 launch + 7 grid barriers {t_b:.4f} ms")
+    print(json.dumps({"stream_probe": out, "phase_boundaries": phases}), flush=True)
 
 
 def main():

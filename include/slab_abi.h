@@ -157,9 +157,14 @@ slab_status slab_device_info(int32_t out[3]);
  * and writes); 1: read n bytes; 2: write n bytes; 3: the particle streams of
  * G2P for n particles (28 B read, 32 B written in place, one per thread);
  * 4 and 5: the copy and the write with non-temporal accesses and 128 B in
- * flight per thread.
- * *ms = mean kernel time of iters launches (HIP events).  No reference
- * counterpart: measurement only. */
+ * flight per thread; 6: eight dependent copy phases of n bytes (a -> b,
+ * b -> a, ...), one launch each; 7: the same eight phases in one cooperative
+ * launch, separated by grid-wide barriers with device-scope release/acquire
+ * (6 and 7 use the same workgroup count, four per CU: their difference is the
+ * cost of a kernel boundary against a grid barrier); 8 and 9: 6 and 7 with
+ * one workgroup per CU.
+ * *ms = mean time of iters launches (of the eight launches for kind 6; HIP
+ * events).  No reference counterpart: measurement only. */
 slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters, double* ms);
 
 /* ---------------------------------------------------------------------------

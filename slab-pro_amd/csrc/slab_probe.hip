@@ -113,6 +113,87 @@ __global__ void __launch_bounds__(WG) k_probe_particles(int32_t* __restrict__ ce
     z[s] = po.x + hp;
 }
 
+// Phase boundaries: kPhases dependent copy phases (a -> b, then b -> a, ...,
+// so each phase reads what the previous one wrote, possibly on another XCD),
+// either one launch per phase (the kernel boundary orders them) or all in
+// one cooperative launch whose phases are separated by grid-wide barriers
+// with device-scope release / acquire fences.  Both run the same grid-stride
+// copy with the same number of workgroups, so the difference of their times
+// is the cost of a kernel boundary against that of a grid barrier.
+constexpr int kPhases = 8;
+
+__device__ __forceinline__ void phase_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
+    const size_t stride = (size_t)gridDim.x * WG;
+    for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(WG) k_probe_phase(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
+    phase_copy(a, b, n4);
+}
+
+// Grid-wide barrier for a cooperative launch (every workgroup resident).
+// Thread 0 of each workgroup releases the workgroup's writes device-wide and
+// arrives on its group's counter (workgroup index % 8, one group per XCD when
+// workgroups are dealt round-robin); the last of a group arrives on the top
+// counter, and the last group advances `gen`.  The others wait until `gen`
+// moves.  The wait is bounded by `spin_limit` polls, after which it raises
+// *fail and returns false (every workgroup then leaves the kernel).
+// bar: 8 group counters 64 bytes apart, then the top counter, then gen.
+constexpr int kBarGroups = 8;
+constexpr int kBarStride = 16; // unsigned ints between group counters
+__device__ bool grid_barrier(unsigned* bar, int* fail, unsigned long long spin_limit) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned* top = bar + kBarGroups * kBarStride;
+        unsigned* gen = top + kBarStride;
+        const unsigned grp = blockIdx.x % kBarGroups;
+        const unsigned gsize = (gridDim.x - grp + kBarGroups - 1) / kBarGroups; // workgroups in this group
+        const unsigned ngrp = gridDim.x < (unsigned)kBarGroups ? gridDim.x : (unsigned)kBarGroups;
+        int ok = 1;
+        __threadfence();
+        const unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool last = false;
+        if (atomicAdd(bar + grp * kBarStride, 1u) == gsize - 1) {
+            atomicExch(bar + grp * kBarStride, 0u);
+            if (atomicAdd(top, 1u) == ngrp - 1) {
+                atomicExch(top, 0u);
+                last = true;
+            }
+        }
+        if (last) {
+            __threadfence();
+            atomicAdd(gen, 1u);
+        } else {
+            unsigned long long it = 0;
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g &&
+                   __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                if (++it > spin_limit) {
+                    atomicExch(fail, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            ok = __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        }
+        __threadfence();
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+__global__ void __launch_bounds__(WG) k_probe_phases_coop(float4* a, float4* b, size_t n4, unsigned* bar,
+                                                          int* fail) {
+    for (int k = 0; k < kPhases; ++k) {
+        if (k > 0 && !grid_barrier(bar, fail, 1ull << 22)) return;
+        if (k & 1)
+            phase_copy(b, a, n4);
+        else
+            phase_copy(a, b, n4);
+    }
+}
+
 template <typename T>
 bool palloc(std::vector<void*>& l, T** p, size_t n) {
     void* q = nullptr;
@@ -128,7 +209,7 @@ bool palloc(std::vector<void*>& l, T** p, size_t n) {
 using namespace slab;
 
 extern "C" slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters, double* ms_out) {
-    if (kind < 0 || kind > 5 || n < 1 || iters < 1 || !ms_out) return set_error(SLAB_EINVAL, "bad probe arguments");
+    if (kind < 0 || kind > 9 || n < 1 || iters < 1 || !ms_out) return set_error(SLAB_EINVAL, "bad probe arguments");
     const bool bytes_kind = kind != 3;
     if (bytes_kind && (n % 16 != 0 || n > ((int64_t)1 << 34)))
         return set_error(SLAB_EINVAL, "bytes: multiple of 16, <= 16 GiB");
@@ -151,11 +232,32 @@ extern "C" slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters,
         float2 *o = nullptr, *v = nullptr;
         float *h = nullptr, *vz = nullptr, *z = nullptr;
         const size_t n4 = bytes_kind ? (size_t)n / 16 : 0;
-        const int pv = kind >= 4 ? PVN : PV;
-        const unsigned nb = bytes_kind ? (unsigned)((n4 + (size_t)WG * pv - 1) / ((size_t)WG * pv))
-                                       : (unsigned)((n + WG - 1) / WG);
+        const int pv = kind == 4 || kind == 5 ? PVN : PV;
+        unsigned nb = bytes_kind ? (unsigned)((n4 + (size_t)WG * pv - 1) / ((size_t)WG * pv))
+                                 : (unsigned)((n + WG - 1) / WG);
+        unsigned* bar = nullptr;
+        int* fail = nullptr;
+        const bool phases = kind >= 6;
+        const bool coop = kind == 7 || kind == 9;
+        if (phases) {
+            // phase probes: four workgroups per CU (kinds 6, 7) or one (8, 9),
+            // all resident at once (the cooperative launch needs them all), the
+            // same for both forms
+            int dev = 0, cus = 0, per = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_probe_phases_coop),
+                                                             WG, 0) != hipSuccess ||
+                cus < 1 || per < 1) {
+                res = SLAB_EDEVICE;
+                why = "occupancy query failed";
+                break;
+            }
+            nb = (unsigned)(cus * std::min(per, kind <= 7 ? 4 : 1));
+        }
         bool ok = true;
-        if (kind == 0 || kind == 4) ok = palloc(l, &a, n4) && palloc(l, &b, n4);
+        if (kind == 0 || kind == 4 || phases) ok = palloc(l, &a, n4) && palloc(l, &b, n4);
+        if (coop) ok = ok && palloc(l, &bar, (kBarGroups + 2) * kBarStride) && palloc(l, &fail, 1);
         if (kind == 1) ok = palloc(l, &a, n4) && palloc(l, &red, nb);
         if (kind == 2 || kind == 5) ok = palloc(l, &b, n4);
         if (kind == 3)
@@ -167,6 +269,10 @@ extern "C" slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters,
             break;
         }
         if (a) ok = hipMemsetAsync(a, 0, n4 * 16, st) == hipSuccess;
+        if (b && phases) ok = ok && hipMemsetAsync(b, 0, n4 * 16, st) == hipSuccess;
+        if (bar)
+            ok = ok && hipMemsetAsync(bar, 0, (kBarGroups + 2) * kBarStride * 4, st) == hipSuccess &&
+                 hipMemsetAsync(fail, 0, 4, st) == hipSuccess;
         if (kind == 3) {
             const size_t n1 = (size_t)n;
             ok = hipMemsetAsync(cell, 0, n1 * 4, st) == hipSuccess && hipMemsetAsync(slot, 0, n1 * 4, st) == hipSuccess &&
@@ -185,8 +291,27 @@ extern "C" slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters,
             if (kind == 3) k_probe_particles<<<nb, WG, 0, st>>>(cell, o, v, h, slot, vz, z, (int)n);
             if (kind == 4) k_probe_copy_nt<<<nb, WG, 0, st>>>(a, b, n4);
             if (kind == 5) k_probe_write_nt<<<nb, WG, 0, st>>>(b, n4, 1.0f);
+            if (phases && !coop)
+                for (int k = 0; k < kPhases; ++k) {
+                    if (k & 1)
+                        k_probe_phase<<<nb, WG, 0, st>>>(b, a, n4);
+                    else
+                        k_probe_phase<<<nb, WG, 0, st>>>(a, b, n4);
+                }
+            if (coop) {
+                void* args[] = {&a, &b, const_cast<size_t*>(&n4), &bar, &fail};
+                if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_probe_phases_coop), dim3(nb), dim3(WG),
+                                               args, 0, st) != hipSuccess)
+                    ok = false;
+            }
         };
         launch(); // warm-up
+        if (!ok) {
+            (void)hipGetLastError();
+            res = SLAB_EDEVICE;
+            why = "cooperative launch refused";
+            break;
+        }
         double tot = 0.0;
         for (int it = 0; it < iters; ++it) {
             (void)hipEventRecord(e0, st);
@@ -202,6 +327,14 @@ extern "C" slab_status slab_stream_probe(int32_t kind, int64_t n, int32_t iters,
             tot += ms;
         }
         *ms_out = tot / iters;
+        if (fail) {
+            int f = 0;
+            if (hipMemcpy(&f, fail, 4, hipMemcpyDeviceToHost) != hipSuccess || f != 0) {
+                res = SLAB_EDEVICE;
+                why = "grid barrier wait exceeded its bound";
+                break;
+            }
+        }
     } while (false);
     if (st) (void)hipStreamSynchronize(st);
     for (void* p : l) (void)hipFree(p);

@@ -174,6 +174,9 @@ def device_info():
 
 
 PROBE_COPY, PROBE_READ, PROBE_WRITE, PROBE_PARTICLES, PROBE_COPY_NT, PROBE_WRITE_NT = range(6)
+# eight dependent copy phases: one launch each / one cooperative launch with
+# grid barriers; four workgroups per CU, then one
+PROBE_PHASES_LAUNCHES, PROBE_PHASES_BARRIERS, PROBE_PHASES_LAUNCHES_1, PROBE_PHASES_BARRIERS_1 = 6, 7, 8, 9
 
 
 def stream_probe(kind, n, iters=20):

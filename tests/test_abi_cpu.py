@@ -75,7 +75,7 @@ def test_batch_and_probe_entry_points_reject_bad_arguments():
     assert L.slab_batch_tiles(None, out) == abi.EINVAL
     assert L.slab_batch_set_tile_growth(None, 4) == abi.EINVAL
     assert L.slab_batch_download_particles_noalloc(None, 0, d, d, d, d, d, d, d, d, d) == abi.EINVAL
-    assert L.slab_stream_probe(9, 1 << 20, 1, ctypes.byref(ms)) == abi.EINVAL   # unknown kind
+    assert L.slab_stream_probe(10, 1 << 20, 1, ctypes.byref(ms)) == abi.EINVAL  # unknown kind
     assert L.slab_stream_probe(0, 1000, 1, ctypes.byref(ms)) == abi.EINVAL      # not a multiple of 16 bytes
     assert L.slab_stream_probe(0, 1 << 20, 0, ctypes.byref(ms)) == abi.EINVAL   # no iterations
     assert L.slab_stream_probe(0, 1 << 20, 1, None) == abi.EINVAL

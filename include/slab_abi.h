@@ -269,12 +269,28 @@ slab_status slab_batch_set_resort(slab_batch* b, int32_t steps);
 slab_status slab_run_set_resort(slab_run* run, int32_t steps);
 /* Autotuner of the FP32 form: times candidate launch shapes -- storage
  * re-sort period {1, 4, 16, 64}, then workgroups of the tile-list kernels
- * {1024, 2048, 4096, 8192} -- on the next steps of the live run (each trial
- * runs max(trial_steps, 2 x period) ordinary steps; results do not depend on
- * the shape) and keeps the fastest.  out[0] = period, out[1] = workgroups.  The run variant
- * needs slab_run_prime first and is a no-op for the FP64 form. */
+ * {1024, 2048, 4096, 8192}, then the in-LDS capacity of the tile sort {1024,
+ * 2048, 4096} -- on the next steps of the live run (each trial runs
+ * max(trial_steps, 2 x period) ordinary steps; results do not depend on the
+ * shape) and keeps the fastest.  out[0] = period, out[1] = workgroups; the
+ * whole shape is read back with slab_*_launch_shape.  The run variant needs
+ * slab_run_prime first and is a no-op for the FP64 form.  No reference
+ * counterpart (its sort block is fixed at 4x4 cells, src/simulation.cpp:645). */
 slab_status slab_batch_autotune(slab_batch* b, int32_t trial_steps, int32_t out[2]);
 slab_status slab_run_autotune(slab_run* run, int32_t trial_steps, int32_t out[2]);
+/* In-LDS capacity of the FP32 tile sort, in particles: 1024, 2048 or 4096
+ * (default).  Tiles above it are ordered by cell in global memory and then in
+ * rounds of whole cells of at most half the capacity.  A smaller capacity holds
+ * fewer registers and less LDS per workgroup (more workgroups per CU); results
+ * do not depend on it.  A no-op for the FP64 form.  No reference counterpart
+ * (src/simulation.cpp:644-690 sorts by 4x4-cell blocks on the host). */
+slab_status slab_batch_set_sort_capacity(slab_batch* b, int32_t particles);
+slab_status slab_run_set_sort_capacity(slab_run* run, int32_t particles);
+/* The current FP32 launch shape: out[0] = storage re-sort period, out[1] =
+ * workgroups of the tile-list kernels, out[2] = sort capacity (the FP64 form
+ * reports {1, 0, 0}).  No reference counterpart. */
+slab_status slab_batch_launch_shape(slab_batch* b, int32_t out[3]);
+slab_status slab_run_launch_shape(slab_run* run, int32_t out[3]);
 slab_status slab_batch_timing(slab_batch* b, int32_t enable, double* ms, int64_t* counts, int32_t n);
 void slab_batch_destroy(slab_batch* b);
 

@@ -1019,10 +1019,49 @@ slab_status slab_run_set_resort(slab_run* r, int32_t steps) {
 slab_status slab_batch_autotune(slab_batch* b, int32_t trial_steps, int32_t out[2]) {
     if (!b || !out) return fail(SLAB_EINVAL, "null argument");
     std::string why;
-    int o[2] = {0, 0};
+    int o[3] = {0, 0, 0};
     if (!engine_autotune(b->eng, trial_steps, o, &why)) return fail(SLAB_EDEVICE, why);
     out[0] = o[0];
     out[1] = o[1];
+    return SLAB_OK;
+}
+
+slab_status slab_batch_set_sort_capacity(slab_batch* b, int32_t particles) {
+    if (!b) return fail(SLAB_EINVAL, "null batch");
+    std::string why;
+    return engine_set_sort_capacity(b->eng, particles, &why) ? SLAB_OK : fail(SLAB_EINVAL, why);
+}
+
+slab_status slab_batch_launch_shape(slab_batch* b, int32_t out[3]) {
+    if (!b || !out) return fail(SLAB_EINVAL, "null argument");
+    int o[3] = {0, 0, 0};
+    engine_launch_shape(b->eng, o);
+    for (int k = 0; k < 3; ++k) out[k] = o[k];
+    return SLAB_OK;
+}
+
+slab_status slab_run_set_sort_capacity(slab_run* r, int32_t particles) {
+    if (!r) return fail(SLAB_EINVAL, "null run");
+    if (particles != 1024 && particles != 2048 && particles != 4096)
+        return fail(SLAB_EINVAL, "sort capacity must be 1024, 2048 or 4096 particles");
+    if (!r->f32) return SLAB_OK; // the FP64 form has one sort
+    if (!r->eng) return fail(SLAB_ESTATE, "no particles uploaded");
+    std::string why;
+    return engine_set_sort_capacity(r->eng, particles, &why) ? SLAB_OK : fail(SLAB_EINVAL, why);
+}
+
+slab_status slab_run_launch_shape(slab_run* r, int32_t out[3]) {
+    if (!r || !out) return fail(SLAB_EINVAL, "null argument");
+    if (!r->f32) { // the FP64 form has one launch shape
+        out[0] = 1;
+        out[1] = 0;
+        out[2] = 0;
+        return SLAB_OK;
+    }
+    if (!r->eng) return fail(SLAB_ESTATE, "no particles uploaded");
+    int o[3] = {0, 0, 0};
+    engine_launch_shape(r->eng, o);
+    for (int k = 0; k < 3; ++k) out[k] = o[k];
     return SLAB_OK;
 }
 
@@ -1035,7 +1074,7 @@ slab_status slab_run_autotune(slab_run* r, int32_t trial_steps, int32_t out[2]) 
         return SLAB_OK;
     }
     std::string why;
-    int o[2] = {0, 0};
+    int o[3] = {0, 0, 0};
     if (!engine_autotune(r->eng, trial_steps, o, &why)) return eng_fail(why);
     out[0] = o[0];
     out[1] = o[1];

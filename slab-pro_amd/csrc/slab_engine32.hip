@@ -611,7 +611,7 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
 
 // Per particle tile: counting sort by cell, then each cell's run sorted by
 // creation index: canonical order (slot, tile, cell, creation index).
-//  * Tiles of at most ES_CAP particles stage (cell, creation index) in LDS with
+//  * Tiles of at most CAP particles stage (cell, creation index) in LDS with
 //    one coalesced pass.  If no cell holds more than ES_RANK particles, each
 //    particle's place in its cell is its rank there: the number of the cell's
 //    particles with a smaller creation index (unique within a slot), counted
@@ -620,14 +620,16 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
 //    arrival position).
 //  * Larger tiles place their particles by cell in global memory, then sort
 //    the cells in rounds: each round bitonic-sorts a contiguous range of whole
-//    cells (at most ES_ROUND particles) in LDS; only a single cell of more than
-//    ES_ROUND particles sorts in global memory.
+//    cells (at most CAP / 2 particles) in LDS; only a single cell of more than
+//    CAP / 2 particles sorts in global memory.
 //  * Parking tiles keep their bucket order (their particles are only carried).
-constexpr int ES_CAP = 4096;
-constexpr int ES_ROUND = 2048;
+// CAP, the in-LDS sort capacity, is a launch-shape choice (1,024, 2,048 or
+// 4,096 particles): a smaller one holds fewer registers and LDS per workgroup
+// (more workgroups per CU) but sends more tiles through the rounds; the
+// autotuner picks it on the live workload.  The order does not depend on it.
 constexpr int ES_RANK = 256;
 constexpr int ES_GROUP = 4; // particles per thread whose global reads are issued together
-constexpr int ES_LDS_BYTES = ES_CAP * 8;
+constexpr int ES_CAPS[3] = {1024, 2048, 4096};
 
 // Ascending bitonic sort of k[0, P), P a power of two, by one workgroup.
 __device__ __forceinline__ void bitonic_lds(unsigned long long* k, int P) {
@@ -653,11 +655,14 @@ __device__ __forceinline__ int pow2_ceil(int v) {
     return p;
 }
 
+template <int CAP>
 __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uint8_t* __restrict__ kcell,
                                               const int* __restrict__ ptiles, int* __restrict__ tile_count,
                                               int* __restrict__ cursor, const int* __restrict__ tile_begin,
                                               const int* __restrict__ tmp,
                                               int* __restrict__ idx, int* __restrict__ cell_begin) {
+    constexpr int ROUND = CAP / 2; // keys and storage indices of a round fit the staging area
+    static_assert(CAP % (WG * ES_GROUP) == 0 && CAP <= 8192, "sort capacity");
     if (c->stop) return;
     __shared__ int s_hist[TILE_CELLS];
     __shared__ int s_off[TILE_CELLS];
@@ -666,12 +671,12 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
     // one LDS region, three uses: (a) creation index, arrival order and cell of
     // a small tile; (b) 64-bit keys of a small tile; (c) keys + storage index of
     // a round
-    __shared__ __align__(16) unsigned char s_mem[ES_LDS_BYTES];
-    int* s_pid = reinterpret_cast<int*>(s_mem);                                         // (a) ES_CAP x i32
-    unsigned short* s_ord = reinterpret_cast<unsigned short*>(s_mem + ES_CAP * 4);      // (a) ES_CAP x u16
-    unsigned char* s_cell = s_mem + ES_CAP * 6;                                         // (a) ES_CAP x u8
-    unsigned long long* s_key = reinterpret_cast<unsigned long long*>(s_mem);           // (b) ES_CAP, (c) ES_ROUND
-    int* s_sv = reinterpret_cast<int*>(s_mem + ES_ROUND * 8);                           // (c) ES_ROUND x i32
+    __shared__ __align__(16) unsigned char s_mem[CAP * 8];
+    int* s_pid = reinterpret_cast<int*>(s_mem);                                         // (a) CAP x i32
+    unsigned short* s_ord = reinterpret_cast<unsigned short*>(s_mem + CAP * 4);      // (a) CAP x u16
+    unsigned char* s_cell = s_mem + CAP * 6;                                         // (a) CAP x u8
+    unsigned long long* s_key = reinterpret_cast<unsigned long long*>(s_mem);           // (b) CAP, (c) ROUND
+    int* s_sv = reinterpret_cast<int*>(s_mem + ROUND * 8);                           // (c) ROUND x i32
     const int32_t* __restrict__ pid = ecur(c, A, B).pid;
     const int nt = c->n_ptiles, live = c->nslots * c->ntd;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
@@ -702,16 +707,16 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
             __syncthreads();
             continue;
         }
-        const bool small = n <= ES_CAP;
+        const bool small = n <= CAP;
         // storage index of each arrival this thread stages (small tiles): the
         // final write places it from registers instead of re-reading tmp
-        int svr[ES_CAP / WG];
+        int svr[CAP / WG];
         if (small) {
             // groups of ES_GROUP arrivals per thread: the bucket reads of a group
             // are issued together, then their key and creation-index reads
             // (clamped indices keep every read in the tile)
 #pragma unroll
-            for (int k0 = 0; k0 < ES_CAP / WG; k0 += ES_GROUP) {
+            for (int k0 = 0; k0 < CAP / WG; k0 += ES_GROUP) {
                 if (k0 * WG < n) {
                     int sv[ES_GROUP], kc[ES_GROUP], pd[ES_GROUP];
                     if (k0 == 0) {
@@ -795,7 +800,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
             // s_off[c] .. s_hist[c] now lists cell c's arrivals; each arrival
             // is placed from its registers at the cell start plus its rank
 #pragma unroll
-            for (int k = 0; k < ES_CAP / WG; ++k) {
+            for (int k = 0; k < CAP / WG; ++k) {
                 const int r = threadIdx.x + k * WG;
                 if (r < n) {
                     const int cr = s_cell[r], pr = s_pid[r];
@@ -808,9 +813,9 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
         } else if (small) {
             // keys (cell << 44 | creation index << 13 | arrival position), built
             // in registers first because they overwrite the staged arrays
-            unsigned long long kv[ES_CAP / WG];
+            unsigned long long kv[CAP / WG];
 #pragma unroll
-            for (int k = 0; k < ES_CAP / WG; ++k) {
+            for (int k = 0; k < CAP / WG; ++k) {
                 const int r = threadIdx.x + k * WG;
                 kv[k] = r < n ? ((unsigned long long)s_cell[r] << 44) | ((unsigned long long)(uint32_t)s_pid[r] << 13) |
                                     (unsigned long long)r
@@ -819,7 +824,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
             __syncthreads();
             const int P = pow2_ceil(n);
 #pragma unroll
-            for (int k = 0; k < ES_CAP / WG; ++k) {
+            for (int k = 0; k < CAP / WG; ++k) {
                 const int r = threadIdx.x + k * WG;
                 if (r < P) s_key[r] = kv[k];
             }
@@ -828,20 +833,20 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
             // final position of each arrival: keys to registers, then positions
             // over the keys, then each arrival placed from registers
 #pragma unroll
-            for (int k = 0; k < ES_CAP / WG; ++k) {
+            for (int k = 0; k < CAP / WG; ++k) {
                 const int q = threadIdx.x + k * WG;
                 kv[k] = q < n ? s_key[q] : 0ull;
             }
             __syncthreads();
             unsigned short* s_pos = reinterpret_cast<unsigned short*>(s_mem);
 #pragma unroll
-            for (int k = 0; k < ES_CAP / WG; ++k) {
+            for (int k = 0; k < CAP / WG; ++k) {
                 const int q = threadIdx.x + k * WG;
                 if (q < n) s_pos[(int)(kv[k] & 0x1fffull)] = (unsigned short)q;
             }
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < ES_CAP / WG; ++k) {
+            for (int k = 0; k < CAP / WG; ++k) {
                 const int r = threadIdx.x + k * WG;
                 if (r < n) idx[b + s_pos[r]] = svr[k];
             }
@@ -858,16 +863,16 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
             }
             __syncthreads();
             for (int c0 = 0; c0 < TILE_CELLS;) {
-                if (threadIdx.x == 0) { // extend the round by whole cells up to ES_ROUND particles
+                if (threadIdx.x == 0) { // extend the round by whole cells up to ROUND particles
                     const int base = s_off[c0];
                     int c1 = c0 + 1;
-                    while (c1 < TILE_CELLS && ((c1 + 1 < TILE_CELLS) ? s_off[c1 + 1] : n) - base <= ES_ROUND) ++c1;
+                    while (c1 < TILE_CELLS && ((c1 + 1 < TILE_CELLS) ? s_off[c1 + 1] : n) - base <= ROUND) ++c1;
                     s_c1 = c1;
                 }
                 __syncthreads();
                 const int c1 = s_c1;
                 const int lo = s_off[c0], len = ((c1 < TILE_CELLS) ? s_off[c1] : n) - lo;
-                if (len <= ES_ROUND) {
+                if (len <= ROUND) {
                     const int P = pow2_ceil(len > 0 ? len : 1);
                     for (int q0 = threadIdx.x; q0 < P; q0 += WG * ES_GROUP) {
                         int sv[ES_GROUP], kc[ES_GROUP], pd[ES_GROUP];
@@ -896,7 +901,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
                     __syncthreads();
                     bitonic_lds(s_key, P);
                     for (int q = threadIdx.x; q < len; q += WG) idx[b + lo + q] = s_sv[(int)(s_key[q] & 0x1fffull)];
-                } else if ((int)threadIdx.x == c0) { // one cell beyond ES_ROUND: in global memory
+                } else if ((int)threadIdx.x == c0) { // one cell beyond ROUND: in global memory
                     const int glo = b + off, ghi = b + off + cnt;
                     for (int q = glo + 1; q < ghi; ++q) {
                         const int sv = idx[q];
@@ -1620,6 +1625,7 @@ struct Engine32 {
     int64_t cnt[kNumK] = {};
     bool use_graph = false;
     int persist = kPersistE; // workgroups of the tile-list kernels (sort + P2G, grid)
+    int sort_cap = ES_CAPS[2]; // in-LDS capacity of the tile sort (particles)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int device = 0;
@@ -1716,8 +1722,15 @@ void enqueue_estep(Engine32* e, int step) {
         k_escatter<<<nb, WG, 0, s>>>(c, e->key, e->tile_begin, e->cursor, e->tmp);
     });
     tk(e, 5, step, [&] {
-        k_esort<<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->tile_count, e->cursor, e->tile_begin, e->tmp,
-                                  e->idx, e->cell_begin);
+        if (e->sort_cap == ES_CAPS[0])
+            k_esort<ES_CAPS[0]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->tile_count, e->cursor,
+                                                  e->tile_begin, e->tmp, e->idx, e->cell_begin);
+        else if (e->sort_cap == ES_CAPS[1])
+            k_esort<ES_CAPS[1]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->tile_count, e->cursor,
+                                                  e->tile_begin, e->tmp, e->idx, e->cell_begin);
+        else
+            k_esort<ES_CAPS[2]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->tile_count, e->cursor,
+                                                  e->tile_begin, e->tmp, e->idx, e->cell_begin);
     });
     tk(e, 6, step, [&] {
         k_ep2g<<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
@@ -2538,11 +2551,12 @@ int engine_use_graph(Engine32* e, int on) {
     return 0;
 }
 
-bool engine_autotune(Engine32* e, int trial_steps, int out[2], std::string* err) {
+bool engine_autotune(Engine32* e, int trial_steps, int out[3], std::string* err) {
     // Candidate launch shapes are timed on the live workload: each trial runs
     // real steps of the simulation (the numbers do not depend on the shape),
     // first the storage re-sort period at the default workgroup count, then
-    // the workgroup count at the best period.
+    // the workgroup count at the best period, then the in-LDS capacity of the
+    // tile sort at both.
     trial_steps = std::min(std::max(trial_steps, 2), 256);
     static const int kPeriods[] = {1, 4, 16, 64};
     static const int kPersist[] = {1024, 2048, 4096, 8192};
@@ -2551,9 +2565,10 @@ bool engine_autotune(Engine32* e, int trial_steps, int out[2], std::string* err)
     const bool tm = e->timing;
     e->timing = false;
     bool ok = true, finished = false;
-    auto trial = [&](int k, int p, double* ms_step) {
+    auto trial = [&](int k, int p, int cap, double* ms_step) {
         if (!engine_set_reorder(e, k, err)) return false;
         e->persist = p;
+        e->sort_cap = cap;
         drop_graph(e);
         if (!engine_step(e, 1, err)) return false; // captures the graph, warms caches
         // at least two re-sort periods, so a trial pays for its re-sorts and
@@ -2579,11 +2594,12 @@ bool engine_autotune(Engine32* e, int trial_steps, int out[2], std::string* err)
         }
         return true;
     };
-    int best_k = e->h_c->reorder_every, best_p = e->persist;
+    int best_k = e->h_c->reorder_every, best_p = e->persist, best_c = e->sort_cap;
+    const int cap0 = e->sort_cap;
     double best = 1e30;
     for (int k : kPeriods) {
         double ms = 0;
-        if (!(ok = trial(k, kPersistE, &ms)) || finished) break;
+        if (!(ok = trial(k, kPersistE, cap0, &ms)) || finished) break;
         if (ms < best) {
             best = ms;
             best_k = k;
@@ -2594,10 +2610,21 @@ bool engine_autotune(Engine32* e, int trial_steps, int out[2], std::string* err)
         for (int p : kPersist) {
             if (p == kPersistE) continue;
             double ms = 0;
-            if (!(ok = trial(best_k, p, &ms)) || finished) break;
+            if (!(ok = trial(best_k, p, cap0, &ms)) || finished) break;
             if (ms < best) {
                 best = ms;
                 best_p = p;
+            }
+        }
+    }
+    if (ok && !finished) {
+        for (int cap : ES_CAPS) {
+            if (cap == cap0) continue;
+            double ms = 0;
+            if (!(ok = trial(best_k, best_p, cap, &ms)) || finished) break;
+            if (ms < best) {
+                best = ms;
+                best_c = cap;
             }
         }
     }
@@ -2606,10 +2633,29 @@ bool engine_autotune(Engine32* e, int trial_steps, int out[2], std::string* err)
     e->timing = tm;
     if (!ok) return false;
     e->persist = best_p;
+    e->sort_cap = best_c;
     drop_graph(e);
     out[0] = best_k;
     out[1] = best_p;
+    out[2] = best_c;
     return engine_set_reorder(e, best_k, err);
+}
+
+bool engine_set_sort_capacity(Engine32* e, int cap, std::string* err) {
+    for (int c : ES_CAPS)
+        if (c == cap) {
+            e->sort_cap = cap;
+            drop_graph(e);
+            return true;
+        }
+    if (err) *err = "sort capacity must be 1024, 2048 or 4096 particles";
+    return false;
+}
+
+void engine_launch_shape(const Engine32* e, int out[3]) {
+    out[0] = e->h_c->reorder_every;
+    out[1] = e->persist;
+    out[2] = e->sort_cap;
 }
 
 bool engine_set_reorder(Engine32* e, int k, std::string* err) {

@@ -119,8 +119,14 @@ int engine_use_graph(Engine32* e, int on);             // capture the step seque
 // in place.  Results do not depend on k (the computation order is canonical).
 bool engine_set_reorder(Engine32* e, int k, std::string* err);
 // Times candidate launch shapes (storage re-sort period, workgroups of the
-// tile-list kernels) on the next steps of the live run and keeps the fastest;
-// out = {period, workgroups}.  The steps it runs are ordinary steps.
-bool engine_autotune(Engine32* e, int trial_steps, int out[2], std::string* err);
+// tile-list kernels, in-LDS capacity of the tile sort) on the next steps of the
+// live run and keeps the fastest; out = {period, workgroups, sort capacity}.
+// The steps it runs are ordinary steps.
+bool engine_autotune(Engine32* e, int trial_steps, int out[3], std::string* err);
+// In-LDS capacity of the tile sort: 1024, 2048 or 4096 particles (default
+// 4096).  Results do not depend on it.
+bool engine_set_sort_capacity(Engine32* e, int cap, std::string* err);
+// The current launch shape {re-sort period, workgroups, sort capacity}.
+void engine_launch_shape(const Engine32* e, int out[3]);
 
 } // namespace slab

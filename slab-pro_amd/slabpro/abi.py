@@ -106,6 +106,10 @@ SIGNATURES = {
     "slab_run_set_resort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "slab_batch_autotune": (ctypes.c_int, [_VP, ctypes.c_int32, _IP]),
     "slab_run_autotune": (ctypes.c_int, [_VP, ctypes.c_int32, _IP]),
+    "slab_batch_set_sort_capacity": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "slab_run_set_sort_capacity": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "slab_batch_launch_shape": (ctypes.c_int, [_VP, _IP]),
+    "slab_run_launch_shape": (ctypes.c_int, [_VP, _IP]),
     # include/slab_frontend.h
     "slab_sim_load_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
     "slab_sim_initialize": (ctypes.c_int, [_VP]),
@@ -309,7 +313,15 @@ class Run:
     def autotune(self, trial_steps=8):
         out = (ctypes.c_int32 * 2)()
         _check(lib().slab_run_autotune(self.h, int(trial_steps), out))
-        return {"resort_period": out[0], "workgroups": out[1]}
+        return self.launch_shape()
+
+    def set_sort_capacity(self, particles):
+        _check(lib().slab_run_set_sort_capacity(self.h, int(particles)))
+
+    def launch_shape(self):
+        out = (ctypes.c_int32 * 3)()
+        _check(lib().slab_run_launch_shape(self.h, out))
+        return {"resort_period": out[0], "workgroups": out[1], "sort_capacity": out[2]}
 
     def set_time(self, t, frame):
         _check(lib().slab_run_set_time(self.h, float(t), int(frame)))
@@ -416,7 +428,15 @@ class Batch:
     def autotune(self, trial_steps=8):
         out = (ctypes.c_int32 * 2)()
         _check(lib().slab_batch_autotune(self.h, int(trial_steps), out))
-        return {"resort_period": out[0], "workgroups": out[1]}
+        return self.launch_shape()
+
+    def set_sort_capacity(self, particles):
+        _check(lib().slab_batch_set_sort_capacity(self.h, int(particles)))
+
+    def launch_shape(self):
+        out = (ctypes.c_int32 * 3)()
+        _check(lib().slab_batch_launch_shape(self.h, out))
+        return {"resort_period": out[0], "workgroups": out[1], "sort_capacity": out[2]}
 
     def timing(self, enable):
         n = lib().slab_kernel_count()

@@ -79,3 +79,13 @@ def test_batch_and_probe_entry_points_reject_bad_arguments():
     assert L.slab_stream_probe(0, 1000, 1, ctypes.byref(ms)) == abi.EINVAL      # not a multiple of 16 bytes
     assert L.slab_stream_probe(0, 1 << 20, 0, ctypes.byref(ms)) == abi.EINVAL   # no iterations
     assert L.slab_stream_probe(0, 1 << 20, 1, None) == abi.EINVAL
+
+
+def test_launch_shape_entry_points_reject_bad_arguments():
+    import ctypes
+    L = abi.lib()
+    shape = (ctypes.c_int32 * 3)()
+    assert L.slab_batch_set_sort_capacity(None, 2048) == abi.EINVAL
+    assert L.slab_run_set_sort_capacity(None, 2048) == abi.EINVAL
+    assert L.slab_batch_launch_shape(None, shape) == abi.EINVAL
+    assert L.slab_run_launch_shape(None, shape) == abi.EINVAL

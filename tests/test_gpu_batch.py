@@ -42,12 +42,15 @@ def _params(total=3.0):
     return abi.params(rho=300.0, total_time=total, output_dt=0.5, precision=abi.FP32)
 
 
-def _run_batch(scens, max_slots=8, cap=200000, steps=100000, total=3.0, resort=None):
+def _run_batch(scens, max_slots=8, cap=200000, steps=100000, total=3.0, resort=None, sort_cap=None):
     from slabpro import abi
     z, dx = _terrain()
     b = abi.Batch(z, dx, max_slots, cap)
     if resort is not None:
         b.set_resort(resort)
+    if sort_cap is not None:
+        b.set_sort_capacity(sort_cap)
+        assert b.launch_shape()["sort_capacity"] == sort_cap
     slots = []
     for part, fr in scens:
         s = b.admit(_params(total), abi.friction(**fr), part["x"], part["y"], part["h"], part["A"], part["m"])
@@ -283,6 +286,24 @@ def test_dense_tiles_sort_canonically(n_per_axis, cells, corner):
     r.close()
     assert np.max(np.abs(pa["x"] - q["x"])) < 1e-3
     assert np.max(np.abs(pa["vx"] - q["vx"])) < 1e-2 * max(1.0, np.abs(q["vx"]).max())
+
+
+@pytest.mark.parametrize("n_per_axis,cells,corner", [(6, 6, 68.0), (17, 3, 66.0), (20, 4, 66.0), (5, 16, 64.0)])
+def test_sort_capacity_does_not_change_results(n_per_axis, cells, corner):
+    """The in-LDS capacity of the tile sort (1,024, 2,048 or 4,096 particles) is
+    a launch-shape choice: tiles of 1,296 (cells of 36), 2,601 (cells of 289),
+    6,400 (cells of 400) and ~6,000 particles, sorted in LDS or in rounds of
+    whole cells of at most half the capacity, give bitwise the same results."""
+    dense = (_dense(n_per_axis, cells, corner, corner), FRICS[1])
+    other = _scenario(21, FRICS[0])
+    ref = _run_batch([other, dense], cap=400000, total=0.3)
+    for sc in (1024, 2048, 4096):
+        got = _run_batch([other, dense], cap=400000, total=0.3, sort_cap=sc)
+        for (ma, pa, ia), (mb, pb, ib) in zip(ref, got):
+            assert ia["step"] == ib["step"] and ia["status"] == ib["status"], sc
+            for k in ("x", "y", "vx", "vy", "h", "z"):
+                assert np.array_equal(pa[k], pb[k]), (sc, k)
+            assert np.array_equal(ma[0], mb[0]) and np.array_equal(ma[1], mb[1]), sc
 
 
 def test_active_tile_overflow_stops_the_batch(monkeypatch):

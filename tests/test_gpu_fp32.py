@@ -133,6 +133,7 @@ def test_fp32_autotune_keeps_results():
             chosen = r.autotune(2)
             print("autotune", chosen, flush=True)
             assert chosen["resort_period"] in (1, 4, 16, 64) and chosen["workgroups"] in (1024, 2048, 4096, 8192)
+            assert chosen["sort_capacity"] in (1024, 2048, 4096)
         n, info = r.run()
         outs.append((r.particles(), r.max_maps(), info["step"]))
         r.close()

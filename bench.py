@@ -447,8 +447,10 @@ def accuracy_main(args):
 def conditioning_main(args):
     """Conditioning of the max-height map for chosen ensemble scenarios: the
     FP32 form's h_max L1 error against the FP64 form beside the change of the
-    FP64 result when every initial position moves by one FP32 offset ulp of a
-    cell (dx * 2^-24, random sign per particle and axis)."""
+    FP64 result, and of the FP32 result, when every initial position moves by
+    one FP32 offset ulp of a cell (dx * 2^-24, random sign per particle and
+    axis), and the change of the FP64 result when the friction coefficient mu
+    is rounded to FP32 or scaled by 1 +- 1e-7."""
     import numpy as np
     import slabpro
     from slabpro import abi, configs
@@ -459,7 +461,8 @@ def conditioning_main(args):
         cfg = dict(s)
         cfg["z"] = dem
         maps = {}
-        for tag, prec, pert in (("fp64", abi.FP64, False), ("fp64_moved", abi.FP64, True), ("fp32", abi.FP32, False)):
+        for tag, prec, pert in (("fp64", abi.FP64, False), ("fp64_moved", abi.FP64, True), ("fp32", abi.FP32, False),
+                                ("fp32_moved", abi.FP32, True)):
             c = dict(cfg)
             if pert:
                 g = np.random.default_rng(7)
@@ -471,12 +474,28 @@ def conditioning_main(args):
             r.run()
             maps[tag] = r.max_maps()[0]
             r.close()
+        mu = cfg["friction"].get("mu", 0.0)
+        if mu:
+            for tag, m in (("fp64_mu_f32", float(np.float32(mu))), ("fp64_mu_up", mu * (1 + 1e-7)),
+                           ("fp64_mu_dn", mu * (1 - 1e-7))):
+                c = dict(cfg)
+                c["friction"] = dict(cfg["friction"], mu=m)
+                r = slabpro.make_run(c, precision=abi.FP64)
+                r.prime()
+                r.run()
+                maps[tag] = r.max_maps()[0]
+                r.close()
         l1 = lambda a, b: float(np.abs(a - b).sum() / np.abs(b).sum())
         row = {"scenario": i, "particles": len(s["x"]), "friction": s["friction"],
-               "fp32_vs_fp64": l1(maps["fp32"], maps["fp64"]), "fp64_moved_vs_fp64": l1(maps["fp64_moved"], maps["fp64"])}
+               "fp32_vs_fp64": l1(maps["fp32"], maps["fp64"]), "fp64_moved_vs_fp64": l1(maps["fp64_moved"], maps["fp64"]),
+               "fp32_moved_vs_fp32": l1(maps["fp32_moved"], maps["fp32"])}
+        for tag in ("fp64_mu_f32", "fp64_mu_up", "fp64_mu_dn"):
+            if tag in maps:
+                row[tag + "_vs_fp64"] = l1(maps[tag], maps["fp64"])
         log(row)
         rows.append(row)
-    print(json.dumps({"metric": "h_max L1: FP32 error and FP64 sensitivity to one FP32 offset ulp", "rows": rows}),
+    print(json.dumps({"metric": "h_max L1: FP32 error; FP64 / FP32 sensitivity to one FP32 offset ulp and FP64 "
+                                "sensitivity to mu rounded to FP32 or scaled by 1 +- 1e-7", "rows": rows}),
           flush=True)
 
 

@@ -80,6 +80,19 @@ __device__ __forceinline__ int xcd_block(int nblk) {
 }
 
 
+// Element i of an array.  With O32 the caller guarantees i * sizeof(T) <
+// 2^32: the access then takes the array base from scalar registers and a
+// 32-bit byte offset, instead of 64-bit address arithmetic per lane.
+template <bool O32, typename T>
+__device__ __forceinline__ T& el(T* base, size_t i) {
+    if constexpr (O32) {
+        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+        return *reinterpret_cast<T*>(reinterpret_cast<B*>(base) + (unsigned)i * (unsigned)sizeof(T));
+    } else {
+        return base[i];
+    }
+}
+
 // True when a cell lies outside [1, N-3] on either axis (outflow).
 __device__ __forceinline__ bool cell_out(const ECtl* c, int ci, int cj) {
     return ci < 1 || ci > c->nx - 3 || cj < 1 || cj > c->ny - 3;
@@ -1208,7 +1221,7 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
 // constant mass and creation index are not rewritten).  Also writes the next
 // keys and tile counts (keys indexed like the storage) and the per-slot max
 // |v|^2.
-template <bool POOL>
+template <bool POOL, bool O32>
 __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                              const int* __restrict__ idx, const float4* __restrict__ vf,
                                              const int* __restrict__ gtile_pos,
@@ -1242,29 +1255,31 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
         dt0 = (float)slots[0].dt;
         fr0 = (float)slots[0].flip_ratio;
     }
+    // node and particle indices: 32-bit when every array's byte offsets fit
+    using IX = typename std::conditional<O32, unsigned, size_t>::type;
     int t = 0, kk = -1, sl = 0, o = 0, cb = 0;
     float v2 = 0.f;
     bool pub = false;
     if (in) {
         const int nx = c->nx, ny = c->ny;
-        const size_t nn = (size_t)nx * ny;
-        const int s = reo ? idx[r] : r;
+        const IX nn = (IX)nx * (IX)ny;
+        const int s = reo ? el<O32>(idx, r) : r;
         o = r; // storage index written
-        const int raw = P.slot[s];
+        const int raw = el<O32>(P.slot, s);
         sl = POOL ? raw & SLOT_MASK : 0;
         bool out = (raw & SLOT_OUT) != 0;
         const bool was_out = out;
         const int st = POOL ? slots[sl].status : st0;
-        int cell = P.cell[s];
-        const float2 po = P.o[s], pv = P.v[s];
-        float ox = po.x, oy = po.y, vxp = pv.x, vyp = pv.y, hp = P.h[s];
+        int cell = el<O32>(P.cell, s);
+        const float2 po = el<O32>(P.o, s), pv = el<O32>(P.v, s);
+        float ox = po.x, oy = po.y, vxp = pv.x, vyp = pv.y, hp = el<O32>(P.h, s);
         float vzp, zp; // recomputed by a running particle, carried otherwise
         if (st == SLOT_RUNNING && !c->compact && !out) {
             const float inv = (float)c->inv_dx;
             const float dt = POOL ? (float)slots[sl].dt : dt0, fr = POOL ? (float)slots[sl].flip_ratio : fr0;
             const int ci = cell >> 16, cj = cell & 0xffff;
-            const size_t fc = (size_t)ci * ny + cj;
-            const float Gx = tgx[fc], Gy = tgy[fc];
+            const IX fc = (IX)ci * (IX)ny + (IX)cj;
+            const float Gx = el<O32>(tgx, fc), Gy = el<O32>(tgy, fc);
             const int ib = ebase(ci, ox, nx - 3), jb = ebase(cj, oy, ny - 3);
             const float fx = (float)(ci - ib) + ox, fy = (float)(cj - jb) + oy;
             float wx[3], wy[3], dwx[3], dwy[3];
@@ -1284,28 +1299,39 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
                 gp[0][1] = gtile_pos[t0 + oy1];
                 gp[1][1] = gtile_pos[t0 + ox1 + oy1];
             }
+            // The weights factor by axis, w = wx[di] wy[dj]: each stencil row is
+            // first summed with wy (and dwy for the strain trace), then the
+            // rows with wx (dwx):  tr = sum dwx wy vx + wx dwy vy.
 #pragma unroll
             for (int di = 0; di < 3; ++di) {
+                // the stencil row's first node; the row's 3 nodes follow it
+                const IX k0 = (IX)(ib + di) * (IX)ny + (IX)jb;
+                const float* tzr = &el<O32>(tz, k0);
+                const float4* vfr = POOL ? vf : &el<O32>(vf, (IX)sl * nn + k0);
+                float rx = 0.f, ry = 0.f, rz = 0.f, rw = 0.f, rt = 0.f, ryd = 0.f;
 #pragma unroll
                 for (int dj = 0; dj < 3; ++dj) {
-                    const size_t k = (size_t)(ib + di) * ny + (jb + dj);
-                    const float w = wx[di] * wy[dj];
-                    size_t kv;
+                    float4 q;
                     if (POOL) {
                         const int i = ib + di, j = jb + dj;
                         const int qx = (i >> 4) - (ib >> 4), qy = (j >> 4) - (jb >> 4);
-                        kv = (size_t)gp[qx][qy] * TILE_CELLS + (((i & 15) << 4) | (j & 15));
+                        q = vf[(size_t)gp[qx][qy] * TILE_CELLS + (((i & 15) << 4) | (j & 15))];
                     } else {
-                        kv = (size_t)sl * nn + k;
+                        q = vfr[dj];
                     }
-                    const float4 q = vf[kv];
-                    picx += q.x * w;
-                    picy += q.y * w;
-                    flx += q.z * w;
-                    fly += q.w * w;
-                    z += tz[k] * w;
-                    tr += dwx[di] * wy[dj] * q.x + dwy[dj] * wx[di] * q.y;
+                    rx += wy[dj] * q.x;
+                    ry += wy[dj] * q.y;
+                    rz += wy[dj] * q.z;
+                    rw += wy[dj] * q.w;
+                    rt += wy[dj] * tzr[dj];
+                    ryd += dwy[dj] * q.y;
                 }
+                picx += wx[di] * rx;
+                picy += wx[di] * ry;
+                flx += wx[di] * rz;
+                fly += wx[di] * rw;
+                z += wx[di] * rt;
+                tr += dwx[di] * rx + wx[di] * ryd;
             }
             tr *= dt;
             const float nvx = fr * (vxp + flx) + (1.f - fr) * picx;
@@ -1336,28 +1362,28 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
                 vxp = vyp = vzp = 0.f;
             }
         } else {
-            vzp = P.vz[s];
-            zp = P.z[s];
+            vzp = el<O32>(P.vz, s);
+            zp = el<O32>(P.z, s);
         }
-        Q.cell[o] = cell;
-        Q.o[o] = make_float2(ox, oy);
-        Q.v[o] = make_float2(vxp, vyp);
-        Q.vz[o] = vzp;
-        Q.h[o] = hp;
-        Q.z[o] = zp;
+        el<O32>(Q.cell, o) = cell;
+        el<O32>(Q.o, o) = make_float2(ox, oy);
+        el<O32>(Q.v, o) = make_float2(vxp, vyp);
+        el<O32>(Q.vz, o) = vzp;
+        el<O32>(Q.h, o) = hp;
+        el<O32>(Q.z, o) = zp;
         if (reo) {
-            Q.m[o] = P.m[s];
-            Q.pid[o] = P.pid[s];
+            el<O32>(Q.m, o) = el<O32>(P.m, s);
+            el<O32>(Q.pid, o) = el<O32>(P.pid, s);
         }
-        if (reo || out != was_out) Q.slot[o] = sl | (out ? SLOT_OUT : 0);
+        if (reo || out != was_out) el<O32>(Q.slot, o) = sl | (out ? SLOT_OUT : 0);
         kk = ekey(c, st, sl, out, cell, ox, oy, &t, &cb);
         pub = slot_active(st) && !out;
         v2 = vxp * vxp + vyp * vyp;
     }
     wave_count_any(t, in && kk >= 0, tile_count);
     if (in) {
-        key[o] = kk;
-        kcell[o] = (uint8_t)cb;
+        el<O32>(key, o) = kk;
+        el<O32>(kcell, o) = (uint8_t)cb;
     }
     publish_vmax(sl, pub, v2, vmx);
 }
@@ -1633,6 +1659,9 @@ struct Engine32 {
     // batches keep the grid velocity in a pool of gtile_cap grid tiles
     // (single runs: dense over the terrain)
     bool pool_vf = false;
+    // a single run whose node and particle arrays all stay below 4 GiB: G2P
+    // addresses them with 32-bit byte offsets
+    bool off32 = false;
     int gtile_cap = 0;
     int* gtile_pos = nullptr;
     // admission assumes a scenario's active tiles may grow to this many
@@ -1747,11 +1776,14 @@ void enqueue_estep(Engine32* e, int step) {
     tk(e, 8, step, [&] {
         const unsigned gb = xcd_grid((e->launch_n + WG - 1) / WG);
         if (e->pool_vf)
-            k_eg2p<true><<<gb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->gtile_pos, e->tz, e->tgx, e->tgy,
-                                           e->key, e->kcell, e->tile_count, e->vmx);
+            k_eg2p<true, false><<<gb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->gtile_pos, e->tz,
+                                                  e->tgx, e->tgy, e->key, e->kcell, e->tile_count, e->vmx);
+        else if (e->off32)
+            k_eg2p<false, true><<<gb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->gtile_pos, e->tz,
+                                                  e->tgx, e->tgy, e->key, e->kcell, e->tile_count, e->vmx);
         else
-            k_eg2p<false><<<gb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->gtile_pos, e->tz, e->tgx,
-                                            e->tgy, e->key, e->kcell, e->tile_count, e->vmx);
+            k_eg2p<false, false><<<gb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->vf, e->gtile_pos, e->tz,
+                                                   e->tgx, e->tgy, e->key, e->kcell, e->tile_count, e->vmx);
     });
 }
 
@@ -1935,6 +1967,8 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     // a batch's grid tiles are its particle tiles plus right / upper /
     // diagonal neighbours: at most 4x, in practice well under 2x
     e->pool_vf = !single;
+    e->off32 = single && (int64_t)e->nn * (int64_t)sizeof(float4) < ((int64_t)1 << 32) &&
+               max_particles * (int64_t)sizeof(float4) < ((int64_t)1 << 32);
     e->gtile_cap = single ? e->ntiles : (int)std::min<int64_t>(e->ntiles, 2 * (int64_t)e->tile_cap);
     // smaller active-tile capacities (tests of admission and the overflow path)
     if (const char* v = std::getenv("SLAB_TILE_CAP")) {

@@ -21,6 +21,35 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+// Data-parallel-primitive (DPP) lane moves of gfx9: VALU operations on
+// registers, no LDS round trip.  Every lane of the wave must be active.
+//   row_shr:k (0x110 + k): lane i of a row of 16 takes lane i - k of its row
+//   row_bcast:15 (0x142) / row_bcast:31 (0x143): lane 15 / 31 to the next
+//     row(s) named by the row mask
+//   wave_shr:1 (0x138): lane i takes lane i - 1 of the wave
+// A lane whose source does not exist, or whose row the mask leaves out,
+// keeps `old` (0 here).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned dpp_u32(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
+}
+
+// Maximum of an unsigned value over the wave (every lane active), returned to
+// every lane: within rows of 16 by row shifts, across rows by broadcasts to
+// lane 63, then read from lane 63.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+    v = max(v, dpp_u32<0x111, 0xf>(v));
+    v = max(v, dpp_u32<0x112, 0xf>(v));
+    v = max(v, dpp_u32<0x114, 0xf>(v));
+    v = max(v, dpp_u32<0x118, 0xf>(v));
+    v = max(v, dpp_u32<0x142, 0xa>(v));
+    v = max(v, dpp_u32<0x143, 0xc>(v));
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// The value of the previous lane of the wave (lane 0 gets 0; every lane active).
+__device__ __forceinline__ int lane_prev(int v) { return (int)dpp_u32<0x138, 0xf>((unsigned)v); }
+
 // Block-wide (256 threads = 4 waves) reduction of |v|^2 max and the bounding
 // box of new positions, merged by one integer min/max atomic per field into
 // shard blockIdx % PART_SHARDS (order-independent); the consumer reduces the

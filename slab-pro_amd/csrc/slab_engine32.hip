@@ -129,19 +129,24 @@ __device__ __forceinline__ int ekey(const ECtl* c, int st, int slot, bool out, i
 // always holds one slot: it reduces across lanes and issues one atomic on its
 // shard (wave index % VMAX_SHARDS); a wave that straddles slots issues one
 // atomic per lane.  Every lane of the wave must call this.
+// The wave maximum is taken over the FP32 bits (|v|^2 >= 0 orders like its
+// bits, NaN above +inf) and widened to the FP64 bits the shards hold, which
+// is the maximum of the widened values.
 __device__ __forceinline__ void publish_vmax(int slot, bool valid, float v2, unsigned long long* __restrict__ vmx) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const unsigned long long bits = valid ? (unsigned long long)__double_as_longlong((double)v2) : 0ull;
+    const unsigned bits32 = valid ? __float_as_uint(v2) : 0u;
     const unsigned long long vmask = __ballot(valid);
     const int first = vmask ? __ffsll((long long)vmask) - 1 : 0;
     const int s0 = __shfl(slot, first, 64);
     const unsigned long long same = __ballot(valid && slot == s0);
-    const unsigned long long m = wave_max_u64(bits);
+    const unsigned m32 = wave_max_u32(bits32);
     const unsigned sh = (blockIdx.x * (WG / 64) + wv) & (VMAX_SHARDS - 1);
     if (same == vmask) {
-        if (lane == 0 && vmask) atomicMax(&vmx[(size_t)s0 * VMAX_SHARDS + sh], m);
+        if (lane == 0 && vmask)
+            atomicMax(&vmx[(size_t)s0 * VMAX_SHARDS + sh],
+                      (unsigned long long)__double_as_longlong((double)__uint_as_float(m32)));
     } else if (valid) {
-        atomicMax(&vmx[(size_t)slot * VMAX_SHARDS + sh], bits);
+        atomicMax(&vmx[(size_t)slot * VMAX_SHARDS + sh], (unsigned long long)__double_as_longlong((double)v2));
     }
 }
 
@@ -149,8 +154,8 @@ __device__ __forceinline__ void publish_vmax(int slot, bool valid, float v2, uns
 // among consecutive valid lanes.  Every lane of the wave must call this.
 __device__ __forceinline__ int wave_slot_any(int t, bool valid, int* __restrict__ tile_count) {
     const int lane = threadIdx.x & 63;
-    const int tprev = __shfl_up(t, 1, 64);
-    const bool vprev = __shfl_up((int)valid, 1, 64) != 0;
+    const int tprev = lane_prev(t);
+    const bool vprev = lane_prev((int)valid) != 0;
     const bool head = valid && (lane == 0 || !vprev || tprev != t);
     const unsigned long long vmask = __ballot(valid);
     const unsigned long long hmask = __ballot(head);
@@ -170,8 +175,8 @@ __device__ __forceinline__ int wave_slot_any(int t, bool valid, int* __restrict_
 // tile is taken later, by the scatter).  Every lane of the wave must call this.
 __device__ __forceinline__ void wave_count_any(int t, bool valid, int* __restrict__ tile_count) {
     const int lane = threadIdx.x & 63;
-    const int tprev = __shfl_up(t, 1, 64);
-    const bool vprev = __shfl_up((int)valid, 1, 64) != 0;
+    const int tprev = lane_prev(t);
+    const bool vprev = lane_prev((int)valid) != 0;
     const bool head = valid && (lane == 0 || !vprev || tprev != t);
     const unsigned long long vmask = __ballot(valid);
     const unsigned long long hmask = __ballot(head);

@@ -1974,6 +1974,9 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
     e->pool_vf = !single;
     e->off32 = single && (int64_t)e->nn * (int64_t)sizeof(float4) < ((int64_t)1 << 32) &&
                max_particles * (int64_t)sizeof(float4) < ((int64_t)1 << 32);
+    // the 64-bit addressing of larger runs, on any run (tests of that path)
+    if (const char* v = std::getenv("SLAB_G2P_OFFSET64"))
+        if (std::atoi(v) != 0) e->off32 = false;
     e->gtile_cap = single ? e->ntiles : (int)std::min<int64_t>(e->ntiles, 2 * (int64_t)e->tile_cap);
     // smaller active-tile capacities (tests of admission and the overflow path)
     if (const char* v = std::getenv("SLAB_TILE_CAP")) {

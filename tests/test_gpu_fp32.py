@@ -165,3 +165,24 @@ def test_fp32_c0_full_run_error_report():
     assert rep["vmax_L1"] <= 0.02
     assert rep["runout_diff_cells"] <= 2.0
     assert abs(rep["mass_fp32"] - rep["mass_fp64"]) <= 1e-6 * rep["mass_fp64"]
+
+
+def test_fp32_g2p_addressing_forms_agree(monkeypatch):
+    """A single run addresses G2P's arrays with 32-bit byte offsets when they
+    fit (every workload here) and with 64-bit offsets otherwise; forcing the
+    64-bit form (SLAB_G2P_OFFSET64, read when a run is created) gives bitwise
+    the same particles and max maps."""
+    from slabpro import configs
+    cfg = configs.c0()
+    outs = []
+    for force64 in (False, True):
+        if force64:
+            monkeypatch.setenv("SLAB_G2P_OFFSET64", "1")
+        r = _mk(cfg, 32, keep=False)
+        r.prime()
+        r.step(200)
+        outs.append((r.particles(), r.max_maps()))
+        r.close()
+    for k in outs[0][0]:
+        assert np.array_equal(outs[0][0][k], outs[1][0][k]), k
+    assert np.array_equal(outs[0][1][0], outs[1][1][0]) and np.array_equal(outs[0][1][1], outs[1][1][1])

@@ -21,6 +21,8 @@ untimed + 50 timed steps from the t = 0 prime and from the run's state at
 t = 60 s, with the host's core count and CPU model.  ensemble: the
 256-scenario ensemble (FP32 batched scenarios/h, the FP64 one-at-a-time rate
 on a stratified sample and each sampled scenario's max-height L1 error).
+serving: a bounded sample of the request server (300 requests at 5 per
+second: p50/p95/p99 completion latency and scenarios/h).
 """
 import argparse
 import json
@@ -211,6 +213,29 @@ def ensemble_numbers(scenarios=256, fp64_sample=16):
     }
 
 
+def serving_sample(n=300, lam=5.0, slots=128, capacity=24_000_000):
+    """A bounded sample of the serving workload for the default line: n
+    seeded requests from the ensemble distribution (configs.c3_scenario(1000
+    + k)) on the 2000x2000 DEM, open-loop Poisson arrivals at lam per second,
+    continuous batching (ensemble.serve) with admission by device footprint;
+    p50/p95/p99 completion latency and scenarios/h.  The full 2,000 requests
+    per rate at 2, 5 and 10 per second run with --serve."""
+    from slabpro import configs, ensemble
+    t0 = time.time()
+    dem = configs.c1_dem()
+    reqs = [configs.c3_scenario(1000 + k) for k in range(n)]
+    arr, _ = configs.c4_requests(n, lam, seed_index=int(lam * 10))
+    log(f"serving sample: {n} requests generated in {time.time() - t0:.1f}s")
+    r = ensemble.serve(dem, 5.0, reqs, arr, max_slots=slots, capacity=capacity, progress=log)
+    log(f"serving sample: p50 {r['p50_s']:.2f}s p95 {r['p95_s']:.2f}s p99 {r['p99_s']:.2f}s, "
+        f"{r['scenarios_per_hour']:.0f} scen/h over {r['wall_s']:.1f}s")
+    return {"workload": f"c4 sample: {n} seeded requests (c3 distribution, 2000x2000 DEM), Poisson arrivals "
+                        f"at {lam:g}/s, {slots} slots, {capacity / 1e6:g}M-particle capacity",
+            "served": r["served"], "rejected": len(r["rejected"]), "p50_s": r["p50_s"], "p95_s": r["p95_s"],
+            "p99_s": r["p99_s"], "mean_s": r["mean_s"], "scenarios_per_hour": r["scenarios_per_hour"],
+            "wall_s": r["wall_s"], "mean_active_particles": r["mean_active_particles"]}
+
+
 def copy_probe():
     """Streaming kernels of known size written for gfx950 (slab_stream_probe):
     the practical HBM ceiling for a read+write copy, a read, a write, and
@@ -257,6 +282,7 @@ def main():
     ap.add_argument("--cpu-mid", type=float, default=60.0,
                     help="simulated time (s) of the CPU baseline's mid-run sample (0: t = 0 sample only)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the bounded serving sample of the default line")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 256-scenario ensemble numbers in the default line")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (FP32 form)")
@@ -380,6 +406,11 @@ def main():
     ens = None
     if not args.no_ensemble and args.workload == "c2" and args.form == "fp32":
         ens = ensemble_numbers()
+    srv = None
+    if not args.no_serve and args.workload == "c2" and args.form == "fp32":
+        srv = serving_sample()
+        if ens:
+            srv["vs_fp64_one_at_a_time"] = srv["scenarios_per_hour"] / ens["fp64_one_at_a_time_scenarios_per_hour"]
 
     out = {
         "metric": "particle-updates/s at 8M particles",
@@ -410,6 +441,7 @@ def main():
         "stream_ceiling": ceiling,
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "serving": srv,
     }
     out["roofline"]["frac_of_copy_ceiling"] = achieved / ceiling["copy_1GiB_GBs"]
     print(json.dumps(out), flush=True)

@@ -150,26 +150,6 @@ __device__ __forceinline__ void publish_vmax(int slot, bool valid, float v2, uns
     }
 }
 
-// Tile slots for arbitrary valid masks: one atomic per run of equal tiles
-// among consecutive valid lanes.  Every lane of the wave must call this.
-__device__ __forceinline__ int wave_slot_any(int t, bool valid, int* __restrict__ tile_count) {
-    const int lane = threadIdx.x & 63;
-    const int tprev = lane_prev(t);
-    const bool vprev = lane_prev((int)valid) != 0;
-    const bool head = valid && (lane == 0 || !vprev || tprev != t);
-    const unsigned long long vmask = __ballot(valid);
-    const unsigned long long hmask = __ballot(head);
-    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const unsigned long long hle = hmask & upto;
-    const unsigned long long brk = (hmask | ~vmask) & ~upto; // next head or next invalid lane
-    const int start = hle ? 63 - __clzll((long long)hle) : 0;
-    const int end = brk ? (__ffsll((long long)brk) - 1) : 64;
-    int base = 0;
-    if (head) base = atomicAdd(&tile_count[t], end - start);
-    base = __shfl(base, start, 64);
-    return base + (lane - start);
-}
-
 // Counts each lane's particle into its tile: one atomic per run of equal tiles
 // among consecutive valid lanes, and no value returned (the slot inside the
 // tile is taken later, by the scatter).  Every lane of the wave must call this.
@@ -690,11 +670,11 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
     // a small tile; (b) 64-bit keys of a small tile; (c) keys + storage index of
     // a round
     __shared__ __align__(16) unsigned char s_mem[CAP * 8];
-    int* s_pid = reinterpret_cast<int*>(s_mem);                                         // (a) CAP x i32
-    unsigned short* s_ord = reinterpret_cast<unsigned short*>(s_mem + CAP * 4);      // (a) CAP x u16
-    unsigned char* s_cell = s_mem + CAP * 6;                                         // (a) CAP x u8
-    unsigned long long* s_key = reinterpret_cast<unsigned long long*>(s_mem);           // (b) CAP, (c) ROUND
-    int* s_sv = reinterpret_cast<int*>(s_mem + ROUND * 8);                           // (c) ROUND x i32
+    int* s_pid = reinterpret_cast<int*>(s_mem);                                    // (a) CAP x i32
+    unsigned short* s_ord = reinterpret_cast<unsigned short*>(s_mem + CAP * 4);    // (a) CAP x u16
+    unsigned char* s_cell = s_mem + CAP * 6;                                       // (a) CAP x u8
+    unsigned long long* s_key = reinterpret_cast<unsigned long long*>(s_mem);      // (b) CAP, (c) ROUND
+    int* s_sv = reinterpret_cast<int*>(s_mem + ROUND * 8);                         // (c) ROUND x i32
     const int32_t* __restrict__ pid = ecur(c, A, B).pid;
     const int nt = c->n_ptiles, live = c->nslots * c->ntd;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {

@@ -999,18 +999,22 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         for (int k = 0; k < 9; ++k)
 #pragma unroll
             for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
-        // the next particle's loads are issued before the current one is summed;
-        // the vz momentum (grid vz output only) is summed on output steps only
+        // the next particle's loads are issued before the current one is
+        // summed, and the index of the one after it a particle further ahead,
+        // so no iteration waits for an index load; the vz momentum (grid vz
+        // output only) is summed on output steps only
         auto sum_particles = [&](auto vz_tag) {
         constexpr bool VZ = decltype(vz_tag)::value;
+        int sn_next = r0 + 1 < r1 ? idx[r0 + 1] : 0;
         for (int r = r0; r < r1; ++r) {
             const float ox = nox, oy = noy, mp = nmp, hp = nhp, pvx = nvx, pvy = nvy, pvz = nvz;
             if (r + 1 < r1) {
-                const int sn = idx[r + 1];
+                const int sn = sn_next;
                 const float2 po = P.o[sn], pv = P.v[sn];
                 nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
                 nvx = pv.x; nvy = pv.y;
                 if (VZ) nvz = P.vz[sn];
+                if (r + 2 < r1) sn_next = idx[r + 2];
             }
             // live particles sit in cells [1, N-3] (outflow parks the others),
             // so the stencil base is never clamped and the floor cell is the

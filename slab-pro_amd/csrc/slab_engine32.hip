@@ -507,7 +507,8 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
                                                  int* __restrict__ ptiles, int* __restrict__ gtiles,
                                                  int* __restrict__ pstamp, int* __restrict__ gstamp,
                                                  int* __restrict__ tile_pos, int* __restrict__ gtile_pos,
-                                                 int* __restrict__ hm_map, const int* __restrict__ hm_stack) {
+                                                 int* __restrict__ hm_map, const int* __restrict__ hm_stack,
+                                                 int2* __restrict__ pinfo) {
     if (c->stop) return;
     const bool idle = scan_block_idle(c, slots, ntiles);
     if (idle && blockIdx.x != 0) return; // block 0 writes the totals even when idle
@@ -539,6 +540,7 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
         tile_begin[t] = run[0];
         if (f[k][1]) {
             ptiles[run[1]] = t;
+            pinfo[run[1]] = make_int2(run[0], f[k][0]); // the tile's begin and count, by list position
             tile_pos[t] = run[1];
             pstamp[t] = ep;
         }
@@ -655,7 +657,8 @@ __device__ __forceinline__ int pow2_ceil(int v) {
 
 template <int CAP>
 __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uint8_t* __restrict__ kcell,
-                                              const int* __restrict__ ptiles, int* __restrict__ tile_count,
+                                              const int* __restrict__ ptiles, const int2* __restrict__ pinfo,
+                                              int* __restrict__ tile_count,
                                               int* __restrict__ cursor, const int* __restrict__ tile_begin,
                                               const int* __restrict__ tmp,
                                               int* __restrict__ idx, int* __restrict__ cell_begin) {
@@ -678,8 +681,11 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
     const int32_t* __restrict__ pid = ecur(c, A, B).pid;
     const int nt = c->n_ptiles, live = c->nslots * c->ntd;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        // the tile, its begin and its count, by list position: read-only here,
+        // so read together through the scalar cache
         const int t = ptiles[w];
-        const int n = tile_count[t], b = tile_begin[t]; // n >= 1: listed tiles hold particles
+        const int2 nb = pinfo[w];
+        const int n = nb.y, b = nb.x; // n >= 1: listed tiles hold particles
         // the first group of bucket reads and their key and creation-index
         // reads touch no LDS: issued before the barrier, they overlap it
         int sv0[ES_GROUP], kc0[ES_GROUP], pd0[ES_GROUP];
@@ -1615,6 +1621,7 @@ struct Engine32 {
     int *tile_count = nullptr, *cursor = nullptr, *tile_begin = nullptr, *pstamp = nullptr, *gstamp = nullptr,
         *tile_pos = nullptr;
     int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr;
+    int2* pinfo = nullptr; // per particle-tile list position: {begin, count}
     int* cell_begin = nullptr;
     float* tpart = nullptr;
     PB32 A{}, B{};
@@ -1733,7 +1740,7 @@ void enqueue_estep(Engine32* e, int step) {
     tk(e, 3, step, [&] {
         k_eoffsets<<<e->nblk, WG, 0, s>>>(c, e->d_slots, e->tile_count, e->ntiles, e->boff, e->bsum, e->nblk, merged,
                                            e->tile_begin, e->ptiles, e->gtiles, e->pstamp, e->gstamp, e->tile_pos,
-                                           e->gtile_pos, e->hm_map, e->hm_stack);
+                                           e->gtile_pos, e->hm_map, e->hm_stack, e->pinfo);
     });
     tk(e, 4, step, [&] {
         const unsigned nb = (unsigned)std::max<int64_t>(1, (e->launch_n + WG * SC_PER - 1) / (WG * SC_PER));
@@ -1741,13 +1748,13 @@ void enqueue_estep(Engine32* e, int step) {
     });
     tk(e, 5, step, [&] {
         if (e->sort_cap == ES_CAPS[0])
-            k_esort<ES_CAPS[0]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->tile_count, e->cursor,
+            k_esort<ES_CAPS[0]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count, e->cursor,
                                                   e->tile_begin, e->tmp, e->idx, e->cell_begin);
         else if (e->sort_cap == ES_CAPS[1])
-            k_esort<ES_CAPS[1]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->tile_count, e->cursor,
+            k_esort<ES_CAPS[1]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count, e->cursor,
                                                   e->tile_begin, e->tmp, e->idx, e->cell_begin);
         else
-            k_esort<ES_CAPS[2]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->tile_count, e->cursor,
+            k_esort<ES_CAPS[2]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count, e->cursor,
                                                   e->tile_begin, e->tmp, e->idx, e->cell_begin);
     });
     tk(e, 6, step, [&] {
@@ -2001,6 +2008,7 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
               ealloc(e->allocs, &e->tile_begin, e->ntiles, &why) &&
               ealloc(e->allocs, &e->pstamp, e->ntiles, &why) && ealloc(e->allocs, &e->gstamp, e->ntiles, &why) &&
               ealloc(e->allocs, &e->tile_pos, e->ntiles, &why) && ealloc(e->allocs, &e->ptiles, e->ntiles, &why) &&
+              ealloc(e->allocs, &e->pinfo, e->ntiles, &why) &&
               ealloc(e->allocs, &e->gtiles, e->ntiles, &why) && ealloc(e->allocs, &e->bsum, 3 * (size_t)e->nblk, &why) &&
               ealloc(e->allocs, &e->boff, 3 * (size_t)e->nblk, &why) &&
               ealloc(e->allocs, &e->cell_begin, (size_t)e->tile_cap * CELL_STRIDE, &why) &&

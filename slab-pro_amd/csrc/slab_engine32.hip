@@ -589,11 +589,16 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
         valid[j] = k >= 0;
         t[j] = valid[j] ? k : 0;
     }
+    // the tiles' begins and the previous lane's tile of every group first
+    // (all in flight together), then the run heads and cursor atomics
+    int tb[SC_PER], tprev[SC_PER];
+#pragma unroll
+    for (int j = 0; j < SC_PER; ++j) tb[j] = valid[j] ? tile_begin[t[j]] : 0;
+#pragma unroll
+    for (int j = 0; j < SC_PER; ++j) tprev[j] = __shfl_up(valid[j] ? t[j] : -1, 1, 64);
 #pragma unroll
     for (int j = 0; j < SC_PER; ++j) {
-        const int tprev = __shfl_up(t[j], 1, 64);
-        const bool vprev = __shfl_up((int)valid[j], 1, 64) != 0;
-        const bool head = valid[j] && (lane == 0 || !vprev || tprev != t[j]);
+        const bool head = valid[j] && (lane == 0 || tprev[j] != t[j]);
         const unsigned long long vmask = __ballot(valid[j]);
         const unsigned long long hmask = __ballot(head);
         const unsigned long long hle = hmask & upto;
@@ -605,7 +610,7 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
 #pragma unroll
     for (int j = 0; j < SC_PER; ++j) {
         const int b = __shfl(base[j], start[j], 64);
-        if (valid[j]) tmp[tile_begin[t[j]] + b + (lane - start[j])] = s0 + 64 * j;
+        if (valid[j]) tmp[tb[j] + b + (lane - start[j])] = s0 + 64 * j;
     }
 }
 

@@ -723,7 +723,12 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
         if (small) {
             // groups of ES_GROUP arrivals per thread: the bucket reads of a group
             // are issued together, then their key and creation-index reads
-            // (clamped indices keep every read in the tile)
+            // (clamped indices keep every read in the tile); the next group's
+            // bucket reads are issued before this group's keys are waited for
+            int svn[ES_GROUP];
+#pragma unroll
+            for (int j = 0; j < ES_GROUP; ++j)
+                svn[j] = ES_GROUP * WG < n ? tmp[b + min((int)threadIdx.x + (ES_GROUP + j) * WG, n - 1)] : 0;
 #pragma unroll
             for (int k0 = 0; k0 < CAP / WG; k0 += ES_GROUP) {
                 if (k0 * WG < n) {
@@ -737,8 +742,12 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
                         }
                     } else {
 #pragma unroll
-                        for (int j = 0; j < ES_GROUP; ++j)
-                            sv[j] = tmp[b + min((int)threadIdx.x + (k0 + j) * WG, n - 1)];
+                        for (int j = 0; j < ES_GROUP; ++j) sv[j] = svn[j];
+                        if (k0 + ES_GROUP < CAP / WG && (k0 + ES_GROUP) * WG < n) {
+#pragma unroll
+                            for (int j = 0; j < ES_GROUP; ++j)
+                                svn[j] = tmp[b + min((int)threadIdx.x + (k0 + ES_GROUP + j) * WG, n - 1)];
+                        }
 #pragma unroll
                         for (int j = 0; j < ES_GROUP; ++j) {
                             kc[j] = kcell[sv[j]];

@@ -283,6 +283,9 @@ def main():
                     help="simulated time (s) of the CPU baseline's mid-run sample (0: t = 0 sample only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the bounded serving sample of the default line")
+    ap.add_argument("--batch-window", action="store_true",
+                    help="only a window of the 256-scenario batch with eager launches (kernel traces, counters)")
+    ap.add_argument("--batch-start", type=int, default=0, help="batch window: untimed steps before it")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 256-scenario ensemble numbers in the default line")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (FP32 form)")
@@ -318,6 +321,8 @@ def main():
 
     if args.ensemble:
         return ensemble_main(args)
+    if args.batch_window:
+        return batch_window_main(args)
     if args.serve:
         return serve_main(args)
     if args.accuracy:
@@ -456,6 +461,41 @@ def ensemble_main(args):
            "unit": "scenarios/h", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
            "data": "synthetic (seeded DEM + ellipse releases)", "config": {"workload": e["workload"]},
            "ensemble": e}
+    print(json.dumps(out), flush=True)
+
+
+def batch_window_main(args):
+    """The 256-scenario batch for a window of steps, every launch eager (no
+    hipGraph), for kernel traces and counters of the batched kernels: all
+    scenarios admitted at once, `--batch-start` untimed steps, then `--steps`
+    steps with HIP-event kernel times."""
+    from slabpro import abi, configs, ensemble
+    dem = configs.c1_dem()
+    scen = [configs.c3_scenario(i) for i in range(args.scenarios)]
+    cap = sum(len(x["x"]) for x in scen)
+    b = abi.Batch(dem, 5.0, len(scen), cap)
+    b.use_graph(False)
+    for x in scen:
+        if b.admit(ensemble._params(x, abi.FP32), ensemble._fric(x), x["x"], x["y"], x["h"], x["A"], x["m"]) is None:
+            raise RuntimeError("a scenario does not fit the batch")
+    done = 0
+    while done < args.batch_start:  # Batch.step returns the number of active slots
+        k = min(64, args.batch_start - done)
+        b.step(k)
+        done += k
+    b.timing(True)
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    el = time.perf_counter() - t0
+    n = args.steps
+    tim = b.timing(False)
+    b.close()
+    out = {"metric": "batched step kernels (256-scenario batch window)", "value": 1e3 * el / max(n, 1),
+           "unit": "ms/step (eager launches, event-timed)", "n_gpus": 1, "higher_is_better": False,
+           "dtype": "f32", "data": "synthetic (seeded DEM + ellipse releases)",
+           "config": {"workload": f"c3: {len(scen)} scenarios, {cap} particles, steps {args.batch_start}.."
+                                  f"{args.batch_start + n}"},
+           "kernels_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in tim.items() if v[1]}}
     print(json.dumps(out), flush=True)
 
 

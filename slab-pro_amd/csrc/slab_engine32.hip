@@ -998,20 +998,31 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         if (need_vz && r0 < r1) nvz = P.vz[sn0];
         const int tx = local / nty, ty = local - tx * nty;
         for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
-        // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged once
-        for (int q = threadIdx.x; q < TCELLS * TCELLS; q += WG) {
+        // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged
+        // once, every load of a thread issued before its first LDS write
+        constexpr int TPT = (TCELLS * TCELLS + WG - 1) / WG;
+        float tv[TPT][3];
+#pragma unroll
+        for (int i = 0; i < TPT; ++i) {
+            const int q = threadIdx.x + i * WG;
             const int a = q / TCELLS, b = q - a * TCELLS;
             const int gi = tx * TILE + a, gj = ty * TILE + b;
-            float vx = 0.f, vy = 0.f, vc = 0.f;
-            if (gi < nx && gj < ny) {
+            tv[i][0] = tv[i][1] = tv[i][2] = 0.f;
+            if (q < TCELLS * TCELLS && gi < nx && gj < ny) {
                 const size_t k = (size_t)gi * ny + gj;
-                vx = tgx[k];
-                vy = tgy[k];
-                vc = tcth[k];
+                tv[i][0] = tgx[k];
+                tv[i][1] = tgy[k];
+                tv[i][2] = tcth[k];
             }
-            s_ter[q] = vx;
-            s_ter[TCELLS * TCELLS + q] = vy;
-            s_ter[2 * TCELLS * TCELLS + q] = vc;
+        }
+#pragma unroll
+        for (int i = 0; i < TPT; ++i) {
+            const int q = threadIdx.x + i * WG;
+            if (q < TCELLS * TCELLS) {
+                s_ter[q] = tv[i][0];
+                s_ter[TCELLS * TCELLS + q] = tv[i][1];
+                s_ter[2 * TCELLS * TCELLS + q] = tv[i][2];
+            }
         }
         __syncthreads();
         float acc[9][P2G_FIELDS];

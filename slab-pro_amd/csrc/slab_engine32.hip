@@ -1240,7 +1240,10 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
 // other buffer in canonical order, otherwise updates them in place (the
 // constant mass and creation index are not rewritten).  Also writes the next
 // keys and tile counts (keys indexed like the storage) and the per-slot max
-// |v|^2.
+// |v|^2.  The particle elevation z (simulation.cpp:930) is read by nothing but
+// the particle output of a frame (save_function.cpp:58): it is interpolated on
+// the slot's output steps only (frames, the prime, every step with keep_grid)
+// and carried unchanged otherwise.
 template <bool POOL, bool O32>
 __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                              const int* __restrict__ idx, const float4* __restrict__ vf,
@@ -1268,10 +1271,11 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
     // A single run (dense layout) has one slot: its status, dt and FLIP ratio
     // are read once per workgroup, so no particle waits for a per-particle
     // slot lookup before its stencil reads.
-    int st0 = 0;
+    int st0 = 0, zo0 = 0;
     float dt0 = 0.f, fr0 = 0.f;
     if (!POOL) {
         st0 = slots[0].status;
+        zo0 = slots[0].out_step;
         dt0 = (float)slots[0].dt;
         fr0 = (float)slots[0].flip_ratio;
     }
@@ -1293,7 +1297,8 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
         int cell = el<O32>(P.cell, s);
         const float2 po = el<O32>(P.o, s), pv = el<O32>(P.v, s);
         float ox = po.x, oy = po.y, vxp = pv.x, vyp = pv.y, hp = el<O32>(P.h, s);
-        float vzp, zp; // recomputed by a running particle, carried otherwise
+        float vzp, zp = 0.f; // recomputed by a running particle, carried otherwise
+        bool zset = false;   // zp holds this step's elevation
         if (st == SLOT_RUNNING && !c->compact && !out) {
             const float inv = (float)c->inv_dx;
             const float dt = POOL ? (float)slots[sl].dt : dt0, fr = POOL ? (float)slots[sl].flip_ratio : fr0;
@@ -1326,9 +1331,8 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
             for (int di = 0; di < 3; ++di) {
                 // the stencil row's first node; the row's 3 nodes follow it
                 const IX k0 = (IX)(ib + di) * (IX)ny + (IX)jb;
-                const float* tzr = &el<O32>(tz, k0);
                 const float4* vfr = POOL ? vf : &el<O32>(vf, (IX)sl * nn + k0);
-                float rx = 0.f, ry = 0.f, rz = 0.f, rw = 0.f, rt = 0.f, ryd = 0.f;
+                float rx = 0.f, ry = 0.f, rz = 0.f, rw = 0.f, ryd = 0.f;
 #pragma unroll
                 for (int dj = 0; dj < 3; ++dj) {
                     float4 q;
@@ -1343,15 +1347,26 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
                     ry += wy[dj] * q.y;
                     rz += wy[dj] * q.z;
                     rw += wy[dj] * q.w;
-                    rt += wy[dj] * tzr[dj];
                     ryd += dwy[dj] * q.y;
                 }
                 picx += wx[di] * rx;
                 picy += wx[di] * ry;
                 flx += wx[di] * rz;
                 fly += wx[di] * rw;
-                z += wx[di] * rt;
                 tr += dwx[di] * rx + wx[di] * ryd;
+            }
+            // the elevation, on output steps, after the velocity gathers (its
+            // loads do not add to their registers)
+            const bool zl = (POOL ? slots[sl].out_step : zo0) != 0;
+            if (zl) {
+#pragma unroll
+                for (int di = 0; di < 3; ++di) {
+                    const float* tzr = &el<O32>(tz, (IX)(ib + di) * (IX)ny + (IX)jb);
+                    float rt = 0.f;
+#pragma unroll
+                    for (int dj = 0; dj < 3; ++dj) rt += wy[dj] * tzr[dj];
+                    z += wx[di] * rt;
+                }
             }
             tr *= dt;
             const float nvx = fr * (vxp + flx) + (1.f - fr) * picx;
@@ -1377,20 +1392,23 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
             vzp = nvx * Gx + nvy * Gy;
             hp = hp / (1.f + tr);
             zp = z;
+            zset = zl;
             if (cell_out(c, nci, ncj)) { // outflow: frozen from here on
                 out = true;
                 vxp = vyp = vzp = 0.f;
             }
         } else {
             vzp = el<O32>(P.vz, s);
-            zp = el<O32>(P.z, s);
         }
+        // the elevation is carried (re-sort steps move it) unless this step
+        // set it
+        if (!zset && reo) zp = el<O32>(P.z, s);
         el<O32>(Q.cell, o) = cell;
         el<O32>(Q.o, o) = make_float2(ox, oy);
         el<O32>(Q.v, o) = make_float2(vxp, vyp);
         el<O32>(Q.vz, o) = vzp;
         el<O32>(Q.h, o) = hp;
-        el<O32>(Q.z, o) = zp;
+        if (zset || reo) el<O32>(Q.z, o) = zp;
         if (reo) {
             el<O32>(Q.m, o) = el<O32>(P.m, s);
             el<O32>(Q.pid, o) = el<O32>(P.pid, s);

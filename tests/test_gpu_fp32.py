@@ -120,6 +120,36 @@ def test_fp32_storage_resort_period_does_not_change_results(period):
             assert np.array_equal(a[k], b[k]), k
 
 
+def test_fp32_elevation_at_output_frames():
+    """The particle elevation (simulation.cpp:930), read by nothing but a frame's
+    particle output (save_function.cpp:58), is interpolated on output steps
+    only: at every frame a run without keep_grid holds bitwise the particles of
+    one whose every step is an output step, and its z tracks the FP64 form's
+    (pinned to the oracle) to 1e-5 of the largest |z| (measured ~1e-7)."""
+    from slabpro import configs
+    cfg = configs.c0()
+    runs = [_mk(cfg, 32, keep=False), _mk(cfg, 32, keep=True), _mk(cfg, 64, keep=False)]
+    for r in runs:
+        r.prime()
+    for frame in range(3):                      # three output frames (1 s apart)
+        infos = []
+        for r in runs:
+            while True:
+                info = r.step(1 << 20)
+                if info["frame_hit"]:
+                    break
+            infos.append(info)
+        assert infos[0]["step"] == infos[1]["step"] and infos[0]["frame"] == infos[2]["frame"] == frame + 1
+        pa, pk, p64 = (r.particles() for r in runs)
+        for k in pa:
+            assert np.array_equal(pa[k], pk[k]), (frame, k)
+        e = _rel(pa["z"], p64["z"])
+        print("frame", frame + 1, "z vs fp64", e, flush=True)
+        assert e <= 1e-5
+    for r in runs:
+        r.close()
+
+
 def test_fp32_autotune_keeps_results():
     """The autotuner's trial steps are ordinary steps: a tuned run ends bitwise
     where an untuned one does."""

@@ -997,7 +997,9 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         const bool need_vz = slots[slot].out_step != 0;
         if (need_vz && r0 < r1) nvz = P.vz[sn0];
         const int tx = local / nty, ty = local - tx * nty;
-        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) s_acc[q] = 0.f;
+        // the region's fields; P_z (the last) on output steps only
+        const int nreg = (need_vz ? P2G_FIELDS : P2G_PZ) * REGION_NODES;
+        for (int q = threadIdx.x; q < nreg; q += WG) s_acc[q] = 0.f;
         // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged
         // once, every load of a thread issued before its first LDS write
         constexpr int TPT = (TCELLS * TCELLS + WG - 1) / WG;
@@ -1080,10 +1082,10 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
                         a[0] = fmaf(mp, wgt, a[0]);
                         a[1] = fmaf(vmx, wgt, a[1]);
                         a[2] = fmaf(vmy, wgt, a[2]);
-                        if (VZ) a[3] = fmaf(vzm, wgt, a[3]);
-                        a[4] = fmaf(hm, wgt, a[4]);
-                        a[5] = fmaf(-psx[d], wy[e], fmaf(mbx, wgt, a[5]));
-                        a[6] = fmaf(-psy[e], wx[d], fmaf(mby, wgt, a[6]));
+                        a[3] = fmaf(hm, wgt, a[3]);
+                        a[4] = fmaf(-psx[d], wy[e], fmaf(mbx, wgt, a[4]));
+                        a[5] = fmaf(-psy[e], wx[d], fmaf(mby, wgt, a[5]));
+                        if (VZ) a[P2G_PZ] = fmaf(vzm, wgt, a[P2G_PZ]);
                     }
                 }
             }
@@ -1098,11 +1100,12 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         for (int ph = 0; ph < 9; ++ph) {
             const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
 #pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) s_acc[f * REGION_NODES + node] += acc[ph][f];
+            for (int f = 0; f < P2G_FIELDS; ++f)
+                if (f < P2G_PZ || need_vz) s_acc[f * REGION_NODES + node] += acc[ph][f];
             __syncthreads();
         }
         float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
-        for (int q = threadIdx.x; q < P2G_FIELDS * REGION_NODES; q += WG) dst[q] = s_acc[q];
+        for (int q = threadIdx.x; q < nreg; q += WG) dst[q] = s_acc[q];
         __syncthreads();
     }
 }
@@ -1174,12 +1177,14 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
             ok[q] = has[q] && st == ep;
             pos[q] = ps;
         }
+        // P_z (the last field) is stored on output steps only
+        const bool pz = S.out_step != 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (!ok[q]) continue;
             const float* pp = tpart + (size_t)pos[q] * (P2G_FIELDS * REGION_NODES) + rn[q];
 #pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) val[q][f] = pp[f * REGION_NODES];
+            for (int f = 0; f < P2G_FIELDS; ++f) val[q][f] = (f < P2G_PZ || pz) ? pp[f * REGION_NODES] : 0.f;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1193,9 +1198,9 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
         if (am > 1e-6f) {
             const float rm = 1.0f / am;
             const float vx = acc[1] * rm, vy = acc[2] * rm;
-            vz = acc[3] * rm;
-            hh = acc[4] * rm;
-            const float vtx = vx + dt * (acc[5] * rm), vty = vy + dt * (acc[6] * rm);
+            vz = acc[P2G_PZ] * rm;
+            hh = acc[3] * rm;
+            const float vtx = vx + dt * (acc[4] * rm), vty = vy + dt * (acc[5] * rm);
             const float vsq = vtx * vtx + vty * vty;
             if (vsq > 1e-16f) {
                 const float v_z = vtx * Gx + vty * Gy;

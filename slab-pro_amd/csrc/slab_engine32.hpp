@@ -48,7 +48,8 @@ constexpr int32_t SLOT_MASK = SLOT_OUT - 1;
 
 constexpr int REGION = TILE + 2;              // 18x18 P2G partial region per particle tile
 constexpr int REGION_NODES = REGION * REGION;
-constexpr int P2G_FIELDS = 7;                 // m, P_x, P_y, P_z, H, F_x, F_y
+constexpr int P2G_FIELDS = 7;                 // m, P_x, P_y, H, F_x, F_y, P_z
+constexpr int P2G_PZ = P2G_FIELDS - 1;        // P_z last: only output steps store it
 
 // Global control of the engine (device); extends the per-step fields of Ctl.
 struct ECtl {

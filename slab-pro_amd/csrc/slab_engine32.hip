@@ -93,9 +93,17 @@ __device__ __forceinline__ T& el(T* base, size_t i) {
     }
 }
 
+// The terrain and slot sizes keys are formed from, read from the control
+// block once, before a kernel's first store (after one, the compiler can no
+// longer read the block through the scalar cache and waits on the stores).
+struct EGeo {
+    int nx, ny, nty, ntd, nslots;
+};
+__device__ __forceinline__ EGeo egeo(const ECtl* c) { return {c->nx, c->ny, c->nty, c->ntd, c->nslots}; }
+
 // True when a cell lies outside [1, N-3] on either axis (outflow).
-__device__ __forceinline__ bool cell_out(const ECtl* c, int ci, int cj) {
-    return ci < 1 || ci > c->nx - 3 || cj < 1 || cj > c->ny - 3;
+__device__ __forceinline__ bool cell_out(const EGeo& g, int ci, int cj) {
+    return ci < 1 || ci > g.nx - 3 || cj < 1 || cj > g.ny - 3;
 }
 
 // Sort key of a particle: its tile -- live (slot, tile), the parking tile of
@@ -104,19 +112,19 @@ __device__ __forceinline__ bool cell_out(const ECtl* c, int ci, int cj) {
 // in the tile (the stencil base, (ib & 15) << 4 | (jb & 15)).  Tile ids use the
 // whole int32 range, so the number of slots on a terrain is bounded by
 // slots x terrain tiles < 2^31.
-__device__ __forceinline__ int ekey(const ECtl* c, int st, int slot, bool out, int cell, float ox, float oy,
+__device__ __forceinline__ int ekey(const EGeo& g, int st, int slot, bool out, int cell, float ox, float oy,
                                     int* tile, int* cb) {
     *cb = 0;
     if (slot_active(st) && !out) {
         const int ci = cell >> 16, cj = cell & 0xffff;
-        const int ib = ebase(ci, ox, c->nx - 3), jb = ebase(cj, oy, c->ny - 3);
-        const int t = slot * c->ntd + (ib >> 4) * c->nty + (jb >> 4);
+        const int ib = ebase(ci, ox, g.nx - 3), jb = ebase(cj, oy, g.ny - 3);
+        const int t = slot * g.ntd + (ib >> 4) * g.nty + (jb >> 4);
         *tile = t;
         *cb = ((ib & 15) << 4) | (jb & 15);
         return t;
     }
     if (slot_active(st) || st == SLOT_DONE || st == SLOT_FAILED) {
-        const int t = c->nslots * c->ntd + slot;
+        const int t = g.nslots * g.ntd + slot;
         *tile = t;
         return t;
     }
@@ -343,7 +351,7 @@ __global__ void __launch_bounds__(WG) k_ekeys(ECtl* c, const Slot* __restrict__ 
     if (in) {
         const int raw = P.slot[s], sl = raw & SLOT_MASK;
         const float2 o = P.o[s];
-        k = ekey(c, slots[sl].status, sl, (raw & SLOT_OUT) != 0, P.cell[s], o.x, o.y, &t, &cb);
+        k = ekey(egeo(c), slots[sl].status, sl, (raw & SLOT_OUT) != 0, P.cell[s], o.x, o.y, &t, &cb);
     }
     wave_count_any(t, in && k >= 0, tile_count);
     if (in) {
@@ -1284,13 +1292,16 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
         dt0 = (float)slots[0].dt;
         fr0 = (float)slots[0].flip_ratio;
     }
+    const EGeo geo = egeo(c);
+    const bool compact = c->compact != 0;
+    const float inv = (float)c->inv_dx;
     // node and particle indices: 32-bit when every array's byte offsets fit
     using IX = typename std::conditional<O32, unsigned, size_t>::type;
     int t = 0, kk = -1, sl = 0, o = 0, cb = 0;
     float v2 = 0.f;
     bool pub = false;
     if (in) {
-        const int nx = c->nx, ny = c->ny;
+        const int nx = geo.nx, ny = geo.ny;
         const IX nn = (IX)nx * (IX)ny;
         const int s = reo ? el<O32>(idx, r) : r;
         o = r; // storage index written
@@ -1304,8 +1315,7 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
         float ox = po.x, oy = po.y, vxp = pv.x, vyp = pv.y, hp = el<O32>(P.h, s);
         float vzp, zp = 0.f; // recomputed by a running particle, carried otherwise
         bool zset = false;   // zp holds this step's elevation
-        if (st == SLOT_RUNNING && !c->compact && !out) {
-            const float inv = (float)c->inv_dx;
+        if (st == SLOT_RUNNING && !compact && !out) {
             const float dt = POOL ? (float)slots[sl].dt : dt0, fr = POOL ? (float)slots[sl].flip_ratio : fr0;
             const int ci = cell >> 16, cj = cell & 0xffff;
             const IX fc = (IX)ci * (IX)ny + (IX)cj;
@@ -1321,8 +1331,8 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
             // its base tile's entry)
             int gp[2][2] = {{0, 0}, {0, 0}};
             if (POOL) {
-                const int t0 = sl * c->ntd + (ib >> 4) * c->nty + (jb >> 4);
-                const int ox1 = ((ib + 2) >> 4) != (ib >> 4) ? c->nty : 0;
+                const int t0 = sl * geo.ntd + (ib >> 4) * geo.nty + (jb >> 4);
+                const int ox1 = ((ib + 2) >> 4) != (ib >> 4) ? geo.nty : 0;
                 const int oy1 = ((jb + 2) >> 4) != (jb >> 4) ? 1 : 0;
                 gp[0][0] = gtile_pos[t0];
                 gp[1][0] = gtile_pos[t0 + ox1];
@@ -1398,7 +1408,7 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
             hp = hp / (1.f + tr);
             zp = z;
             zset = zl;
-            if (cell_out(c, nci, ncj)) { // outflow: frozen from here on
+            if (cell_out(geo, nci, ncj)) { // outflow: frozen from here on
                 out = true;
                 vxp = vyp = vzp = 0.f;
             }
@@ -1419,7 +1429,7 @@ __global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ s
             el<O32>(Q.pid, o) = el<O32>(P.pid, s);
         }
         if (reo || out != was_out) el<O32>(Q.slot, o) = sl | (out ? SLOT_OUT : 0);
-        kk = ekey(c, st, sl, out, cell, ox, oy, &t, &cb);
+        kk = ekey(geo, st, sl, out, cell, ox, oy, &t, &cb);
         pub = slot_active(st) && !out;
         v2 = vxp * vxp + vyp * vyp;
     }

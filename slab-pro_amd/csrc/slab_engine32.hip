@@ -191,16 +191,23 @@ __global__ void k_eclear(ECtl* c, Slot* __restrict__ slots) {
 // updateDt (565-582) from the slot's own max |v|; priming slots get dt = 0
 // (simulation.hpp:39, simulation.cpp:493-495).
 // Each slot's dt shards -> shard 0, the others cleared: one wave per slot.
+static_assert(VMAX_SHARDS % 64 == 0, "shards per lane");
+// The maximum of a slot's shards, in every lane of the (whole) wave.
+__device__ __forceinline__ unsigned long long vmax_shards(const unsigned long long* __restrict__ b) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long q[VMAX_SHARDS / 64];
+#pragma unroll
+    for (int j = 0; j < VMAX_SHARDS / 64; ++j) q[j] = b[lane + 64 * j]; // all in flight together
+    unsigned long long v = 0;
+#pragma unroll
+    for (int j = 0; j < VMAX_SHARDS / 64; ++j) v = q[j] > v ? q[j] : v;
+    return wave_max_u64(v);
+}
 __device__ __forceinline__ void vmax_slot(unsigned long long* __restrict__ b) {
     const int lane = threadIdx.x & 63;
-    unsigned long long v = 0;
-    for (int k = lane; k < VMAX_SHARDS; k += 64) {
-        const unsigned long long q = b[k];
-        v = q > v ? q : v;
-        if (k) b[k] = 0ull;
-    }
-    v = wave_max_u64(v);
-    if (lane == 0) b[0] = v;
+    const unsigned long long v = vmax_shards(b);
+#pragma unroll
+    for (int j = 0; j < VMAX_SHARDS / 64; ++j) b[lane + 64 * j] = (lane == 0 && j == 0) ? v : 0ull;
 }
 
 // Batches: the shard reduction of every slot, one wave (block) per slot.
@@ -209,44 +216,133 @@ __global__ void __launch_bounds__(64) k_evmax(const ECtl* c, unsigned long long*
     vmax_slot(vmx + (size_t)blockIdx.x * VMAX_SHARDS);
 }
 
-// Run by one workgroup: the extra block of k_ereduce (the tile counting does
-// not read what this writes: its counts come from the previous G2P).
-__device__ void ebegin_block(ECtl* c, Slot* __restrict__ slots, unsigned long long* __restrict__ vmx,
-                             int reduced) {
-    if (c->compact) { // re-sort pass: no slot advances
-        if (threadIdx.x == 0) {
-            c->epoch += 1;
-            c->p2g_parity = c->parity;
-            c->reorder = 1;
-            c->pend_end = 1;
-        }
-        return;
-    }
-    __shared__ int s_act, s_out;
-    if (threadIdx.x == 0) s_act = s_out = 0;
-    // each slot's shards -> shard 0, unless k_evmax did it (batches)
-    if (!reduced)
-        for (int s = threadIdx.x >> 6; s < c->nslots; s += WG / 64) vmax_slot(vmx + (size_t)s * VMAX_SHARDS);
+// End of a started step and start of the next one, run by one workgroup: the
+// extra block of k_ereduce (the tile counting does not read what this writes:
+// its counts come from the previous G2P), or k_eend after the last step the
+// host enqueued (begin = false).
+//  * End (pend_end), src/simulation.cpp:503-509: time += dt and the frame
+//    trigger per running slot; priming slots start running.  A single run
+//    stops at a frame (the caller writes its outputs), so no step starts.
+//  * Start (begin): per slot, the while condition (src/simulation.cpp:500)
+//    and updateDt (565-582) from the slot's own max |v|; priming slots get
+//    dt = 0 (simulation.hpp:39, simulation.cpp:493-495).
+//  * A re-sort pass (compact) ends by flipping the buffers and starts with no
+//    slot advancing.
+// Every control and slot field is read before the block's first store, so
+// the reads are issued together (the control block through the scalar cache)
+// instead of one dependent reload after each store.
+struct ECtlIn {
+    int pend, compact, nslots, single, reorder, parity, n_sorted, n_active, sic, epoch, every;
+    double dx;
+};
+__device__ __forceinline__ ECtlIn ectl_in(const ECtl* c) {
+    return {c->pend_end, c->compact, c->nslots, c->single, c->reorder, c->parity,
+            c->n_sorted, c->n_active, c->steps_in_call, c->epoch, c->reorder_every, c->dx};
+}
+
+// Sum over the workgroup, in every thread (every thread calls it; a caller
+// that calls it again first passes a barrier, as its LDS is reused).
+__device__ __forceinline__ int block_sum_i32(int v) {
+    __shared__ int s_w[WG / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
     __syncthreads();
-    const int ep = c->epoch + 1;
-    int act = 0;
-    for (int s = threadIdx.x; s < c->nslots; s += WG) {
+    int t = 0;
+    for (int w = 0; w < WG / 64; ++w) t += s_w[w];
+    return t;
+}
+
+// A slot's fields the end and start of a step read.
+struct ESlotIn {
+    int status, frame, sic, step, keep;
+    double time, dt, output_dt, total, cfl, dt_max;
+};
+__device__ __forceinline__ ESlotIn eslot_in(const Slot& S) {
+    return {S.status, S.frame, S.steps_in_call, S.step, S.keep_grid,
+            S.time, S.dt, S.output_dt, S.total_time, S.C_cfl, S.dt_max};
+}
+
+// End of a started step and start of the next one, run by one workgroup: the
+// extra block of k_ereduce (the tile counting does not read what this writes:
+// its counts come from the previous G2P), or k_eend after the last step the
+// host enqueued (begin = false).
+//  * End (pend_end), src/simulation.cpp:503-509: time += dt and the frame
+//    trigger per running slot; priming slots start running.  A single run
+//    stops at a frame (the caller writes its outputs), so no step starts.
+//  * Start (begin): per slot, the while condition (src/simulation.cpp:500)
+//    and updateDt (565-582) from the slot's own max |v|; priming slots get
+//    dt = 0 (simulation.hpp:39, simulation.cpp:493-495).
+//  * A re-sort pass (compact) ends by flipping the buffers and starts with no
+//    slot advancing.
+// The control fields, the slot fields and the max |v| shards are all read
+// before the block's first store and before any branch on them, so they are
+// in flight together instead of a dependent reload after each store.
+// Slot ownership: with k_evmax (reduced), thread s owns slots s, s + WG, ...;
+// otherwise (at most 4 slots) lane 0 of wave s owns slot s, whose shards its
+// wave reduces.  nsl is the slot count (from the terrain geometry).
+__device__ void ectl_block(ECtl* c, int nsl, Slot* __restrict__ slots, unsigned long long* __restrict__ vmx,
+                           int reduced, bool begin) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int s_first = reduced ? (int)threadIdx.x : (lane == 0 && wv < nsl ? wv : nsl);
+    const int s_step = reduced ? WG : nsl;
+    // first owned slot's fields and shards, read unconditionally (clamped to
+    // a valid slot) so that they are in flight with the control fields
+    const int s_pre = min(s_first, nsl - 1);
+    ESlotIn q0 = eslot_in(slots[s_pre]);
+    const unsigned long long vr0 = (begin && reduced) ? vmx[(size_t)s_pre * VMAX_SHARDS] : 0ull;
+    unsigned long long vsh = 0; // this wave's slot max (used when not reduced)
+    if (begin) vsh = vmax_shards(vmx + (size_t)min(wv, nsl - 1) * VMAX_SHARDS);
+    const ECtlIn in = ectl_in(c);
+    const int pend = in.pend, compact = in.compact, single = in.single;
+    // the step that starts is a re-sort pass (a pending pass ends first)
+    const bool start_compact = !pend && compact;
+    const bool slot_steps = begin && !start_compact;
+    const bool slot_ends = pend && !compact;
+    const int ep = in.epoch + 1;
+    int hits = 0, act = 0, outs = 0;
+    for (int s = s_first; s < nsl; s += s_step) {
         Slot& S = slots[s];
-        const unsigned long long v = vmx[(size_t)s * VMAX_SHARDS];
-        vmx[(size_t)s * VMAX_SHARDS] = 0ull;
-        if (S.status == SLOT_PRIMING) {
+        ESlotIn q = q0;
+        unsigned long long vr = vr0;
+        if (s != s_first) {
+            q = eslot_in(S);
+            vr = (begin && reduced) ? vmx[(size_t)s * VMAX_SHARDS] : 0ull;
+        }
+        if (!slot_ends && !slot_steps) continue;
+        bool hit = false;
+        if (slot_ends) {
+            if (q.status == SLOT_PRIMING) {
+                q.status = SLOT_RUNNING;
+                S.status = q.status;
+            } else if (q.status == SLOT_RUNNING) {
+                q.time += q.dt;
+                S.time = q.time;
+                S.steps_in_call = q.sic + 1;
+                if (q.output_dt * (double)(q.frame + 1) - q.time < 1e-13) {
+                    q.frame += 1;
+                    S.frame = q.frame;
+                    S.frame_hit = 1;
+                    hit = true;
+                    ++hits;
+                }
+            }
+        }
+        // a single run holds one slot: its frame stops the run before a step starts
+        if (!slot_steps || (single && hit)) continue;
+        if (reduced) vmx[(size_t)s * VMAX_SHARDS] = 0ull;
+        if (q.status == SLOT_PRIMING) {
             S.dt = 0.0;
             S.out_step = 1;
             S.last_out_epoch = ep;
             ++act;
             continue;
         }
-        if (S.status != SLOT_RUNNING) continue;
-        if (!(S.time < S.total_time)) {
+        if (q.status != SLOT_RUNNING) continue;
+        if (!(q.time < q.total)) {
             S.status = SLOT_DONE;
             continue;
         }
-        const double v2 = __longlong_as_double(v);
+        const double v2 = __longlong_as_double(reduced ? vr : vsh);
         if (!(v2 <= 1.7976931348623157e308)) {
             S.status = SLOT_FAILED;
             S.err = 5; // SLAB_ENAN
@@ -255,88 +351,80 @@ __device__ void ebegin_block(ECtl* c, Slot* __restrict__ slots, unsigned long lo
         const double max_speed = sqrt(v2);
         double dt;
         if (max_speed > 0.0) {
-            const double a = S.C_cfl * c->dx / max_speed;
-            dt = (S.dt_max < a) ? S.dt_max : a;
+            const double a = q.cfl * in.dx / max_speed;
+            dt = (q.dt_max < a) ? q.dt_max : a;
         } else {
-            dt = S.dt_max;
+            dt = q.dt_max;
         }
-        const double next_frame_time = S.output_dt * (double)(S.frame + 1);
-        if (S.time + dt > next_frame_time - 1e-12) dt = next_frame_time - S.time;
+        const double next_frame_time = q.output_dt * (double)(q.frame + 1);
+        if (q.time + dt > next_frame_time - 1e-12) dt = next_frame_time - q.time;
         S.dt = dt;
-        S.step += 1;
-        const double tnew = S.time + dt;
-        const int hit = (S.output_dt * (double)(S.frame + 1) - tnew < 1e-13);
-        S.out_step = hit || S.keep_grid;
-        if (S.out_step) {
+        S.step = q.step + 1;
+        const double tnew = q.time + dt;
+        const int hit2 = (q.output_dt * (double)(q.frame + 1) - tnew < 1e-13);
+        const int out = hit2 || q.keep;
+        S.out_step = out;
+        if (out) {
             S.last_out_epoch = ep;
-            s_out = 1;
+            outs = 1;
         }
         ++act;
     }
-    if (act) atomicAdd(&s_act, act);
+    const int n_hit = block_sum_i32(hits);
+    __syncthreads(); // block_sum_i32's LDS is reused below
+    const int n_act = block_sum_i32(act);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        c->epoch = ep;
-        c->p2g_parity = c->parity;
-        c->n_active = s_act;
-        // a single run keeps the pre-G2P positions of output steps (its remesh
-        // window is read from them), so those steps write the other buffer
-        c->reorder = c->reorder_every <= 1 || ep % c->reorder_every == 0 || (c->single && s_out);
-        if (s_act == 0)
-            c->stop = STOP_DONE; // nothing to advance
-        else
-            c->pend_end = 1; // ended by the next step's first kernel or by k_eend
+    const int n_out = block_sum_i32(outs);
+    const bool stop_frame = slot_ends && single && n_hit > 0;
+    // the shards of the slots (at most 4) that start a step are cleared
+    if (slot_steps && !stop_frame && !reduced && wv < nsl) {
+        unsigned long long* sh = vmx + (size_t)wv * VMAX_SHARDS;
+#pragma unroll
+        for (int j = 0; j < VMAX_SHARDS / 64; ++j) sh[lane + 64 * j] = 0ull;
     }
-}
-
-// End of a step: time += dt and the frame trigger (src/simulation.cpp:503-509).
-// Run by one workgroup, once per started step (pend_end), after its G2P:
-// by the step-start block of the next step's k_ereduce, or by k_eend after
-// the last step the host enqueued.
-__device__ void eend_block(ECtl* c, Slot* __restrict__ slots) {
-    if (c->compact) {
-        if (threadIdx.x == 0) {
-            c->parity ^= 1;
-            c->n_store = c->n_sorted;
-            c->compact = 0;
-            c->reorder = 0;
+    if (threadIdx.x == 0) {
+        int par = in.parity;
+        if (pend) {
+            if (compact) {
+                par ^= 1;
+                c->n_store = in.n_sorted;
+                c->compact = 0;
+                c->reorder = 0;
+            } else {
+                if (in.reorder) {
+                    par ^= 1;
+                    c->n_store = in.n_sorted;
+                }
+                if (in.n_active > 0) c->steps_in_call = in.sic + 1;
+                if (stop_frame) c->stop = STOP_FRAME;
+            }
+            c->parity = par;
             c->pend_end = 0;
         }
-        return;
-    }
-    __shared__ int s_hit;
-    if (threadIdx.x == 0) s_hit = 0;
-    __syncthreads();
-    for (int s = threadIdx.x; s < c->nslots; s += WG) {
-        Slot& S = slots[s];
-        if (S.status == SLOT_PRIMING) {
-            S.status = SLOT_RUNNING;
-            continue;
+        if (begin && !stop_frame) {
+            c->epoch = ep;
+            c->p2g_parity = par;
+            if (start_compact) {
+                c->reorder = 1;
+                c->pend_end = 1;
+            } else {
+                c->n_active = n_act;
+                // a single run keeps the pre-G2P positions of output steps (its
+                // remesh window is read from them), so those steps write the
+                // other buffer
+                c->reorder = in.every <= 1 || ep % in.every == 0 || (single && n_out > 0);
+                if (n_act == 0)
+                    c->stop = STOP_DONE; // nothing to advance
+                else
+                    c->pend_end = 1; // ended by the next step's first kernel or by k_eend
+            }
         }
-        if (S.status != SLOT_RUNNING) continue;
-        S.time += S.dt;
-        S.steps_in_call += 1;
-        if (S.output_dt * (double)(S.frame + 1) - S.time < 1e-13) {
-            S.frame += 1;
-            S.frame_hit = 1;
-            atomicAdd(&s_hit, 1);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (c->reorder) {
-            c->parity ^= 1;
-            c->n_store = c->n_sorted;
-        }
-        if (c->n_active > 0) c->steps_in_call += 1;
-        if (c->single && s_hit) c->stop = STOP_FRAME;
-        c->pend_end = 0;
     }
 }
 
 __global__ void __launch_bounds__(WG) k_eend(ECtl* c, Slot* __restrict__ slots) {
     if (c->stop || !c->pend_end) return;
-    eend_block(c, slots);
+    ectl_block(c, c->nslots, slots, nullptr, 1, false);
 }
 
 // --- keys of admitted particles ---------------------------------------------
@@ -381,22 +469,39 @@ __global__ void k_ez(ECtl* c, PB32 A, PB32 B, int s0, int s1, const float* __res
 
 // --- tile scan over (slot, tile) + parking tiles ------------------------------
 
-__device__ __forceinline__ void etile_flags(const ECtl* c, const int* __restrict__ tile_count, int t, int ntiles,
-                                            int f[3]) {
-    f[0] = f[1] = f[2] = 0;
-    if (t >= ntiles) return;
-    const int n = tile_count[t];
-    f[0] = n;
-    f[1] = n > 0;
-    const int live = c->nslots * c->ntd;
-    if (t >= live) return; // parking tile: particles, never a grid tile
-    const int local = t % c->ntd, nty = c->nty;
-    const int tx = local / nty, ty = local - tx * nty;
-    int g = n > 0;
-    if (tx > 0) g |= tile_count[t - nty] > 0;
-    if (ty > 0) g |= tile_count[t - 1] > 0;
-    if (tx > 0 && ty > 0) g |= tile_count[t - nty - 1] > 0;
-    f[2] = g;
+// Flags of a thread's SCAN_ITEMS consecutive tiles t0 + k: particles, is a
+// particle tile, is a grid tile (it or its left / lower / diagonal neighbour
+// holds particles).  Every count is read first, at clamped in-range indices
+// (unused where the tile has no such neighbour), so the reads are in flight
+// together.
+__device__ __forceinline__ void etile_flags(const EGeo& geo, const int* __restrict__ tile_count, int t0, int ntiles,
+                                            int f[SCAN_ITEMS][3]) {
+    const int nty = geo.nty, live = geo.nslots * geo.ntd;
+    int n[SCAN_ITEMS], nl[SCAN_ITEMS], nb[SCAN_ITEMS], nd[SCAN_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const int t = min(t0 + k, ntiles - 1);
+        n[k] = tile_count[t];
+        nl[k] = tile_count[max(t - nty, 0)];
+        nb[k] = tile_count[max(t - 1, 0)];
+        nd[k] = tile_count[max(t - nty - 1, 0)];
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const int t = t0 + k;
+        f[k][0] = f[k][1] = f[k][2] = 0;
+        if (t >= ntiles) continue;
+        f[k][0] = n[k];
+        f[k][1] = n[k] > 0;
+        if (t >= live) continue; // parking tile: particles, never a grid tile
+        const int local = t % geo.ntd;
+        const int tx = local / nty, ty = local - tx * nty;
+        int g = n[k] > 0;
+        if (tx > 0) g |= nl[k] > 0;
+        if (ty > 0) g |= nb[k] > 0;
+        if (tx > 0 && ty > 0) g |= nd[k] > 0;
+        f[k][2] = g;
+    }
 }
 
 __device__ __forceinline__ void block_exscan3e(int v[3], int out[3], int tot[3]) {
@@ -427,15 +532,15 @@ __device__ __forceinline__ void block_exscan3e(int v[3], int out[3], int tot[3])
 // True when the block's tiles all belong to one FREE slot: a released slot's
 // tiles hold no particles (release re-keys and compacts), so the block has
 // nothing to count or list.
-__device__ __forceinline__ bool scan_block_idle(const ECtl* c, const Slot* __restrict__ slots, int ntiles) {
+__device__ __forceinline__ bool scan_block_idle(const EGeo& geo, const Slot* __restrict__ slots, int ntiles) {
     const int t0 = blockIdx.x * SCAN_BLOCK, t1 = min(t0 + SCAN_BLOCK, ntiles) - 1;
-    const int s0 = t0 / c->ntd, s1 = t1 / c->ntd;
-    return s0 == s1 && s0 < c->nslots && slots[s0].status == SLOT_FREE;
+    const int s0 = t0 / geo.ntd, s1 = t1 / geo.ntd;
+    return s0 == s1 && s0 < geo.nslots && slots[s0].status == SLOT_FREE;
 }
 
 // Per scan block: tile counts -> the block's numbers of particles, particle
 // tiles and grid tiles.  The last block ends the previous step if it is
-// pending (eend_block) and starts this one (ebegin_block); the counting reads
+// pending and starts this one (ectl_block); the counting reads
 // only the tile counts and which slots are FREE, neither of which the end or
 // start of a step changes.
 __global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, Slot* __restrict__ slots,
@@ -443,30 +548,24 @@ __global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, Slot* __restrict__ slot
                                                 int* __restrict__ bsum, unsigned long long* __restrict__ vmx,
                                                 int reduced) {
     if (c->stop) return;
+    // control fields read before any store on any path
+    const EGeo geo = egeo(c);
     if (blockIdx.x == gridDim.x - 1) {
-        __shared__ int s_stop;
-        if (c->pend_end) { // end of the previous step first (it may stop at a frame)
-            eend_block(c, slots);
-            __syncthreads();
-            if (threadIdx.x == 0) s_stop = c->stop;
-            __syncthreads();
-            if (s_stop) return;
-        }
-        ebegin_block(c, slots, vmx, reduced);
+        ectl_block(c, geo.nslots, slots, vmx, reduced, true);
         return;
     }
-    if (scan_block_idle(c, slots, ntiles)) {
+    if (scan_block_idle(geo, slots, ntiles)) {
         if (threadIdx.x < 3) bsum[3 * blockIdx.x + threadIdx.x] = 0;
         return;
     }
     int acc[3] = {0, 0, 0};
     const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    int f[SCAN_ITEMS][3];
+    etile_flags(geo, tile_count, t0, ntiles, f);
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        int f[3];
-        etile_flags(c, tile_count, t0 + k, ntiles, f);
-        acc[0] += f[0];
-        acc[1] += f[1];
-        acc[2] += f[2];
+        acc[0] += f[k][0];
+        acc[1] += f[k][1];
+        acc[2] += f[k][2];
     }
     int ex[3], tot[3];
     block_exscan3e(acc, ex, tot);
@@ -475,13 +574,14 @@ __global__ void __launch_bounds__(WG) k_ereduce(ECtl* c, Slot* __restrict__ slot
 }
 
 // The totals of k_escan (particles, particle tiles, grid tiles) into the
-// control block; an active-tile count above the capacity stops the step with
-// an error.  Thread 0 of one block.
-__device__ __forceinline__ void scan_totals(ECtl* c, const int tot[3]) {
+// control block; an active-tile count above the capacity (tile_cap, gtile_cap:
+// read by the caller before its first store) stops the step with an error.
+// Thread 0 of one block.
+__device__ __forceinline__ void scan_totals(ECtl* c, const int tot[3], int tile_cap, int gtile_cap) {
     c->n_sorted = tot[0];
     c->n_ptiles = tot[1];
     c->n_gtiles = tot[2];
-    if (tot[1] > c->tile_cap || tot[2] > c->gtile_cap) {
+    if (tot[1] > tile_cap || tot[2] > gtile_cap) {
         c->err = 7;
         c->stop = STOP_ERROR;
     }
@@ -490,6 +590,7 @@ __device__ __forceinline__ void scan_totals(ECtl* c, const int tot[3]) {
 __global__ void __launch_bounds__(WG) k_escan(ECtl* c, const int* __restrict__ bsum, int nblk,
                                               int* __restrict__ boff) {
     if (c->stop) return;
+    const int tcap = c->tile_cap, gcap = c->gtile_cap;
     int run[3] = {0, 0, 0};
     for (int b0 = 0; b0 < nblk; b0 += WG) {
         const int b = b0 + threadIdx.x;
@@ -502,7 +603,7 @@ __global__ void __launch_bounds__(WG) k_escan(ECtl* c, const int* __restrict__ b
             for (int k = 0; k < 3; ++k) boff[3 * b + k] = run[k] + ex[k];
         for (int k = 0; k < 3; ++k) run[k] += tot[k];
     }
-    if (threadIdx.x == 0) scan_totals(c, run);
+    if (threadIdx.x == 0) scan_totals(c, run, tcap, gcap);
 }
 
 // Per scan block: tile begin offsets and the particle / grid tile lists.  The
@@ -518,14 +619,17 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
                                                  int* __restrict__ hm_map, const int* __restrict__ hm_stack,
                                                  int2* __restrict__ pinfo) {
     if (c->stop) return;
-    const bool idle = scan_block_idle(c, slots, ntiles);
+    // control-block fields read before this kernel's first store
+    const EGeo geo = egeo(c);
+    const int ep = c->epoch, tcap = c->tile_cap, gcap = c->gtile_cap;
+    const bool idle = scan_block_idle(geo, slots, ntiles);
     if (idle && blockIdx.x != 0) return; // block 0 writes the totals even when idle
     int base[3];
     if (merged) {
         int tot[3];
         block_prefix3(bsum, nblk, base, tot);
-        if (blockIdx.x == 0 && threadIdx.x == 0) scan_totals(c, tot);
-        if (tot[1] > c->tile_cap || tot[2] > c->gtile_cap) return; // every block sees the same totals
+        if (blockIdx.x == 0 && threadIdx.x == 0) scan_totals(c, tot, tcap, gcap);
+        if (tot[1] > tcap || tot[2] > gcap) return; // every block sees the same totals
     } else {
         for (int q = 0; q < 3; ++q) base[q] = boff[3 * blockIdx.x + q];
     }
@@ -533,15 +637,13 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
     const int t0 = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
     int f[SCAN_ITEMS][3];
     int acc[3] = {0, 0, 0};
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        etile_flags(c, tile_count, t0 + k, ntiles, f[k]);
+    etile_flags(geo, tile_count, t0, ntiles, f);
+    for (int k = 0; k < SCAN_ITEMS; ++k)
         for (int q = 0; q < 3; ++q) acc[q] += f[k][q];
-    }
     int ex[3], tot[3];
     block_exscan3e(acc, ex, tot);
     int run[3];
     for (int q = 0; q < 3; ++q) run[q] = base[q] + ex[q];
-    const int ep = c->epoch;
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         const int t = t0 + k;
         if (t >= ntiles) break;

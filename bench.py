@@ -54,15 +54,16 @@ STEP_BYTES_FP64 = {"particle": 104, "node": 120}
 # written by G2P); "p2g" = per-tile LDS reduction of the particle state (reads
 # offsets vx vy h m: the cell follows from the tile and the offsets, vz is read
 # on output steps only), "grid" = partial merge + grid update (terrain 12 B,
-# max maps 16 B, vf 16 B).
+# max maps 16 B, vf 16 B), "g2p" per node = vf 16 B (the elevation z is
+# interpolated and written on output steps only: not counted).
 ALG_BYTES_FP32 = {
     "p2g": {"particle": 24, "node": 0},
     "grid": {"particle": 0, "node": 12 + 16 + 16},
-    "g2p": {"particle": 28 + 28, "node": 16 + 4},
+    "g2p": {"particle": 28 + 28, "node": 16},
     "tile_sort": {"particle": 8, "node": 0},
     "scatter": {"particle": 12, "node": 0},
 }
-STEP_BYTES_FP32 = {"particle": 60, "node": 60}
+STEP_BYTES_FP32 = {"particle": 60, "node": 56}
 
 # kernel names of the event timers -> rocprof kernel names, per form
 ROCPROF_NAMES = {

@@ -784,13 +784,13 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
     __shared__ int s_off[TILE_CELLS];
     __shared__ int s_w[4];
     __shared__ int s_c1, s_cmax;
-    // one LDS region, three uses: (a) creation index, arrival order and cell of
-    // a small tile; (b) 64-bit keys of a small tile; (c) keys + storage index of
-    // a round
-    __shared__ __align__(16) unsigned char s_mem[CAP * 8];
+    // one LDS region, three uses: (a) creation index by arrival, creation
+    // index in cell order and cell of a small tile; (b) 64-bit keys of a small
+    // tile; (c) keys + storage index of a round
+    __shared__ __align__(16) unsigned char s_mem[CAP * 9];
     int* s_pid = reinterpret_cast<int*>(s_mem);                                    // (a) CAP x i32
-    unsigned short* s_ord = reinterpret_cast<unsigned short*>(s_mem + CAP * 4);    // (a) CAP x u16
-    unsigned char* s_cell = s_mem + CAP * 6;                                       // (a) CAP x u8
+    int* s_pidc = reinterpret_cast<int*>(s_mem + CAP * 4);                         // (a) CAP x i32
+    unsigned char* s_cell = s_mem + CAP * 8;                                       // (a) CAP x u8
     unsigned long long* s_key = reinterpret_cast<unsigned long long*>(s_mem);      // (b) CAP, (c) ROUND
     int* s_sv = reinterpret_cast<int*>(s_mem + ROUND * 8);                         // (c) ROUND x i32
     const int32_t* __restrict__ pid = ecur(c, A, B).pid;
@@ -923,10 +923,11 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
         const bool dense = s_cmax > ES_RANK;
         __syncthreads();
         if (small && !dense) {
-            for (int r = threadIdx.x; r < n; r += WG) s_ord[atomicAdd(&s_hist[s_cell[r]], 1)] = (unsigned short)r;
+            for (int r = threadIdx.x; r < n; r += WG) s_pidc[atomicAdd(&s_hist[s_cell[r]], 1)] = s_pid[r];
             __syncthreads();
-            // s_off[c] .. s_hist[c] now lists cell c's arrivals; each arrival
-            // is placed from its registers at the cell start plus its rank
+            // s_pidc[s_off[c] .. s_hist[c]) now holds cell c's creation indices;
+            // each arrival is placed from its registers at the cell start plus
+            // its rank (creation indices are unique within the tile's slot)
 #pragma unroll
             for (int k = 0; k < CAP / WG; ++k) {
                 const int r = threadIdx.x + k * WG;
@@ -934,7 +935,7 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
                     const int cr = s_cell[r], pr = s_pid[r];
                     const int lo = s_off[cr], hi = s_hist[cr];
                     int rank = 0;
-                    for (int q = lo; q < hi; ++q) rank += s_pid[s_ord[q]] < pr ? 1 : 0;
+                    for (int q = lo; q < hi; ++q) rank += s_pidc[q] < pr ? 1 : 0;
                     idx[b + lo + rank] = svr[k];
                 }
             }

@@ -51,23 +51,23 @@ ALG_BYTES_FP64 = {
 }
 STEP_BYTES_FP64 = {"particle": 104, "node": 120}
 # FP32 form: particle = cell int32 + offsets + vx vy vz h m (32 B read, 28 B
-# written by G2P); "p2g" = per-tile LDS reduction of the particle state (reads
-# offsets vx vy h m: the cell follows from the tile and the offsets, vz is read
-# on output steps only), "grid" = partial merge + grid update (terrain 12 B,
-# max maps 16 B, vf 16 B), "g2p" per node = vf 16 B (the elevation z is
-# interpolated and written on output steps only: not counted).
+# written by G2P); "p2g" = the tile sort (bucket read + sorted index write,
+# 8 B) and the per-tile LDS reduction of the particle state (reads offsets vx
+# vy h m, 24 B: the cell follows from the tile and the offsets, vz is read on
+# output steps only) in one launch, "grid" = partial merge + grid update
+# (terrain 12 B, max maps 16 B, vf 16 B), "g2p" per node = vf 16 B (the
+# elevation z is interpolated and written on output steps only: not counted).
 ALG_BYTES_FP32 = {
-    "p2g": {"particle": 24, "node": 0},
+    "p2g": {"particle": 8 + 24, "node": 0},
     "grid": {"particle": 0, "node": 12 + 16 + 16},
     "g2p": {"particle": 28 + 28, "node": 16},
-    "tile_sort": {"particle": 8, "node": 0},
     "scatter": {"particle": 12, "node": 0},
 }
-STEP_BYTES_FP32 = {"particle": 60, "node": 56}
+STEP_BYTES_FP32 = {"particle": 60, "node": 60}
 
 # kernel names of the event timers -> rocprof kernel names, per form
 ROCPROF_NAMES = {
-    "fp32": {"g2p": "k_eg2p", "p2g": "k_ep2g", "grid": "k_egrid", "tile_sort": "k_esort", "scatter": "k_escatter"},
+    "fp32": {"g2p": "k_eg2p", "p2g": "k_esp", "grid": "k_egrid", "scatter": "k_escatter"},
     "fp64": {"g2p": "k_g2p", "p2g": "k_p2g64", "grid": "k_grid64", "tile_sort": "k_tile_sort",
              "scatter": "k_scatter"},
 }
@@ -444,6 +444,11 @@ def main():
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBs": step_alg / (ms_step * 1e-3) / 1e9,
                           "frac": step_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "n_active_nodes": n_act},
         "kernels_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in tim.items()},
+        # every kernel with an algorithmic byte figure: its fraction of 8 TB/s
+        # by the same event times (the roofline object is the dominant one)
+        "kernels_hbm_frac": {k: (alg_k[k]["particle"] * npart + alg_k[k]["node"] * n_act) /
+                                (v[0] / v[1] * 1e-3) / 1e9 / HBM_PEAK_GBS
+                             for k, v in tim.items() if k in alg_k and v[1] > 0 and v[0] > 0},
         "stream_ceiling": ceiling,
         "cpu_baseline": cpu,
         "ensemble": ens,

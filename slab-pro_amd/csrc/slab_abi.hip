@@ -39,7 +39,8 @@ slab_status fail(slab_status s, const std::string& msg) {
     } while (0)
 
 // Both forms: "p2g" is the per-tile register/LDS reduction into tile partials,
-// "grid" merges the tile partials and updates the grid.
+// "grid" merges the tile partials and updates the grid.  The FP32 form runs
+// its tile sort and P2G as one launch, timed as "p2g" ("tile_sort" has none).
 const char* const kKernelNames[] = {"ctl_begin", "tile_reduce", "tile_scan", "tile_offsets", "scatter",
                                     "tile_sort", "p2g",         "grid",      "g2p",          "ctl_end"};
 constexpr int kNumKernels = 10;

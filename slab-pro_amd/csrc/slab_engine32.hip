@@ -771,23 +771,20 @@ __device__ __forceinline__ int pow2_ceil(int v) {
 }
 
 template <int CAP>
-__global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uint8_t* __restrict__ kcell,
-                                              const int* __restrict__ ptiles, const int2* __restrict__ pinfo,
-                                              int* __restrict__ tile_count,
-                                              int* __restrict__ cursor, const int* __restrict__ tile_begin,
-                                              const int* __restrict__ tmp,
-                                              int* __restrict__ idx, int* __restrict__ cell_begin) {
+__device__ __forceinline__ void esort_tiles(ECtl* c, PB32 A, PB32 B, const uint8_t* __restrict__ kcell,
+                                            const int* __restrict__ ptiles, const int2* __restrict__ pinfo,
+                                            int* __restrict__ tile_count, int* __restrict__ cursor,
+                                            const int* __restrict__ tmp, int* __restrict__ idx,
+                                            int* __restrict__ cell_begin, unsigned char* __restrict__ s_mem) {
     constexpr int ROUND = CAP / 2; // keys and storage indices of a round fit the staging area
     static_assert(CAP % (WG * ES_GROUP) == 0 && CAP <= 8192, "sort capacity");
-    if (c->stop) return;
     __shared__ int s_hist[TILE_CELLS];
     __shared__ int s_off[TILE_CELLS];
     __shared__ int s_w[4];
     __shared__ int s_c1, s_cmax;
-    // one LDS region, three uses: (a) creation index by arrival, creation
-    // index in cell order and cell of a small tile; (b) 64-bit keys of a small
-    // tile; (c) keys + storage index of a round
-    __shared__ __align__(16) unsigned char s_mem[CAP * 9];
+    // one LDS region (s_mem, CAP * 9 bytes), three uses: (a) creation index by
+    // arrival, creation index in cell order and cell of a small tile; (b)
+    // 64-bit keys of a small tile; (c) keys + storage index of a round
     int* s_pid = reinterpret_cast<int*>(s_mem);                                    // (a) CAP x i32
     int* s_pidc = reinterpret_cast<int*>(s_mem + CAP * 4);                         // (a) CAP x i32
     unsigned char* s_cell = s_mem + CAP * 8;                                       // (a) CAP x u8
@@ -1059,15 +1056,16 @@ __global__ void __launch_bounds__(WG) k_esort(ECtl* c, PB32 A, PB32 B, const uin
 // (d,e) of every cell into node cell+(d,e) of an 18x18-node LDS region (one-to
 // -one within a phase: no conflicts, fixed order); the region is this tile's
 // partial, merged per node by k_egrid.
-__global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
-                                             const int* __restrict__ idx, const int* __restrict__ ptiles,
-                                             const int* __restrict__ cell_begin, const float* __restrict__ tgx,
-                                             const float* __restrict__ tgy, const float* __restrict__ tcth,
-                                             float* __restrict__ tpart) {
-    if (c->stop || c->compact) return;
-    constexpr int TCELLS = TILE + 1;
-    __shared__ float s_acc[P2G_FIELDS * REGION_NODES];
-    __shared__ float s_ter[3 * TCELLS * TCELLS];
+constexpr int P2G_TCELLS = TILE + 1;
+constexpr int P2G_LDS_FLOATS = P2G_FIELDS * REGION_NODES + 3 * P2G_TCELLS * P2G_TCELLS;
+// s_acc: P2G_FIELDS x REGION_NODES floats, s_ter: 3 x 17 x 17 floats
+__device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+                                           const int* __restrict__ idx, const int* __restrict__ ptiles,
+                                           const int* __restrict__ cell_begin, const float* __restrict__ tgx,
+                                           const float* __restrict__ tgy, const float* __restrict__ tcth,
+                                           float* __restrict__ tpart, float* __restrict__ s_acc,
+                                           float* __restrict__ s_ter) {
+    constexpr int TCELLS = P2G_TCELLS;
     const PB32 P = ecur(c, A, B);
     const int nt = c->n_ptiles, nx = c->nx, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
     const float inv = (float)c->inv_dx;
@@ -1219,6 +1217,33 @@ __global__ void __launch_bounds__(WG) k_ep2g(ECtl* c, const Slot* __restrict__ s
         for (int q = threadIdx.x; q < nreg; q += WG) dst[q] = s_acc[q];
         __syncthreads();
     }
+}
+
+// Sort and P2G in one launch: the workgroup sorts its tiles, then runs P2G
+// over the same tiles (the same list positions), reading the sorted index and
+// cell ranges it has just written -- no kernel boundary between the two, and
+// both phases share one LDS region.  The two phases stay separate loops, so
+// neither holds the other's registers.
+constexpr int esp_lds_bytes(int cap) {
+    return cap * 9 > P2G_LDS_FLOATS * 4 ? cap * 9 : P2G_LDS_FLOATS * 4;
+}
+template <int CAP>
+__global__ void __launch_bounds__(WG) k_esp(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+                                            const uint8_t* __restrict__ kcell, const int* __restrict__ ptiles,
+                                            const int2* __restrict__ pinfo, int* __restrict__ tile_count,
+                                            int* __restrict__ cursor, const int* __restrict__ tmp,
+                                            int* __restrict__ idx, int* __restrict__ cell_begin,
+                                            const float* __restrict__ tgx, const float* __restrict__ tgy,
+                                            const float* __restrict__ tcth, float* __restrict__ tpart) {
+    if (c->stop) return;
+    __shared__ __align__(16) unsigned char s_mem[esp_lds_bytes(CAP)];
+    esort_tiles<CAP>(c, A, B, kcell, ptiles, pinfo, tile_count, cursor, tmp, idx, cell_begin, s_mem);
+    if (c->compact) return;
+    // each sorted tile ended with a barrier: every write of the sort is
+    // visible to the whole workgroup and its LDS is free
+    float* s_p2g = reinterpret_cast<float*>(s_mem);
+    ep2g_tiles(c, slots, A, B, idx, ptiles, cell_begin, tgx, tgy, tcth, tpart, s_p2g,
+               s_p2g + P2G_FIELDS * REGION_NODES);
 }
 
 // --- grid ------------------------------------------------------------------
@@ -1907,20 +1932,20 @@ void enqueue_estep(Engine32* e, int step) {
         const unsigned nb = (unsigned)std::max<int64_t>(1, (e->launch_n + WG * SC_PER - 1) / (WG * SC_PER));
         k_escatter<<<nb, WG, 0, s>>>(c, e->key, e->tile_begin, e->cursor, e->tmp);
     });
-    tk(e, 5, step, [&] {
-        if (e->sort_cap == ES_CAPS[0])
-            k_esort<ES_CAPS[0]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count, e->cursor,
-                                                  e->tile_begin, e->tmp, e->idx, e->cell_begin);
-        else if (e->sort_cap == ES_CAPS[1])
-            k_esort<ES_CAPS[1]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count, e->cursor,
-                                                  e->tile_begin, e->tmp, e->idx, e->cell_begin);
-        else
-            k_esort<ES_CAPS[2]><<<pb, WG, 0, s>>>(c, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count, e->cursor,
-                                                  e->tile_begin, e->tmp, e->idx, e->cell_begin);
-    });
+    // the tile sort and P2G run as one launch (k_esp), timed as "p2g"
     tk(e, 6, step, [&] {
-        k_ep2g<<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->idx, e->ptiles, e->cell_begin, e->tgx, e->tgy, e->tcth,
-                                 e->tpart);
+        if (e->sort_cap == ES_CAPS[0])
+            k_esp<ES_CAPS[0]><<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count,
+                                                e->cursor, e->tmp, e->idx, e->cell_begin, e->tgx, e->tgy, e->tcth,
+                                                e->tpart);
+        else if (e->sort_cap == ES_CAPS[1])
+            k_esp<ES_CAPS[1]><<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count,
+                                                e->cursor, e->tmp, e->idx, e->cell_begin, e->tgx, e->tgy, e->tcth,
+                                                e->tpart);
+        else
+            k_esp<ES_CAPS[2]><<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count,
+                                                e->cursor, e->tmp, e->idx, e->cell_begin, e->tgx, e->tgy, e->tcth,
+                                                e->tpart);
     });
     tk(e, 7, step, [&] {
         if (e->pool_vf)

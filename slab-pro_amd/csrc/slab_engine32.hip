@@ -1385,8 +1385,11 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
 // the particle output of a frame (save_function.cpp:58): it is interpolated on
 // the slot's output steps only (frames, the prime, every step with keep_grid)
 // and carried unchanged otherwise.
+// The single-run form with 32-bit offsets fits 7 waves per SIMD (71 VGPRs, no
+// spills; 6 waves unhinted): G2P 140 -> 137 us at 8M particles.  The others
+// would spill at 7.
 template <bool POOL, bool O32>
-__global__ void __launch_bounds__(WG) k_eg2p(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(POOL || !O32 ? 1 : 7))) k_eg2p(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                              const int* __restrict__ idx, const float4* __restrict__ vf,
                                              const int* __restrict__ gtile_pos,
                                              const float* __restrict__ tz, const float* __restrict__ tgx,

@@ -770,18 +770,41 @@ __device__ __forceinline__ int pow2_ceil(int v) {
     return p;
 }
 
+// The thread index through an empty asm: values derived from it are computed
+// where they are used instead of being hoisted out of a loop and held in
+// registers across all of it.
+__device__ __forceinline__ int tid_opaque() {
+    int v = threadIdx.x;
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// LDS of the tile sort: histogram, cell offsets, scan and round scratch, and
+// the staging region (declared once by the kernel, shared by its passes).
+struct ESortLds {
+    int* hist;
+    int* off;
+    int* w4;
+    int* c1;
+    int* cmax;
+    unsigned char* mem;
+};
+
 template <int CAP>
 __device__ __forceinline__ void esort_tiles(ECtl* c, PB32 A, PB32 B, const uint8_t* __restrict__ kcell,
                                             const int* __restrict__ ptiles, const int2* __restrict__ pinfo,
                                             int* __restrict__ tile_count, int* __restrict__ cursor,
                                             const int* __restrict__ tmp, int* __restrict__ idx,
-                                            int* __restrict__ cell_begin, unsigned char* __restrict__ s_mem) {
+                                            int* __restrict__ cell_begin, const uint8_t* __restrict__ only,
+                                            const ESortLds& L) {
     constexpr int ROUND = CAP / 2; // keys and storage indices of a round fit the staging area
     static_assert(CAP % (WG * ES_GROUP) == 0 && CAP <= 8192, "sort capacity");
-    __shared__ int s_hist[TILE_CELLS];
-    __shared__ int s_off[TILE_CELLS];
-    __shared__ int s_w[4];
-    __shared__ int s_c1, s_cmax;
+    int* s_hist = L.hist;
+    int* s_off = L.off;
+    int* s_w = L.w4;
+    int& s_c1 = *L.c1;
+    int& s_cmax = *L.cmax;
+    unsigned char* s_mem = L.mem;
     // one LDS region (s_mem, CAP * 9 bytes), three uses: (a) creation index by
     // arrival, creation index in cell order and cell of a small tile; (b)
     // 64-bit keys of a small tile; (c) keys + storage index of a round
@@ -793,6 +816,7 @@ __device__ __forceinline__ void esort_tiles(ECtl* c, PB32 A, PB32 B, const uint8
     const int32_t* __restrict__ pid = ecur(c, A, B).pid;
     const int nt = c->n_ptiles, live = c->nslots * c->ntd;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        if (!only[w]) continue; // the tiles the small-tile pass left
         // the tile, its begin and its count, by list position: read-only here,
         // so read together through the scalar cache
         const int t = ptiles[w];
@@ -1063,24 +1087,19 @@ __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slo
                                            const int* __restrict__ idx, const int* __restrict__ ptiles,
                                            const int* __restrict__ cell_begin, const float* __restrict__ tgx,
                                            const float* __restrict__ tgy, const float* __restrict__ tcth,
-                                           float* __restrict__ tpart, float* __restrict__ s_acc,
-                                           float* __restrict__ s_ter) {
+                                           float* __restrict__ tpart, const uint8_t* __restrict__ only,
+                                           float* __restrict__ s_acc, float* __restrict__ s_ter) {
     constexpr int TCELLS = P2G_TCELLS;
     const PB32 P = ecur(c, A, B);
     const int nt = c->n_ptiles, nx = c->nx, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
     const float inv = (float)c->inv_dx;
     const int cid = threadIdx.x;
     const int lx = cid >> 4, ly = cid & 15;
-    // this thread's particle range of the workgroup's next tile is read one
-    // tile ahead
-    int nr0 = 0, nr1 = 0;
-    if ((int)blockIdx.x < nt) {
-        const int* cb = cell_begin + (size_t)blockIdx.x * CELL_STRIDE;
-        nr0 = cb[cid];
-        nr1 = cb[cid + 1];
-    }
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
-        const int r0 = nr0, r1 = nr1;
+        if (!only[w]) continue; // the tiles the small-tile pass left
+        // this thread's particle range in the sorted index
+        const int* cb = cell_begin + (size_t)w * CELL_STRIDE;
+        const int r0 = cb[cid], r1 = cb[cid + 1];
         // this thread's first particle's loads depend on its range only: they
         // are issued before the tile and slot checks and the LDS staging below
         // (every listed range lies inside the sorted index), and overlap them
@@ -1091,11 +1110,6 @@ __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slo
             const float2 po = P.o[sn0], pv = P.v[sn0];
             nox = po.x; noy = po.y; nmp = P.m[sn0]; nhp = P.h[sn0];
             nvx = pv.x; nvy = pv.y;
-        }
-        if (w + (int)gridDim.x < nt) {
-            const int* cb = cell_begin + (size_t)(w + gridDim.x) * CELL_STRIDE;
-            nr0 = cb[cid];
-            nr1 = cb[cid + 1];
         }
         const int t = ptiles[w];
         if (t >= live) continue;
@@ -1219,13 +1233,341 @@ __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slo
     }
 }
 
-// Sort and P2G in one launch: the workgroup sorts its tiles, then runs P2G
-// over the same tiles (the same list positions), reading the sorted index and
-// cell ranges it has just written -- no kernel boundary between the two, and
-// both phases share one LDS region.  The two phases stay separate loops, so
-// neither holds the other's registers.
+// Small tiles (at most CAP particles, no cell above ES_RANK): sorted and
+// summed in one pass per tile -- placed by rank into the tile's sorted order
+// in LDS, then P2G over that order (the sorted index is written only on
+// re-sort steps, whose G2P walks it).  Other tiles are marked in defer[w].
+template <int CAP>
+__device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
+                                                const uint8_t* __restrict__ kcell, const int* __restrict__ ptiles,
+                                                const int2* __restrict__ pinfo, int* __restrict__ tile_count,
+                                                int* __restrict__ cursor, const int* __restrict__ tmp,
+                                                int* __restrict__ idx, uint8_t* __restrict__ defer,
+                                                const float* __restrict__ tgx, const float* __restrict__ tgy,
+                                                const float* __restrict__ tcth, float* __restrict__ tpart,
+                                                const ESortLds& L) {
+    static_assert(CAP % (WG * ES_GROUP) == 0 && CAP <= 8192, "sort capacity");
+    constexpr int TCELLS = P2G_TCELLS;
+    int* s_hist = L.hist;
+    int* s_off = L.off;
+    int* s_w = L.w4;
+    int& s_cmax = *L.cmax;
+    unsigned char* s_mem = L.mem;
+    // the LDS region: (a) creation index by arrival, creation index in cell
+    // order and cell of the tile's particles; (d) P2G: the tile's sorted
+    // storage indices (over (a)'s first array), the 18x18-node partial region
+    // and the floor cells' terrain
+    int* s_pid = reinterpret_cast<int*>(s_mem);                                    // (a) CAP x i32
+    int* s_pidc = reinterpret_cast<int*>(s_mem + CAP * 4);                         // (a) CAP x i32
+    unsigned char* s_cell = s_mem + CAP * 8;                                       // (a) CAP x u8
+    int* s_sorted = reinterpret_cast<int*>(s_mem);                                 // (d) CAP x i32
+    float* s_acc = reinterpret_cast<float*>(s_mem + CAP * 4);                      // (d) 7 x 324
+    float* s_ter = s_acc + P2G_FIELDS * REGION_NODES;                              // (d) 3 x 17 x 17
+    const PB32 P = ecur(c, A, B);
+    const int32_t* __restrict__ pid = P.pid;
+    const int nt = c->n_ptiles, nx = c->nx, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
+    const bool reo = c->reorder != 0, compact = c->compact != 0;
+    const float inv = (float)c->inv_dx;
+    for (int w = blockIdx.x; w < nt; w += gridDim.x) {
+        // the thread index through an empty asm, once per tile: what derives
+        // from it is recomputed per tile instead of held in registers across
+        // the sort and P2G phases
+        const int tid = tid_opaque();
+        // the tile, its begin and its count, by list position: read-only here,
+        // so read together through the scalar cache
+        const int t = ptiles[w];
+        const int2 nb = pinfo[w];
+        const int n = nb.y, b = nb.x; // n >= 1: listed tiles hold particles
+        if (n > CAP && t < live) { // left to the sort in global memory
+            if (tid == 0) defer[w] = 1;
+            continue;
+        }
+        // the first group of bucket reads and their key and creation-index
+        // reads touch no LDS: issued before the barrier, they overlap it
+        int sv0[ES_GROUP], kc0[ES_GROUP], pd0[ES_GROUP];
+#pragma unroll
+        for (int j = 0; j < ES_GROUP; ++j) sv0[j] = tmp[b + min((int)tid + j * WG, n - 1)];
+#pragma unroll
+        for (int j = 0; j < ES_GROUP; ++j) {
+            kc0[j] = kcell[sv0[j]];
+            pd0[j] = pid[sv0[j]];
+        }
+        if (tid == 0) s_cmax = 0;
+        s_hist[tid] = 0;
+        __syncthreads();
+        if (t >= live) { // parking tile
+            for (int r = tid; r < n; r += WG) idx[b + r] = tmp[b + r];
+            if (tid == 0) {
+                defer[w] = 0;
+                tile_count[t] = 0;
+                cursor[t] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        // storage index of each arrival this thread stages: the final write
+        // places it from registers instead of re-reading tmp
+        int svr[CAP / WG];
+        {
+        // groups of ES_GROUP arrivals per thread: the bucket reads of a group
+        // are issued together, then their key and creation-index reads
+        // (clamped indices keep every read in the tile); the next group's
+        // bucket reads are issued before this group's keys are waited for
+        int svn[ES_GROUP];
+#pragma unroll
+        for (int j = 0; j < ES_GROUP; ++j)
+            svn[j] = ES_GROUP * WG < n ? tmp[b + min((int)tid + (ES_GROUP + j) * WG, n - 1)] : 0;
+#pragma unroll
+        for (int k0 = 0; k0 < CAP / WG; k0 += ES_GROUP) {
+            if (k0 * WG < n) {
+                int sv[ES_GROUP], kc[ES_GROUP], pd[ES_GROUP];
+                if (k0 == 0) {
+#pragma unroll
+                    for (int j = 0; j < ES_GROUP; ++j) {
+                        sv[j] = sv0[j];
+                        kc[j] = kc0[j];
+                        pd[j] = pd0[j];
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < ES_GROUP; ++j) sv[j] = svn[j];
+                    if (k0 + ES_GROUP < CAP / WG && (k0 + ES_GROUP) * WG < n) {
+#pragma unroll
+                        for (int j = 0; j < ES_GROUP; ++j)
+                            svn[j] = tmp[b + min((int)tid + (k0 + ES_GROUP + j) * WG, n - 1)];
+                    }
+#pragma unroll
+                    for (int j = 0; j < ES_GROUP; ++j) {
+                        kc[j] = kcell[sv[j]];
+                        pd[j] = pid[sv[j]];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < ES_GROUP; ++j) {
+                    const int r = tid + (k0 + j) * WG;
+                    svr[k0 + j] = sv[j];
+                    if (r < n) {
+                        const int cc = kc[j];
+                        s_cell[r] = (unsigned char)cc;
+                        s_pid[r] = pd[j];
+                        atomicAdd(&s_hist[cc], 1);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < ES_GROUP; ++j) svr[k0 + j] = 0;
+            }
+        }
+        }
+        __syncthreads();
+        const int cnt = s_hist[tid];
+        if (cnt > ES_RANK) atomicMax(&s_cmax, cnt);
+        {
+            const int lane = tid & 63, wv = tid >> 6;
+            int inc = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                int u = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += u;
+            }
+            if (lane == 63) s_w[wv] = inc;
+            __syncthreads();
+            int pre = 0;
+            for (int q = 0; q < wv; ++q) pre += s_w[q];
+            s_off[tid] = pre + inc - cnt;
+            if (tid == 0) {
+                tile_count[t] = 0;
+                cursor[t] = 0;
+            }
+        }
+        __syncthreads();
+        const int off = s_off[tid];
+        s_hist[tid] = off; // cursor
+        const bool dense = s_cmax > ES_RANK;
+        __syncthreads();
+        if (dense) { // a cell above ES_RANK particles: left to the bitonic sort
+            if (tid == 0) defer[w] = 1;
+            continue;
+        }
+        if (tid == 0) defer[w] = 0;
+        // each arrival's creation index, also kept in registers: the sorted
+        // order is written over s_pid below
+        int pdr[CAP / WG];
+#pragma unroll
+        for (int k = 0; k < CAP / WG; ++k) {
+            const int r = tid + k * WG;
+            pdr[k] = 0;
+            if (r < n) {
+                pdr[k] = s_pid[r];
+                s_pidc[atomicAdd(&s_hist[s_cell[r]], 1)] = pdr[k];
+            }
+        }
+        __syncthreads();
+        // s_pidc[s_off[c] .. s_hist[c]) now holds cell c's creation indices;
+        // each arrival is placed from its registers at the cell start plus
+        // its rank (creation indices are unique within the tile's slot): into
+        // the tile's sorted order in LDS for P2G, and into the sorted index on
+        // a re-sort step (whose G2P walks it)
+#pragma unroll
+        for (int k = 0; k < CAP / WG; ++k) {
+            const int r = tid + k * WG;
+            if (r < n) {
+                const int cr = s_cell[r], pr = pdr[k];
+                const int lo = s_off[cr], hi = s_hist[cr];
+                int rank = 0;
+                for (int q = lo; q < hi; ++q) rank += s_pidc[q] < pr ? 1 : 0;
+                s_sorted[lo + rank] = svr[k];
+                if (reo) idx[b + lo + rank] = svr[k];
+            }
+        }
+        __syncthreads();
+        // ---- P2G of the tile, src/simulation.cpp:726-798: one thread per
+        // stencil-base cell sums its particles (canonical order) into 9 stencil
+        // slots x 7 fields in registers; 9 phases then add slot (d,e) of every
+        // cell into node cell+(d,e) of an 18x18-node LDS region (one-to-one
+        // within a phase: no conflicts, fixed order); the region is this
+        // tile's partial, merged per node by k_egrid.  A re-sort pass has no
+        // P2G; neither has a slot that is not running.
+        if (compact) continue;
+        const int slot = t / ntd, local = t - slot * ntd;
+        if (!slot_active(slots[slot].status)) continue;
+        const float g = (float)slots[slot].gravity;
+        // P_z only feeds the grid's vz output, written on output steps
+        const bool need_vz = slots[slot].out_step != 0;
+        // this thread's cell's particles: positions [r0, r1) of the tile's
+        // sorted order in LDS
+        const int lx = tid >> 4, ly = tid & 15;
+        const int r0 = off, r1 = off + cnt;
+        // the first particle's loads, then the terrain's, before any LDS write
+        float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
+        if (r0 < r1) {
+            const int sn0 = s_sorted[r0];
+            const float2 po = P.o[sn0], pv = P.v[sn0];
+            nox = po.x; noy = po.y; nmp = P.m[sn0]; nhp = P.h[sn0];
+            nvx = pv.x; nvy = pv.y;
+            if (need_vz) nvz = P.vz[sn0];
+        }
+        const int tx = local / nty, ty = local - tx * nty;
+        // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged
+        // once, every load of a thread issued before its first LDS write
+        constexpr int TPT = (TCELLS * TCELLS + WG - 1) / WG;
+        float tv[TPT][3];
+#pragma unroll
+        for (int i = 0; i < TPT; ++i) {
+            const int q = tid + i * WG;
+            const int qa = q / TCELLS, qb = q - qa * TCELLS;
+            const int gi = tx * TILE + qa, gj = ty * TILE + qb;
+            tv[i][0] = tv[i][1] = tv[i][2] = 0.f;
+            if (q < TCELLS * TCELLS && gi < nx && gj < ny) {
+                const size_t k = (size_t)gi * ny + gj;
+                tv[i][0] = tgx[k];
+                tv[i][1] = tgy[k];
+                tv[i][2] = tcth[k];
+            }
+        }
+        // the region's fields; P_z (the last) on output steps only
+        const int nreg = (need_vz ? P2G_FIELDS : P2G_PZ) * REGION_NODES;
+        for (int q = tid; q < nreg; q += WG) s_acc[q] = 0.f;
+#pragma unroll
+        for (int i = 0; i < TPT; ++i) {
+            const int q = tid + i * WG;
+            if (q < TCELLS * TCELLS) {
+                s_ter[q] = tv[i][0];
+                s_ter[TCELLS * TCELLS + q] = tv[i][1];
+                s_ter[2 * TCELLS * TCELLS + q] = tv[i][2];
+            }
+        }
+        __syncthreads();
+        float acc[9][P2G_FIELDS];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
+        // the next particle's loads are issued before the current one is
+        // summed, and the index of the one after it a particle further ahead,
+        // so no iteration waits for an index load; the vz momentum (grid vz
+        // output only) is summed on output steps only
+        auto sum_particles = [&](auto vz_tag) {
+        constexpr bool VZ = decltype(vz_tag)::value;
+        int sn_next = r0 + 1 < r1 ? s_sorted[r0 + 1] : 0;
+        for (int r = r0; r < r1; ++r) {
+            const float ox = nox, oy = noy, mp = nmp, hp = nhp, pvx = nvx, pvy = nvy, pvz = nvz;
+            if (r + 1 < r1) {
+                const int sn = sn_next;
+                const float2 po = P.o[sn], pv = P.v[sn];
+                nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
+                nvx = pv.x; nvy = pv.y;
+                if (VZ) nvz = P.vz[sn];
+                if (r + 2 < r1) sn_next = s_sorted[r + 2];
+            }
+            // live particles sit in cells [1, N-3] (outflow parks the others),
+            // so the stencil base is never clamped and the floor cell is the
+            // base plus (offset < 1/2)
+            const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
+            const int fl = (lx + ax) * TCELLS + (ly + ay);
+            const float fx = (float)ax + ox, fy = (float)ay + oy;
+            const float cth = s_ter[2 * TCELLS * TCELLS + fl];
+            const float mgc = mp * g * cth * cth; // m g cos^2 theta
+            const float vmx = pvx * mp, vmy = pvy * mp, hm = hp * mp, vzm = pvz * mp;
+            // body force m b = -m g cos^2 theta grad z
+            const float mbx = -mgc * s_ter[fl], mby = -mgc * s_ter[TCELLS * TCELLS + fl];
+            // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
+            const float ps = -0.5f * mgc * hp;
+            float wx[3], wy[3], dwx[3], dwy[3];
+            weights3f(fx, inv, wx, dwx);
+            weights3f(fy, inv, wy, dwy);
+            // stress term ps * grad w = (ps * dw_x[d]) * w_y[e], (ps * dw_y[e]) * w_x[d]
+            float psx[3], psy[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                psx[d] = ps * dwx[d];
+                psy[d] = ps * dwy[d];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    const float wgt = wx[d] * wy[e];
+                    if (wgt > 3e-3f) { // simulation.cpp:780
+                        float* a = acc[d * 3 + e];
+                        a[0] = fmaf(mp, wgt, a[0]);
+                        a[1] = fmaf(vmx, wgt, a[1]);
+                        a[2] = fmaf(vmy, wgt, a[2]);
+                        a[3] = fmaf(hm, wgt, a[3]);
+                        a[4] = fmaf(-psx[d], wy[e], fmaf(mbx, wgt, a[4]));
+                        a[5] = fmaf(-psy[e], wx[d], fmaf(mby, wgt, a[5]));
+                        if (VZ) a[P2G_PZ] = fmaf(vzm, wgt, a[P2G_PZ]);
+                    }
+                }
+            }
+        }
+        };
+        if (need_vz)
+            sum_particles(std::true_type{});
+        else
+            sum_particles(std::false_type{});
+        __syncthreads();
+#pragma unroll
+        for (int ph = 0; ph < 9; ++ph) {
+            const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
+#pragma unroll
+            for (int f = 0; f < P2G_FIELDS; ++f)
+                if (f < P2G_PZ || need_vz) s_acc[f * REGION_NODES + node] += acc[ph][f];
+            __syncthreads();
+        }
+        float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
+        for (int q = tid; q < nreg; q += WG) dst[q] = s_acc[q];
+        __syncthreads();
+    }
+}
+
+// Sort and P2G in one launch, in three passes over the workgroup's tiles (list
+// positions blockIdx.x + k * gridDim.x): the small tiles, sorted and summed
+// from LDS; then the tiles they left, sorted in LDS rounds / global memory
+// into the sorted index; then P2G of those from the index.  No kernel
+// boundary between the sort and P2G, one LDS region for all passes, and each
+// pass a loop of its own, so none holds another's registers.
 constexpr int esp_lds_bytes(int cap) {
-    return cap * 9 > P2G_LDS_FLOATS * 4 ? cap * 9 : P2G_LDS_FLOATS * 4;
+    return cap * 9 > cap * 4 + P2G_LDS_FLOATS * 4 ? cap * 9 : cap * 4 + P2G_LDS_FLOATS * 4;
 }
 template <int CAP>
 __global__ void __launch_bounds__(WG) k_esp(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
@@ -1233,16 +1575,25 @@ __global__ void __launch_bounds__(WG) k_esp(ECtl* c, const Slot* __restrict__ sl
                                             const int2* __restrict__ pinfo, int* __restrict__ tile_count,
                                             int* __restrict__ cursor, const int* __restrict__ tmp,
                                             int* __restrict__ idx, int* __restrict__ cell_begin,
-                                            const float* __restrict__ tgx, const float* __restrict__ tgy,
-                                            const float* __restrict__ tcth, float* __restrict__ tpart) {
+                                            uint8_t* __restrict__ defer, const float* __restrict__ tgx,
+                                            const float* __restrict__ tgy, const float* __restrict__ tcth,
+                                            float* __restrict__ tpart) {
     if (c->stop) return;
+    __shared__ int s_hist[TILE_CELLS];
+    __shared__ int s_off[TILE_CELLS];
+    __shared__ int s_w[4];
+    __shared__ int s_c1, s_cmax;
     __shared__ __align__(16) unsigned char s_mem[esp_lds_bytes(CAP)];
-    esort_tiles<CAP>(c, A, B, kcell, ptiles, pinfo, tile_count, cursor, tmp, idx, cell_begin, s_mem);
+    const ESortLds L{s_hist, s_off, s_w, &s_c1, &s_cmax, s_mem};
+    esp_small_tiles<CAP>(c, slots, A, B, kcell, ptiles, pinfo, tile_count, cursor, tmp, idx, defer, tgx, tgy, tcth,
+                         tpart, L);
+    // the pass's marks and sorted tiles are visible to the whole workgroup
+    __syncthreads();
+    esort_tiles<CAP>(c, A, B, kcell, ptiles, pinfo, tile_count, cursor, tmp, idx, cell_begin, defer, L);
     if (c->compact) return;
-    // each sorted tile ended with a barrier: every write of the sort is
-    // visible to the whole workgroup and its LDS is free
+    __syncthreads();
     float* s_p2g = reinterpret_cast<float*>(s_mem);
-    ep2g_tiles(c, slots, A, B, idx, ptiles, cell_begin, tgx, tgy, tcth, tpart, s_p2g,
+    ep2g_tiles(c, slots, A, B, idx, ptiles, cell_begin, tgx, tgy, tcth, tpart, defer, s_p2g,
                s_p2g + P2G_FIELDS * REGION_NODES);
 }
 
@@ -1812,6 +2163,7 @@ struct Engine32 {
     int *ptiles = nullptr, *gtiles = nullptr, *bsum = nullptr, *boff = nullptr;
     int2* pinfo = nullptr; // per particle-tile list position: {begin, count}
     int* cell_begin = nullptr;
+    uint8_t* tdefer = nullptr; // per tile list position: left by the small-tile pass of k_esp
     float* tpart = nullptr;
     PB32 A{}, B{};
     int *key = nullptr, *idx = nullptr, *tmp = nullptr;
@@ -1939,16 +2291,16 @@ void enqueue_estep(Engine32* e, int step) {
     tk(e, 6, step, [&] {
         if (e->sort_cap == ES_CAPS[0])
             k_esp<ES_CAPS[0]><<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count,
-                                                e->cursor, e->tmp, e->idx, e->cell_begin, e->tgx, e->tgy, e->tcth,
-                                                e->tpart);
+                                                e->cursor, e->tmp, e->idx, e->cell_begin, e->tdefer, e->tgx, e->tgy,
+                                                e->tcth, e->tpart);
         else if (e->sort_cap == ES_CAPS[1])
             k_esp<ES_CAPS[1]><<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count,
-                                                e->cursor, e->tmp, e->idx, e->cell_begin, e->tgx, e->tgy, e->tcth,
-                                                e->tpart);
+                                                e->cursor, e->tmp, e->idx, e->cell_begin, e->tdefer, e->tgx, e->tgy,
+                                                e->tcth, e->tpart);
         else
             k_esp<ES_CAPS[2]><<<pb, WG, 0, s>>>(c, e->d_slots, e->A, e->B, e->kcell, e->ptiles, e->pinfo, e->tile_count,
-                                                e->cursor, e->tmp, e->idx, e->cell_begin, e->tgx, e->tgy, e->tcth,
-                                                e->tpart);
+                                                e->cursor, e->tmp, e->idx, e->cell_begin, e->tdefer, e->tgx, e->tgy,
+                                                e->tcth, e->tpart);
     });
     tk(e, 7, step, [&] {
         if (e->pool_vf)
@@ -2201,6 +2553,7 @@ Engine32* engine_create(int nx, int ny, double dx, const double* z, int nslots, 
               ealloc(e->allocs, &e->gtiles, e->ntiles, &why) && ealloc(e->allocs, &e->bsum, 3 * (size_t)e->nblk, &why) &&
               ealloc(e->allocs, &e->boff, 3 * (size_t)e->nblk, &why) &&
               ealloc(e->allocs, &e->cell_begin, (size_t)e->tile_cap * CELL_STRIDE, &why) &&
+              ealloc(e->allocs, &e->tdefer, (size_t)e->tile_cap, &why) &&
               ealloc(e->allocs, &e->tpart, (size_t)e->tile_cap * P2G_FIELDS * REGION_NODES, &why) &&
               ealloc(e->allocs, &e->dpart, 6 * 256, &why);
     if (ok && single)

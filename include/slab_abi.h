@@ -108,10 +108,12 @@ slab_status slab_run_step(slab_run* run, int32_t max_steps, slab_step_info* info
 /* Particles in creation order, as saveParticleData reads them
  * (src/simulation.cpp via src/save_function.cpp:36-76).  Any pointer may be NULL.
  * z, the bed elevation G2P interpolates at each particle (simulation.cpp:930),
- * is read by the reference only when a frame is saved: the FP32 form
- * interpolates it on output steps only (frames, the prime, every step after
- * slab_run_keep_grid(1)) and carries it unchanged between them, so z is the
- * last output step's value (the initial elevation before the first). */
+ * and vz = v . grad z (:933) are read by the reference only when a frame is
+ * saved (and vz by the P2G of such a step): the FP32 form refreshes z on
+ * output steps only (frames, the prime, every step after
+ * slab_run_keep_grid(1)) and vz on those and on a step after which the next
+ * can reach its frame, and carries them unchanged otherwise, so between
+ * frames they hold the last such step's values. */
 slab_status slab_run_download_particles(slab_run* run, double* x, double* y, double* vx, double* vy,
                                         double* vz, double* z, double* h, double* A, double* m);
 
@@ -215,8 +217,8 @@ slab_status slab_batch_slot_info(slab_batch* b, int32_t slot, slab_slot_info* ou
 /* status[max_slots] and time[max_slots] of every slot (either may be NULL). */
 slab_status slab_batch_statuses(slab_batch* b, int32_t* status, double* time);
 slab_status slab_batch_download_max(slab_batch* b, int32_t slot, double* h_max, double* v_max2);
-/* As slab_run_download_particles for one slot (z: the slot's last output
- * step's elevation). */
+/* As slab_run_download_particles for one slot (z as refreshed on the slot's
+ * output steps; vz every step). */
 slab_status slab_batch_download_particles(slab_batch* b, int32_t slot, double* x, double* y, double* vx, double* vy,
                                           double* vz, double* z, double* h, double* A, double* m);
 /* No counterpart in the reference.  For callers that cannot allocate device

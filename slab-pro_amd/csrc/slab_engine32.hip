@@ -333,6 +333,7 @@ __device__ void ectl_block(ECtl* c, int nsl, Slot* __restrict__ slots, unsigned 
         if (q.status == SLOT_PRIMING) {
             S.dt = 0.0;
             S.out_step = 1;
+            S.vz_due = 1;
             S.last_out_epoch = ep;
             ++act;
             continue;
@@ -364,6 +365,12 @@ __device__ void ectl_block(ECtl* c, int nsl, Slot* __restrict__ slots, unsigned 
         const int hit2 = (q.output_dt * (double)(q.frame + 1) - tnew < 1e-13);
         const int out = hit2 || q.keep;
         S.out_step = out;
+        // particle vz feeds only output steps (P2G's grid vz) and the particle
+        // output of a frame: G2P refreshes it on output steps and on steps
+        // after which the next step can reach its frame (its dt is at most
+        // dt_max, and is clipped to land on the frame within 1e-12)
+        const double next_frame = q.output_dt * (double)(q.frame + hit2 + 1);
+        S.vz_due = out || (tnew + q.dt_max >= next_frame - 1e-12);
         if (out) {
             S.last_out_epoch = ep;
             outs = 1;
@@ -1766,11 +1773,12 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(POOL ||
     // A single run (dense layout) has one slot: its status, dt and FLIP ratio
     // are read once per workgroup, so no particle waits for a per-particle
     // slot lookup before its stencil reads.
-    int st0 = 0, zo0 = 0;
+    int st0 = 0, zo0 = 0, vd0 = 0;
     float dt0 = 0.f, fr0 = 0.f;
     if (!POOL) {
         st0 = slots[0].status;
         zo0 = slots[0].out_step;
+        vd0 = slots[0].vz_due;
         dt0 = (float)slots[0].dt;
         fr0 = (float)slots[0].flip_ratio;
     }
@@ -1795,13 +1803,21 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(POOL ||
         int cell = el<O32>(P.cell, s);
         const float2 po = el<O32>(P.o, s), pv = el<O32>(P.v, s);
         float ox = po.x, oy = po.y, vxp = pv.x, vyp = pv.y, hp = el<O32>(P.h, s);
-        float vzp, zp = 0.f; // recomputed by a running particle, carried otherwise
-        bool zset = false;   // zp holds this step's elevation
+        float vzp = 0.f, zp = 0.f; // recomputed by a running particle, carried otherwise
+        bool zset = false;         // zp holds this step's elevation
+        bool vzset = false;        // vzp holds this step's vz
         if (st == SLOT_RUNNING && !compact && !out) {
             const float dt = POOL ? (float)slots[sl].dt : dt0, fr = POOL ? (float)slots[sl].flip_ratio : fr0;
             const int ci = cell >> 16, cj = cell & 0xffff;
-            const IX fc = (IX)ci * (IX)ny + (IX)cj;
-            const float Gx = el<O32>(tgx, fc), Gy = el<O32>(tgy, fc);
+            // vz = v . grad z at the particle's cell, on the steps that need it
+            // (a batch refreshes it every step)
+            const bool vzl = POOL || vd0 != 0;
+            float Gx = 0.f, Gy = 0.f;
+            if (vzl) {
+                const IX fc = (IX)ci * (IX)ny + (IX)cj;
+                Gx = el<O32>(tgx, fc);
+                Gy = el<O32>(tgy, fc);
+            }
             const int ib = ebase(ci, ox, nx - 3), jb = ebase(cj, oy, ny - 3);
             const float fx = (float)(ci - ib) + ox, fy = (float)(cj - jb) + oy;
             float wx[3], wy[3], dwx[3], dwy[3];
@@ -1886,24 +1902,28 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(POOL ||
             oy = noy;
             vxp = nvx;
             vyp = nvy;
-            vzp = nvx * Gx + nvy * Gy;
+            if (vzl) vzp = nvx * Gx + nvy * Gy;
+            vzset = vzl;
             hp = hp / (1.f + tr);
             zp = z;
             zset = zl;
             if (cell_out(geo, nci, ncj)) { // outflow: frozen from here on
                 out = true;
                 vxp = vyp = vzp = 0.f;
+                vzset = true;
             }
-        } else {
-            vzp = el<O32>(P.vz, s);
+        } else if (POOL) {
+            vzp = el<O32>(P.vz, s); // a batch rewrites vz every step
+            vzset = true;
         }
-        // the elevation is carried (re-sort steps move it) unless this step
-        // set it
+        // vz and the elevation are carried (re-sort steps move them) unless
+        // this step set them
+        if (!vzset && reo) vzp = el<O32>(P.vz, s);
         if (!zset && reo) zp = el<O32>(P.z, s);
         el<O32>(Q.cell, o) = cell;
         el<O32>(Q.o, o) = make_float2(ox, oy);
         el<O32>(Q.v, o) = make_float2(vxp, vyp);
-        el<O32>(Q.vz, o) = vzp;
+        if (vzset || reo) el<O32>(Q.vz, o) = vzp;
         el<O32>(Q.h, o) = hp;
         if (zset || reo) el<O32>(Q.z, o) = zp;
         if (reo) {

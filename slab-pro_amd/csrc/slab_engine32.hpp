@@ -26,7 +26,8 @@ struct Slot {
     double mu, xi, mu1, mu2, I0, d, phi;
     int32_t fric, status, frame, step;
     int32_t out_step, frame_hit, keep_grid, np;
-    int32_t last_out_epoch, err, steps_in_call, pad;
+    int32_t last_out_epoch, err, steps_in_call;
+    int32_t vz_due; // this step's G2P refreshes particle vz (an output step may follow)
 };
 
 // FP32 particle state (structure of arrays).  Position = floor cell packed as

@@ -121,11 +121,14 @@ def test_fp32_storage_resort_period_does_not_change_results(period):
 
 
 def test_fp32_elevation_at_output_frames():
-    """The particle elevation (simulation.cpp:930), read by nothing but a frame's
-    particle output (save_function.cpp:58), is interpolated on output steps
-    only: at every frame a run without keep_grid holds bitwise the particles of
-    one whose every step is an output step, and its z tracks the FP64 form's
-    (pinned to the oracle) to 1e-5 of the largest |z| (measured ~1e-7)."""
+    """The particle elevation z (simulation.cpp:930) and vz (:933), read only by
+    a frame's particle output (save_function.cpp:58) and, for vz, by the P2G
+    of output steps, are refreshed on output steps (vz also on a step after
+    which the next can reach its frame) and carried otherwise: at every frame
+    a run without keep_grid holds bitwise the particles of one whose every
+    step is an output step, and its z tracks the FP64 form's (pinned to the
+    oracle) to 1e-5 of the largest |z| (measured ~1e-7), its vz to 1e-3 of the
+    largest |vz|."""
     from slabpro import configs
     cfg = configs.c0()
     runs = [_mk(cfg, 32, keep=False), _mk(cfg, 32, keep=True), _mk(cfg, 64, keep=False)]
@@ -144,8 +147,10 @@ def test_fp32_elevation_at_output_frames():
         for k in pa:
             assert np.array_equal(pa[k], pk[k]), (frame, k)
         e = _rel(pa["z"], p64["z"])
-        print("frame", frame + 1, "z vs fp64", e, flush=True)
+        ev = _rel(pa["vz"], p64["vz"])
+        print("frame", frame + 1, "z vs fp64", e, "vz vs fp64", ev, flush=True)
         assert e <= 1e-5
+        assert ev <= 1e-3
     for r in runs:
         r.close()
 

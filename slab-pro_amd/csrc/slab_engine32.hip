@@ -1795,12 +1795,16 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(POOL ||
         const IX nn = (IX)nx * (IX)ny;
         const int s = reo ? el<O32>(idx, r) : r;
         o = r; // storage index written
-        const int raw = el<O32>(P.slot, s);
+        // a batch reads the particle's slot; a single run has one slot, and a
+        // particle is parked exactly when its cell lies outside [1, N-3]
+        // (admission parks those, and G2P parks a particle when it moves
+        // there and then never moves it), so its stored flag is not read
+        const int raw = POOL ? el<O32>(P.slot, s) : 0;
         sl = POOL ? raw & SLOT_MASK : 0;
-        bool out = (raw & SLOT_OUT) != 0;
-        const bool was_out = out;
-        const int st = POOL ? slots[sl].status : st0;
         int cell = el<O32>(P.cell, s);
+        bool out = POOL ? (raw & SLOT_OUT) != 0 : cell_out(geo, cell >> 16, cell & 0xffff);
+        bool parked = false; // parked by this step's move
+        const int st = POOL ? slots[sl].status : st0;
         const float2 po = el<O32>(P.o, s), pv = el<O32>(P.v, s);
         float ox = po.x, oy = po.y, vxp = pv.x, vyp = pv.y, hp = el<O32>(P.h, s);
         float vzp = 0.f, zp = 0.f; // recomputed by a running particle, carried otherwise
@@ -1872,7 +1876,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(POOL ||
             // loads do not add to their registers)
             const bool zl = (POOL ? slots[sl].out_step : zo0) != 0;
             if (zl) {
-#pragma unroll
+                // one row at a time: output steps are rare, and the rows' loads
+                // in flight together would add registers to every step
+#pragma unroll 1
                 for (int di = 0; di < 3; ++di) {
                     const float* tzr = &el<O32>(tz, (IX)(ib + di) * (IX)ny + (IX)jb);
                     float rt = 0.f;
@@ -1909,6 +1915,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(POOL ||
             zset = zl;
             if (cell_out(geo, nci, ncj)) { // outflow: frozen from here on
                 out = true;
+                parked = true;
                 vxp = vyp = vzp = 0.f;
                 vzset = true;
             }
@@ -1930,7 +1937,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(POOL ||
             el<O32>(Q.m, o) = el<O32>(P.m, s);
             el<O32>(Q.pid, o) = el<O32>(P.pid, s);
         }
-        if (reo || out != was_out) el<O32>(Q.slot, o) = sl | (out ? SLOT_OUT : 0);
+        if (reo || parked) el<O32>(Q.slot, o) = sl | (out ? SLOT_OUT : 0);
         kk = ekey(geo, st, sl, out, cell, ox, oy, &t, &cb);
         pub = slot_active(st) && !out;
         v2 = vxp * vxp + vyp * vyp;

@@ -277,7 +277,7 @@ slab_status slab_batch_set_resort(slab_batch* b, int32_t steps);
  * for the FP64 form, which re-sorts its storage every step. */
 slab_status slab_run_set_resort(slab_run* run, int32_t steps);
 /* Autotuner of the FP32 form: times candidate launch shapes -- storage
- * re-sort period {1, 4, 16, 64}, then workgroups of the tile-list kernels
+ * re-sort period {1, 4, 16, 64}, then workgroups of the tile sort + P2G kernel
  * {1024, 2048, 4096, 8192}, then the in-LDS capacity of the tile sort {1024,
  * 2048, 4096} -- on the next steps of the live run (each trial runs
  * max(trial_steps, 2 x period) ordinary steps; results do not depend on the
@@ -296,8 +296,9 @@ slab_status slab_run_autotune(slab_run* run, int32_t trial_steps, int32_t out[2]
 slab_status slab_batch_set_sort_capacity(slab_batch* b, int32_t particles);
 slab_status slab_run_set_sort_capacity(slab_run* run, int32_t particles);
 /* The current FP32 launch shape: out[0] = storage re-sort period, out[1] =
- * workgroups of the tile-list kernels, out[2] = sort capacity (the FP64 form
- * reports {1, 0, 0}).  No reference counterpart. */
+ * workgroups of the tile sort + P2G kernel, out[2] = sort capacity (the FP64
+ * form reports {1, 0, 0}; the grid kernel's workgroups follow the last step's
+ * grid-tile count).  No reference counterpart. */
 slab_status slab_batch_launch_shape(slab_batch* b, int32_t out[3]);
 slab_status slab_run_launch_shape(slab_run* run, int32_t out[3]);
 slab_status slab_batch_timing(slab_batch* b, int32_t enable, double* ms, int64_t* counts, int32_t n);

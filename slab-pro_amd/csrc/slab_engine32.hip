@@ -2200,6 +2200,7 @@ struct Engine32 {
     // capacity launches few idle blocks (a captured step is rebuilt when it
     // no longer fits)
     int64_t launch_n = 0;
+    int grid_wg = 8192; // workgroups of the grid kernel (sized from the grid-tile count)
     double* dpart = nullptr;
     std::vector<void*> allocs, pallocs;
     std::vector<int64_t> admit_base; // storage offset of each slot's particles at admission
@@ -2214,7 +2215,7 @@ struct Engine32 {
     double ms[kNumK] = {};
     int64_t cnt[kNumK] = {};
     bool use_graph = false;
-    int persist = kPersistE; // workgroups of the tile-list kernels (sort + P2G, grid)
+    int persist = kPersistE; // workgroups of the tile sort + P2G kernel
     int sort_cap = ES_CAPS[2]; // in-LDS capacity of the tile sort (particles)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -2296,7 +2297,7 @@ unsigned xcd_grid(int64_t n) { return (unsigned)((std::max<int64_t>(1, n) + N_XC
 void enqueue_estep(Engine32* e, int step) {
     hipStream_t s = e->stream;
     ECtl* c = e->d_c;
-    const unsigned tb = (unsigned)std::max(1, std::min(e->ntiles, e->persist));
+    const unsigned tb = (unsigned)std::max(1, std::min(e->ntiles, e->grid_wg));
     const unsigned pb = (unsigned)std::max(1, std::min(e->tile_cap, e->persist));
     const int many = e->nslots > 4;
     if (many) tk(e, 0, step, [&] { k_evmax<<<e->nslots, 64, 0, s>>>(c, e->vmx); });
@@ -2364,9 +2365,25 @@ void drop_graph(Engine32* e) {
     e->graph = nullptr;
 }
 
+// The grid kernel's workgroups: about one per two grid tiles of the last step,
+// a power of two in [8192, 32768] (measured on MI355X: the 8M-particle run's
+// ~8.5 k grid tiles 45.0 / 41.8 / 42.1 / 46.6 us with 2,048 / 8,192 / 16,384 /
+// 32,768 workgroups; the 256-scenario batch's ~70 k tiles 223 / 199 / 194 /
+// 191 us).  It grows when the tiles outgrow it and shrinks below a quarter.
+void size_grid_workgroups(Engine32* e) {
+    const int64_t half = std::max<int64_t>(1, e->h_c->n_gtiles / 2);
+    if (half <= e->grid_wg && half * 4 >= e->grid_wg) return;
+    int want = 8192;
+    while (want < half && want < 32768) want <<= 1;
+    if (want == e->grid_wg) return;
+    e->grid_wg = want;
+    drop_graph(e);
+}
+
 // Re-sizes the particle-range grids to the stored count when it outgrew them
 // or fell below a quarter (the captured step is dropped and rebuilt).
 void size_particle_grids(Engine32* e) {
+    size_grid_workgroups(e);
     const int64_t need = std::max<int64_t>(1, e->h_c->n_store);
     if (need <= e->launch_n && need * 4 >= e->launch_n) return;
     int64_t want = 65536;

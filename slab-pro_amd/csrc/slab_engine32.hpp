@@ -121,7 +121,7 @@ int engine_use_graph(Engine32* e, int on);             // capture the step seque
 // in place.  Results do not depend on k (the computation order is canonical).
 bool engine_set_reorder(Engine32* e, int k, std::string* err);
 // Times candidate launch shapes (storage re-sort period, workgroups of the
-// tile-list kernels, in-LDS capacity of the tile sort) on the next steps of the
+// sort + P2G kernel, in-LDS capacity of the tile sort) on the next steps of the
 // live run and keeps the fastest; out = {period, workgroups, sort capacity}.
 // The steps it runs are ordinary steps.
 bool engine_autotune(Engine32* e, int trial_steps, int out[3], std::string* err);

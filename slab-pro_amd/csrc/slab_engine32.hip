@@ -690,14 +690,29 @@ __global__ void __launch_bounds__(WG) k_eoffsets(ECtl* c, const Slot* __restrict
 // for any of them.  Arrival order inside a tile is arbitrary: the sort makes
 // it canonical.
 constexpr int SC_PER = 16;
+constexpr int SC_HASH = 512; // per-workgroup table of the tiles its particles go to
 __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict__ key,
                                                  const int* __restrict__ tile_begin, int* __restrict__ cursor,
                                                  int* __restrict__ tmp) {
     if (c->stop) return;
+    // The workgroup's 4,096 consecutive particles take their slots per tile in
+    // two levels: each run of equal tiles in a 64-particle group adds its
+    // length to its tile's entry of an LDS table (offset within the
+    // workgroup), then each table entry takes the workgroup's range of its
+    // tile with one cursor atomic.  Storage is canonical at its last re-sort,
+    // so a workgroup's particles go to few tiles: a few global atomics per
+    // workgroup instead of one per run.  A run whose tile finds no table entry
+    // (more than SC_HASH tiles in the workgroup) takes its range directly.
+    __shared__ int s_ht[SC_HASH]; // tile id, or -1
+    __shared__ int s_hc[SC_HASH]; // particles, then the workgroup's base in the tile
+    for (int i = threadIdx.x; i < SC_HASH; i += WG) {
+        s_ht[i] = -1;
+        s_hc[i] = 0;
+    }
     const int n = c->n_store, lane = threadIdx.x & 63;
     const int s0 = (blockIdx.x * WG + (threadIdx.x & ~63)) * SC_PER + lane;
     const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    int t[SC_PER], start[SC_PER], base[SC_PER];
+    int t[SC_PER], start[SC_PER], hv[SC_PER];
     bool valid[SC_PER];
 #pragma unroll
     for (int j = 0; j < SC_PER; ++j) {
@@ -707,12 +722,13 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
         t[j] = valid[j] ? k : 0;
     }
     // the tiles' begins and the previous lane's tile of every group first
-    // (all in flight together), then the run heads and cursor atomics
+    // (all in flight together), then the run heads
     int tb[SC_PER], tprev[SC_PER];
 #pragma unroll
     for (int j = 0; j < SC_PER; ++j) tb[j] = valid[j] ? tile_begin[t[j]] : 0;
 #pragma unroll
     for (int j = 0; j < SC_PER; ++j) tprev[j] = __shfl_up(valid[j] ? t[j] : -1, 1, 64);
+    __syncthreads(); // the table is cleared
 #pragma unroll
     for (int j = 0; j < SC_PER; ++j) {
         const bool head = valid[j] && (lane == 0 || tprev[j] != t[j]);
@@ -722,11 +738,31 @@ __global__ void __launch_bounds__(WG) k_escatter(ECtl* c, const int* __restrict_
         const unsigned long long brk = (hmask | ~vmask) & ~upto;
         start[j] = hle ? 63 - __clzll((long long)hle) : 0;
         const int end = brk ? (__ffsll((long long)brk) - 1) : 64;
-        base[j] = head ? atomicAdd(&cursor[t[j]], end - lane) : 0;
+        hv[j] = 0;
+        if (head) {
+            // hv >= 0: table entry << 16 | offset in the workgroup's range;
+            // hv < 0: -1 - the run's base in the tile (taken directly)
+            unsigned h = ((unsigned)t[j] * 2654435761u) >> 23; // top 9 bits
+            int e = -1;
+            for (int probe = 0; probe < SC_HASH; ++probe) {
+                const int old = atomicCAS(&s_ht[h], -1, t[j]);
+                if (old == -1 || old == t[j]) {
+                    e = (int)h;
+                    break;
+                }
+                h = (h + 1) & (SC_HASH - 1);
+            }
+            hv[j] = e >= 0 ? (e << 16) | atomicAdd(&s_hc[e], end - lane) : -1 - atomicAdd(&cursor[t[j]], end - lane);
+        }
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < SC_HASH; i += WG)
+        if (s_ht[i] >= 0) s_hc[i] = atomicAdd(&cursor[s_ht[i]], s_hc[i]);
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < SC_PER; ++j) {
-        const int b = __shfl(base[j], start[j], 64);
+        const int hj = __shfl(hv[j], start[j], 64);
+        const int b = hj >= 0 ? s_hc[hj >> 16] + (hj & 0xffff) : -1 - hj;
         if (valid[j]) tmp[tb[j] + b + (lane - start[j])] = s0 + 64 * j;
     }
 }

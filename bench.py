@@ -50,20 +50,21 @@ ALG_BYTES_FP64 = {
     "scatter": {"particle": 12, "node": 0},
 }
 STEP_BYTES_FP64 = {"particle": 104, "node": 120}
-# FP32 form: particle = cell int32 + offsets + vx vy vz h m (32 B read, 28 B
-# written by G2P); "p2g" = the tile sort (bucket read + sorted index write,
-# 8 B) and the per-tile LDS reduction of the particle state (reads offsets vx
-# vy h m, 24 B: the cell follows from the tile and the offsets, vz is read on
-# output steps only) in one launch, "grid" = partial merge + grid update
-# (terrain 12 B, max maps 16 B, vf 16 B), "g2p" per node = vf 16 B (the
-# elevation z is interpolated and written on output steps only: not counted).
+# FP32 form: particle = cell int32 + offsets + vx vy vz h m + elevation z;
+# "p2g" = the tile sort (bucket read + sorted index write, 8 B) and the
+# per-tile LDS reduction of the particle state (reads offsets vx vy h m, 24 B:
+# the cell follows from the tile and the offsets, vz is read on output steps
+# only) in one launch, "grid" = partial merge + grid update (terrain 12 B, max
+# maps 16 B, vf 16 B), "g2p" = reads cell offsets vx vy h and writes them back
+# (24 + 24 B; a single run's vz and z only around output steps: not counted),
+# per node vf 16 B.
 ALG_BYTES_FP32 = {
     "p2g": {"particle": 8 + 24, "node": 0},
     "grid": {"particle": 0, "node": 12 + 16 + 16},
-    "g2p": {"particle": 28 + 28, "node": 16},
+    "g2p": {"particle": 24 + 24, "node": 16},
     "scatter": {"particle": 12, "node": 0},
 }
-STEP_BYTES_FP32 = {"particle": 60, "node": 60}
+STEP_BYTES_FP32 = {"particle": 52, "node": 60}
 
 # kernel names of the event timers -> rocprof kernel names, per form
 ROCPROF_NAMES = {

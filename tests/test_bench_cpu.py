@@ -27,8 +27,9 @@ def test_traffic_lookup_reads_the_committed_summary():
 
 
 def test_step_bytes_are_the_sum_of_the_persistent_fields():
-    # FP32: read cell ox oy vx vy vz h m (32 B) + write all but m (28 B) per particle;
-    # per node: terrain 12 B, max maps 16 B read+write, vf 16 B written, vf + z re-read by G2P 16 B
-    assert bench.STEP_BYTES_FP32 == {"particle": 60, "node": 60}
+    # FP32: read cell ox oy vx vy h m (28 B) + write cell ox oy vx vy h (24 B) per particle
+    # (vz and z only around output steps); per node: terrain 12 B, max maps 16 B read+write,
+    # vf 16 B written, vf re-read by G2P 16 B
+    assert bench.STEP_BYTES_FP32 == {"particle": 52, "node": 60}
     assert bench.ALG_BYTES_FP32["g2p"]["particle"] + bench.ALG_BYTES_FP32["p2g"]["particle"] <= 104
     assert bench.STEP_BYTES_FP64 == {"particle": 104, "node": 120}

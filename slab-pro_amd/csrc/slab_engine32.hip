@@ -1276,6 +1276,17 @@ __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slo
     }
 }
 
+// The sort + P2G kernel's LDS region (beside the histogram arrays) for an
+// in-LDS sort capacity of cap particles: the sort's three arrays (9 B per
+// particle), or the P2G's sorted storage indices, partial region and terrain.
+constexpr int esp_lds_bytes(int cap) {
+    return cap * 9 > cap * 4 + P2G_LDS_FLOATS * 4 ? cap * 9 : cap * 4 + P2G_LDS_FLOATS * 4;
+}
+constexpr int P2G_TER_BYTES = 3 * P2G_TCELLS * P2G_TCELLS * 4;
+// particle fields a small tile's P2G stages in LDS: offset, velocity, mass,
+// height (24 B), and vz on output steps (28 B)
+constexpr int ESP_STAGE_B = 24, ESP_STAGE_B_VZ = 28;
+
 // Small tiles (at most CAP particles, no cell above ES_RANK): sorted and
 // summed in one pass per tile -- placed by rank into the tile's sorted order
 // in LDS, then P2G over that order (the sorted index is written only on
@@ -1299,13 +1310,20 @@ __device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict_
     // the LDS region: (a) creation index by arrival, creation index in cell
     // order and cell of the tile's particles; (d) P2G: the tile's sorted
     // storage indices (over (a)'s first array), the 18x18-node partial region
-    // and the floor cells' terrain
+    // and, at the region's end, the floor cells' terrain.  A staged tile's
+    // P2G keeps its particle fields (SoA, sorted order) below the terrain,
+    // over the sorted indices, and its partial region over the fields once
+    // they are summed.
+    constexpr int LDS_BYTES = esp_lds_bytes(CAP);
+    constexpr int STAGE_BYTES = LDS_BYTES - P2G_TER_BYTES;
+    constexpr int ESK = (STAGE_BYTES / ESP_STAGE_B + WG - 1) / WG; // staged particles per thread
+    static_assert(CAP * 4 + P2G_FIELDS * REGION_NODES * 4 <= STAGE_BYTES, "P2G LDS layout");
     int* s_pid = reinterpret_cast<int*>(s_mem);                                    // (a) CAP x i32
     int* s_pidc = reinterpret_cast<int*>(s_mem + CAP * 4);                         // (a) CAP x i32
     unsigned char* s_cell = s_mem + CAP * 8;                                       // (a) CAP x u8
     int* s_sorted = reinterpret_cast<int*>(s_mem);                                 // (d) CAP x i32
     float* s_acc = reinterpret_cast<float*>(s_mem + CAP * 4);                      // (d) 7 x 324
-    float* s_ter = s_acc + P2G_FIELDS * REGION_NODES;                              // (d) 3 x 17 x 17
+    float* s_ter = reinterpret_cast<float*>(s_mem + STAGE_BYTES);                  // (d) 3 x 17 x 17
     const PB32 P = ecur(c, A, B);
     const int32_t* __restrict__ pid = P.pid;
     const int nt = c->n_ptiles, nx = c->nx, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
@@ -1477,129 +1495,195 @@ __device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict_
         // P_z only feeds the grid's vz output, written on output steps
         const bool need_vz = slots[slot].out_step != 0;
         // this thread's cell's particles: positions [r0, r1) of the tile's
-        // sorted order in LDS
+        // sorted order
         const int lx = tid >> 4, ly = tid & 15;
         const int r0 = off, r1 = off + cnt;
-        // the first particle's loads, then the terrain's, before any LDS write
-        float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
-        if (r0 < r1) {
-            const int sn0 = s_sorted[r0];
-            const float2 po = P.o[sn0], pv = P.v[sn0];
-            nox = po.x; noy = po.y; nmp = P.m[sn0]; nhp = P.h[sn0];
-            nvx = pv.x; nvy = pv.y;
-            if (need_vz) nvz = P.vz[sn0];
-        }
         const int tx = local / nty, ty = local - tx * nty;
-        // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged
-        // once, every load of a thread issued before its first LDS write
-        constexpr int TPT = (TCELLS * TCELLS + WG - 1) / WG;
-        float tv[TPT][3];
+        // a tile whose particle fields fit the LDS below the terrain has them
+        // staged there in sorted order: consecutive positions on consecutive
+        // lanes, all of a thread's reads issued together, so the summing loop
+        // waits on no global read.  A larger tile's threads read their cell's
+        // particles from global memory as they go, one particle ahead.  The
+        // two forms are separate code paths, so neither holds the other's
+        // registers.
+        auto p2g_tile = [&](auto staged_tag) {
+            constexpr bool STAGED = decltype(staged_tag)::value;
+            float2 so[STAGED ? ESK : 1], sw[STAGED ? ESK : 1];
+            float sm[STAGED ? ESK : 1], sh[STAGED ? ESK : 1], sz[STAGED ? ESK : 1];
+            float nox = 0.f, noy = 0.f, nmp = 0.f, nhp = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
+            if constexpr (STAGED) {
 #pragma unroll
-        for (int i = 0; i < TPT; ++i) {
-            const int q = tid + i * WG;
-            const int qa = q / TCELLS, qb = q - qa * TCELLS;
-            const int gi = tx * TILE + qa, gj = ty * TILE + qb;
-            tv[i][0] = tv[i][1] = tv[i][2] = 0.f;
-            if (q < TCELLS * TCELLS && gi < nx && gj < ny) {
-                const size_t k = (size_t)gi * ny + gj;
-                tv[i][0] = tgx[k];
-                tv[i][1] = tgy[k];
-                tv[i][2] = tcth[k];
-            }
-        }
-        // the region's fields; P_z (the last) on output steps only
-        const int nreg = (need_vz ? P2G_FIELDS : P2G_PZ) * REGION_NODES;
-        for (int q = tid; q < nreg; q += WG) s_acc[q] = 0.f;
-#pragma unroll
-        for (int i = 0; i < TPT; ++i) {
-            const int q = tid + i * WG;
-            if (q < TCELLS * TCELLS) {
-                s_ter[q] = tv[i][0];
-                s_ter[TCELLS * TCELLS + q] = tv[i][1];
-                s_ter[2 * TCELLS * TCELLS + q] = tv[i][2];
-            }
-        }
-        __syncthreads();
-        float acc[9][P2G_FIELDS];
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-#pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
-        // the next particle's loads are issued before the current one is
-        // summed, and the index of the one after it a particle further ahead,
-        // so no iteration waits for an index load; the vz momentum (grid vz
-        // output only) is summed on output steps only
-        auto sum_particles = [&](auto vz_tag) {
-        constexpr bool VZ = decltype(vz_tag)::value;
-        int sn_next = r0 + 1 < r1 ? s_sorted[r0 + 1] : 0;
-        for (int r = r0; r < r1; ++r) {
-            const float ox = nox, oy = noy, mp = nmp, hp = nhp, pvx = nvx, pvy = nvy, pvz = nvz;
-            if (r + 1 < r1) {
-                const int sn = sn_next;
-                const float2 po = P.o[sn], pv = P.v[sn];
-                nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
+                for (int k = 0; k < ESK; ++k) {
+                    const int q = tid + k * WG;
+                    so[k] = sw[k] = make_float2(0.f, 0.f);
+                    sm[k] = sh[k] = sz[k] = 0.f;
+                    if (q < n) {
+                        const int sn = s_sorted[q];
+                        so[k] = P.o[sn];
+                        sw[k] = P.v[sn];
+                        sm[k] = P.m[sn];
+                        sh[k] = P.h[sn];
+                        if (need_vz) sz[k] = P.vz[sn];
+                    }
+                }
+            } else if (r0 < r1) { // the first particle's loads
+                const int sn0 = s_sorted[r0];
+                const float2 po = P.o[sn0], pv = P.v[sn0];
+                nox = po.x; noy = po.y; nmp = P.m[sn0]; nhp = P.h[sn0];
                 nvx = pv.x; nvy = pv.y;
-                if (VZ) nvz = P.vz[sn];
-                if (r + 2 < r1) sn_next = s_sorted[r + 2];
+                if (need_vz) nvz = P.vz[sn0];
             }
-            // live particles sit in cells [1, N-3] (outflow parks the others),
-            // so the stencil base is never clamped and the floor cell is the
-            // base plus (offset < 1/2)
-            const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
-            const int fl = (lx + ax) * TCELLS + (ly + ay);
-            const float fx = (float)ax + ox, fy = (float)ay + oy;
-            const float cth = s_ter[2 * TCELLS * TCELLS + fl];
-            const float mgc = mp * g * cth * cth; // m g cos^2 theta
-            const float vmx = pvx * mp, vmy = pvy * mp, hm = hp * mp, vzm = pvz * mp;
-            // body force m b = -m g cos^2 theta grad z
-            const float mbx = -mgc * s_ter[fl], mby = -mgc * s_ter[TCELLS * TCELLS + fl];
-            // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
-            const float ps = -0.5f * mgc * hp;
-            float wx[3], wy[3], dwx[3], dwy[3];
-            weights3f(fx, inv, wx, dwx);
-            weights3f(fy, inv, wy, dwy);
-            // stress term ps * grad w = (ps * dw_x[d]) * w_y[e], (ps * dw_y[e]) * w_x[d]
-            float psx[3], psy[3];
+            // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}:
+            // staged once, every load of a thread issued before its first LDS
+            // write
+            constexpr int TPT = (TCELLS * TCELLS + WG - 1) / WG;
+            float tv[TPT][3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                psx[d] = ps * dwx[d];
-                psy[d] = ps * dwy[d];
+            for (int i = 0; i < TPT; ++i) {
+                const int q = tid + i * WG;
+                const int qa = q / TCELLS, qb = q - qa * TCELLS;
+                const int gi = tx * TILE + qa, gj = ty * TILE + qb;
+                tv[i][0] = tv[i][1] = tv[i][2] = 0.f;
+                if (q < TCELLS * TCELLS && gi < nx && gj < ny) {
+                    const size_t k = (size_t)gi * ny + gj;
+                    tv[i][0] = tgx[k];
+                    tv[i][1] = tgy[k];
+                    tv[i][2] = tcth[k];
+                }
             }
+            // the region's fields; P_z (the last) on output steps only.  A
+            // staged tile's region lies over its fields and is cleared after
+            // the sums.
+            const int nreg = (need_vz ? P2G_FIELDS : P2G_PZ) * REGION_NODES;
+            float* const s_reg = STAGED ? reinterpret_cast<float*>(s_mem) : s_acc;
+            if (!STAGED)
+                for (int q = tid; q < nreg; q += WG) s_acc[q] = 0.f;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
+            for (int i = 0; i < TPT; ++i) {
+                const int q = tid + i * WG;
+                if (q < TCELLS * TCELLS) {
+                    s_ter[q] = tv[i][0];
+                    s_ter[TCELLS * TCELLS + q] = tv[i][1];
+                    s_ter[2 * TCELLS * TCELLS + q] = tv[i][2];
+                }
+            }
+            float2* const s_o2 = reinterpret_cast<float2*>(s_mem);
+            float2* const s_v2 = s_o2 + n;
+            float* const s_m = reinterpret_cast<float*>(s_v2 + n);
+            float* const s_h = s_m + n;
+            float* const s_z = s_h + n;
+            if constexpr (STAGED) {
+                __syncthreads(); // every thread has read the sorted indices the fields go over
 #pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    const float wgt = wx[d] * wy[e];
-                    if (wgt > 3e-3f) { // simulation.cpp:780
-                        float* a = acc[d * 3 + e];
-                        a[0] = fmaf(mp, wgt, a[0]);
-                        a[1] = fmaf(vmx, wgt, a[1]);
-                        a[2] = fmaf(vmy, wgt, a[2]);
-                        a[3] = fmaf(hm, wgt, a[3]);
-                        a[4] = fmaf(-psx[d], wy[e], fmaf(mbx, wgt, a[4]));
-                        a[5] = fmaf(-psy[e], wx[d], fmaf(mby, wgt, a[5]));
-                        if (VZ) a[P2G_PZ] = fmaf(vzm, wgt, a[P2G_PZ]);
+                for (int k = 0; k < ESK; ++k) {
+                    const int q = tid + k * WG;
+                    if (q < n) {
+                        s_o2[q] = so[k];
+                        s_v2[q] = sw[k];
+                        s_m[q] = sm[k];
+                        s_h[q] = sh[k];
+                        if (need_vz) s_z[q] = sz[k];
                     }
                 }
             }
-        }
-        };
-        if (need_vz)
-            sum_particles(std::true_type{});
-        else
-            sum_particles(std::false_type{});
-        __syncthreads();
-#pragma unroll
-        for (int ph = 0; ph < 9; ++ph) {
-            const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
-#pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f)
-                if (f < P2G_PZ || need_vz) s_acc[f * REGION_NODES + node] += acc[ph][f];
             __syncthreads();
-        }
-        float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
-        for (int q = tid; q < nreg; q += WG) dst[q] = s_acc[q];
-        __syncthreads();
+            float acc[9][P2G_FIELDS];
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+#pragma unroll
+                for (int f = 0; f < P2G_FIELDS; ++f) acc[k][f] = 0.f;
+            // the vz momentum (grid vz output only) is summed on output steps
+            // only.  Unstaged: the next particle's loads are issued before the
+            // current one is summed, and the index of the one after it a
+            // particle further ahead, so no iteration waits for an index load.
+            auto sum_particles = [&](auto vz_tag) {
+                constexpr bool VZ = decltype(vz_tag)::value;
+                int sn_next = !STAGED && r0 + 1 < r1 ? s_sorted[r0 + 1] : 0;
+                for (int r = r0; r < r1; ++r) {
+                    float ox, oy, mp, hp, pvx, pvy, pvz;
+                    if constexpr (STAGED) {
+                        const float2 po = s_o2[r], pv = s_v2[r];
+                        ox = po.x; oy = po.y; mp = s_m[r]; hp = s_h[r];
+                        pvx = pv.x; pvy = pv.y;
+                        pvz = VZ ? s_z[r] : 0.f;
+                    } else {
+                        ox = nox; oy = noy; mp = nmp; hp = nhp; pvx = nvx; pvy = nvy; pvz = nvz;
+                        if (r + 1 < r1) {
+                            const int sn = sn_next;
+                            const float2 po = P.o[sn], pv = P.v[sn];
+                            nox = po.x; noy = po.y; nmp = P.m[sn]; nhp = P.h[sn];
+                            nvx = pv.x; nvy = pv.y;
+                            if (VZ) nvz = P.vz[sn];
+                            if (r + 2 < r1) sn_next = s_sorted[r + 2];
+                        }
+                    }
+                    // live particles sit in cells [1, N-3] (outflow parks the
+                    // others), so the stencil base is never clamped and the
+                    // floor cell is the base plus (offset < 1/2)
+                    const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
+                    const int fl = (lx + ax) * TCELLS + (ly + ay);
+                    const float fx = (float)ax + ox, fy = (float)ay + oy;
+                    const float cth = s_ter[2 * TCELLS * TCELLS + fl];
+                    const float mgc = mp * g * cth * cth; // m g cos^2 theta
+                    const float vmx = pvx * mp, vmy = pvy * mp, hm = hp * mp, vzm = pvz * mp;
+                    // body force m b = -m g cos^2 theta grad z
+                    const float mbx = -mgc * s_ter[fl], mby = -mgc * s_ter[TCELLS * TCELLS + fl];
+                    // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
+                    const float ps = -0.5f * mgc * hp;
+                    float wx[3], wy[3], dwx[3], dwy[3];
+                    weights3f(fx, inv, wx, dwx);
+                    weights3f(fy, inv, wy, dwy);
+                    // stress term ps * grad w = (ps * dw_x[d]) * w_y[e], (ps * dw_y[e]) * w_x[d]
+                    float psx[3], psy[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        psx[d] = ps * dwx[d];
+                        psy[d] = ps * dwy[d];
+                    }
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                        for (int e = 0; e < 3; ++e) {
+                            const float wgt = wx[d] * wy[e];
+                            if (wgt > 3e-3f) { // simulation.cpp:780
+                                float* a = acc[d * 3 + e];
+                                a[0] = fmaf(mp, wgt, a[0]);
+                                a[1] = fmaf(vmx, wgt, a[1]);
+                                a[2] = fmaf(vmy, wgt, a[2]);
+                                a[3] = fmaf(hm, wgt, a[3]);
+                                a[4] = fmaf(-psx[d], wy[e], fmaf(mbx, wgt, a[4]));
+                                a[5] = fmaf(-psy[e], wx[d], fmaf(mby, wgt, a[5]));
+                                if (VZ) a[P2G_PZ] = fmaf(vzm, wgt, a[P2G_PZ]);
+                            }
+                        }
+                    }
+                }
+            };
+            if (need_vz)
+                sum_particles(std::true_type{});
+            else
+                sum_particles(std::false_type{});
+            if constexpr (STAGED) {
+                __syncthreads(); // the fields are summed: the region goes over them
+                for (int q = tid; q < nreg; q += WG) s_reg[q] = 0.f;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ph = 0; ph < 9; ++ph) {
+                const int node = (lx + ph / 3) * REGION + (ly + ph % 3);
+#pragma unroll
+                for (int f = 0; f < P2G_FIELDS; ++f)
+                    if (f < P2G_PZ || need_vz) s_reg[f * REGION_NODES + node] += acc[ph][f];
+                __syncthreads();
+            }
+            float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
+            for (int q = tid; q < nreg; q += WG) dst[q] = s_reg[q];
+            __syncthreads();
+        };
+        if (n * (need_vz ? ESP_STAGE_B_VZ : ESP_STAGE_B) <= STAGE_BYTES)
+            p2g_tile(std::true_type{});
+        else
+            p2g_tile(std::false_type{});
     }
 }
 
@@ -1609,9 +1693,6 @@ __device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict_
 // into the sorted index; then P2G of those from the index.  No kernel
 // boundary between the sort and P2G, one LDS region for all passes, and each
 // pass a loop of its own, so none holds another's registers.
-constexpr int esp_lds_bytes(int cap) {
-    return cap * 9 > cap * 4 + P2G_LDS_FLOATS * 4 ? cap * 9 : cap * 4 + P2G_LDS_FLOATS * 4;
-}
 template <int CAP>
 __global__ void __launch_bounds__(WG) k_esp(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                             const uint8_t* __restrict__ kcell, const int* __restrict__ ptiles,

@@ -1125,6 +1125,62 @@ __device__ __forceinline__ void esort_tiles(ECtl* c, PB32 A, PB32 B, const uint8
 // partial, merged per node by k_egrid.
 constexpr int P2G_TCELLS = TILE + 1;
 constexpr int P2G_LDS_FLOATS = P2G_FIELDS * REGION_NODES + 3 * P2G_TCELLS * P2G_TCELLS;
+
+// One particle's stencil sums into the 9 slots x 7 fields of its cell's
+// thread (cell (lx, ly) of the tile, the floor cells' {Gx, Gy, cos theta} in
+// s_ter, 17 x 17 per field).  Both P2G paths call it, so a tile's sums do not
+// depend on the path that takes it.  VZ adds the vz momentum (the grid's vz
+// output, on output steps only).
+template <bool VZ>
+__device__ __forceinline__ void ep2g_particle(float (&acc)[9][P2G_FIELDS], const float* s_ter, int lx, int ly,
+                                              float g, float inv, float ox, float oy, float mp, float hp, float pvx,
+                                              float pvy, float pvz) {
+    constexpr int TCELLS = P2G_TCELLS;
+    // live particles sit in cells [1, N-3] (outflow parks the others), so the
+    // stencil base is never clamped and the floor cell is the base plus
+    // (offset < 1/2)
+    const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
+    const int fl = (lx + ax) * TCELLS + (ly + ay);
+    const float fx = (float)ax + ox, fy = (float)ay + oy;
+    const float cth = s_ter[2 * TCELLS * TCELLS + fl];
+    const float mgc = mp * g * cth * cth; // m g cos^2 theta
+    const float vmx = pvx * mp, vmy = pvy * mp, hm = hp * mp, vzm = pvz * mp;
+    // body force m b = -m g cos^2 theta grad z
+    const float mbx = -mgc * s_ter[fl], mby = -mgc * s_ter[TCELLS * TCELLS + fl];
+    // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
+    const float ps = -0.5f * mgc * hp;
+    float wx[3], wy[3], dwx[3], dwy[3];
+    weights3f(fx, inv, wx, dwx);
+    weights3f(fy, inv, wy, dwy);
+    // node force m b w - ps grad w with w = w_x[d] w_y[e], per axis one
+    // factor times the other axis' weight: x (mbx w_x[d] - ps dw_x[d]) w_y[e],
+    // y (mby w_y[e] - ps dw_y[e]) w_x[d]
+    float frx[3], fry[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        frx[d] = fmaf(mbx, wx[d], -ps * dwx[d]);
+        fry[d] = fmaf(mby, wy[d], -ps * dwy[d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const float wgt = wx[d] * wy[e];
+            // simulation.cpp:780; the centre weight is at least 1/4 (the
+            // middle weight of an axis is in [1/2, 3/4])
+            if ((d == 1 && e == 1) || wgt > 3e-3f) {
+                float* a = acc[d * 3 + e];
+                a[0] = fmaf(mp, wgt, a[0]);
+                a[1] = fmaf(vmx, wgt, a[1]);
+                a[2] = fmaf(vmy, wgt, a[2]);
+                a[3] = fmaf(hm, wgt, a[3]);
+                a[4] = fmaf(frx[d], wy[e], a[4]);
+                a[5] = fmaf(fry[e], wx[d], a[5]);
+                if (VZ) a[P2G_PZ] = fmaf(vzm, wgt, a[P2G_PZ]);
+            }
+        }
+    }
+}
 // s_acc: P2G_FIELDS x REGION_NODES floats, s_ter: 3 x 17 x 17 floats
 __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slots, PB32 A, PB32 B,
                                            const int* __restrict__ idx, const int* __restrict__ ptiles,
@@ -1215,46 +1271,7 @@ __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slo
                 if (VZ) nvz = P.vz[sn];
                 if (r + 2 < r1) sn_next = idx[r + 2];
             }
-            // live particles sit in cells [1, N-3] (outflow parks the others),
-            // so the stencil base is never clamped and the floor cell is the
-            // base plus (offset < 1/2)
-            const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
-            const int fl = (lx + ax) * TCELLS + (ly + ay);
-            const float fx = (float)ax + ox, fy = (float)ay + oy;
-            const float cth = s_ter[2 * TCELLS * TCELLS + fl];
-            const float mgc = mp * g * cth * cth; // m g cos^2 theta
-            const float vmx = pvx * mp, vmy = pvy * mp, hm = hp * mp, vzm = pvz * mp;
-            // body force m b = -m g cos^2 theta grad z
-            const float mbx = -mgc * s_ter[fl], mby = -mgc * s_ter[TCELLS * TCELLS + fl];
-            // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
-            const float ps = -0.5f * mgc * hp;
-            float wx[3], wy[3], dwx[3], dwy[3];
-            weights3f(fx, inv, wx, dwx);
-            weights3f(fy, inv, wy, dwy);
-            // stress term ps * grad w = (ps * dw_x[d]) * w_y[e], (ps * dw_y[e]) * w_x[d]
-            float psx[3], psy[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                psx[d] = ps * dwx[d];
-                psy[d] = ps * dwy[d];
-            }
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-#pragma unroll
-                for (int e = 0; e < 3; ++e) {
-                    const float wgt = wx[d] * wy[e];
-                    if (wgt > 3e-3f) { // simulation.cpp:780
-                        float* a = acc[d * 3 + e];
-                        a[0] = fmaf(mp, wgt, a[0]);
-                        a[1] = fmaf(vmx, wgt, a[1]);
-                        a[2] = fmaf(vmy, wgt, a[2]);
-                        a[3] = fmaf(hm, wgt, a[3]);
-                        a[4] = fmaf(-psx[d], wy[e], fmaf(mbx, wgt, a[4]));
-                        a[5] = fmaf(-psy[e], wx[d], fmaf(mby, wgt, a[5]));
-                        if (VZ) a[P2G_PZ] = fmaf(vzm, wgt, a[P2G_PZ]);
-                    }
-                }
-            }
+            ep2g_particle<VZ>(acc, s_ter, lx, ly, g, inv, ox, oy, mp, hp, pvx, pvy, pvz);
         }
         };
         if (need_vz)
@@ -1617,46 +1634,7 @@ __device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict_
                             if (r + 2 < r1) sn_next = s_sorted[r + 2];
                         }
                     }
-                    // live particles sit in cells [1, N-3] (outflow parks the
-                    // others), so the stencil base is never clamped and the
-                    // floor cell is the base plus (offset < 1/2)
-                    const int ax = ox < 0.5f ? 1 : 0, ay = oy < 0.5f ? 1 : 0;
-                    const int fl = (lx + ax) * TCELLS + (ly + ay);
-                    const float fx = (float)ax + ox, fy = (float)ay + oy;
-                    const float cth = s_ter[2 * TCELLS * TCELLS + fl];
-                    const float mgc = mp * g * cth * cth; // m g cos^2 theta
-                    const float vmx = pvx * mp, vmy = pvy * mp, hm = hp * mp, vzm = pvz * mp;
-                    // body force m b = -m g cos^2 theta grad z
-                    const float mbx = -mgc * s_ter[fl], mby = -mgc * s_ter[TCELLS * TCELLS + fl];
-                    // h*A * sigma_00 with h*A = m / rho and sigma_00 = -rho g cos^2 h / 2
-                    const float ps = -0.5f * mgc * hp;
-                    float wx[3], wy[3], dwx[3], dwy[3];
-                    weights3f(fx, inv, wx, dwx);
-                    weights3f(fy, inv, wy, dwy);
-                    // stress term ps * grad w = (ps * dw_x[d]) * w_y[e], (ps * dw_y[e]) * w_x[d]
-                    float psx[3], psy[3];
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        psx[d] = ps * dwx[d];
-                        psy[d] = ps * dwy[d];
-                    }
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-#pragma unroll
-                        for (int e = 0; e < 3; ++e) {
-                            const float wgt = wx[d] * wy[e];
-                            if (wgt > 3e-3f) { // simulation.cpp:780
-                                float* a = acc[d * 3 + e];
-                                a[0] = fmaf(mp, wgt, a[0]);
-                                a[1] = fmaf(vmx, wgt, a[1]);
-                                a[2] = fmaf(vmy, wgt, a[2]);
-                                a[3] = fmaf(hm, wgt, a[3]);
-                                a[4] = fmaf(-psx[d], wy[e], fmaf(mbx, wgt, a[4]));
-                                a[5] = fmaf(-psy[e], wx[d], fmaf(mby, wgt, a[5]));
-                                if (VZ) a[P2G_PZ] = fmaf(vzm, wgt, a[P2G_PZ]);
-                            }
-                        }
-                    }
+                    ep2g_particle<VZ>(acc, s_ter, lx, ly, g, inv, ox, oy, mp, hp, pvx, pvy, pvz);
                 }
             };
             if (need_vz)

@@ -293,7 +293,11 @@ def test_sort_capacity_does_not_change_results(n_per_axis, cells, corner):
     """The in-LDS capacity of the tile sort (1,024, 2,048 or 4,096 particles) is
     a launch-shape choice: tiles of 1,296 (cells of 36), 2,601 (cells of 289),
     6,400 (cells of 400) and ~6,000 particles, sorted in LDS or in rounds of
-    whole cells of at most half the capacity, give bitwise the same results."""
+    whole cells of at most half the capacity, give bitwise the same results.
+    The 1,296-particle tiles also take every P2G form: the sort in global
+    memory (capacity 1,024), the in-LDS sort with each cell's particles read
+    from global memory as its thread goes (2,048: the fields of 720 or more
+    particles do not fit the LDS), and with the fields staged in LDS (4,096)."""
     dense = (_dense(n_per_axis, cells, corner, corner), FRICS[1])
     other = _scenario(21, FRICS[0])
     ref = _run_batch([other, dense], cap=400000, total=0.3)

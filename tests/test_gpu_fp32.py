@@ -5,7 +5,7 @@ arithmetic, FP64 time and conservation sums):
   - one step: grid mass / momentum / particle state within 1e-5 relative
     (max |a-b| / max |b|);
   - full C0 run (60 s, 6,000 steps): max-height map L1 error <= 1%
-    (measured 0.28%), max-velocity map L1 <= 2%, runout within 2 cells,
+    (measured 0.32%), max-velocity map L1 <= 2%, runout within 2 cells,
     particle mass exactly conserved.  L-inf of the max-height map is set by
     single nodes at the flow margin, where a node's mass sits near the
     1e-6 kg / w > 3e-3 thresholds (measured 0.14 of the map maximum); it is

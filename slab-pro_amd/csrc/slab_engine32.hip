@@ -1126,6 +1126,16 @@ __device__ __forceinline__ void esort_tiles(ECtl* c, PB32 A, PB32 B, const uint8
 constexpr int P2G_TCELLS = TILE + 1;
 constexpr int P2G_LDS_FLOATS = P2G_FIELDS * REGION_NODES + 3 * P2G_TCELLS * P2G_TCELLS;
 
+// The cell (tile row << 4 | column) a P2G thread sums: the two 16-lane rows
+// of each 32-lane half of a wave are tile rows r and r + 8, whose partial
+// region rows (18 nodes apart per row: offsets 18 r and 18 r + 144) fall in
+// disjoint halves of 32 LDS banks, so the 9-phase reduction's adds meet no
+// bank conflicts (rows r and r + 1 overlap in two banks).  Which thread sums
+// a cell changes no sum.
+__device__ __forceinline__ int p2g_cell(int tid) {
+    return (((tid >> 5) + ((tid >> 4) & 1) * 8) << 4) | (tid & 15);
+}
+
 // One particle's stencil sums into the 9 slots x 7 fields of its cell's
 // thread (cell (lx, ly) of the tile, the floor cells' {Gx, Gy, cos theta} in
 // s_ter, 17 x 17 per field).  Both P2G paths call it, so a tile's sums do not
@@ -1192,7 +1202,7 @@ __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slo
     const PB32 P = ecur(c, A, B);
     const int nt = c->n_ptiles, nx = c->nx, ny = c->ny, nty = c->nty, ntd = c->ntd, live = c->nslots * c->ntd;
     const float inv = (float)c->inv_dx;
-    const int cid = threadIdx.x;
+    const int cid = p2g_cell(threadIdx.x);
     const int lx = cid >> 4, ly = cid & 15;
     for (int w = blockIdx.x; w < nt; w += gridDim.x) {
         if (!only[w]) continue; // the tiles the small-tile pass left
@@ -1513,8 +1523,9 @@ __device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict_
         const bool need_vz = slots[slot].out_step != 0;
         // this thread's cell's particles: positions [r0, r1) of the tile's
         // sorted order
-        const int lx = tid >> 4, ly = tid & 15;
-        const int r0 = off, r1 = off + cnt;
+        const int pc = p2g_cell(tid);
+        const int lx = pc >> 4, ly = pc & 15;
+        const int r0 = s_off[pc], r1 = s_hist[pc]; // s_hist: the cell's end after the placement
         const int tx = local / nty, ty = local - tx * nty;
         // a tile whose particle fields fit the LDS below the terrain has them
         // staged there in sorted order: consecutive positions on consecutive

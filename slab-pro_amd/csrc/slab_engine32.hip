@@ -1136,6 +1136,21 @@ __device__ __forceinline__ int p2g_cell(int tid) {
     return (((tid >> 5) + ((tid >> 4) & 1) * 8) << 4) | (tid & 15);
 }
 
+// A tile's partial region (nreg floats, a multiple of 4: 6 or 7 fields x
+// 324 nodes) is cleared in LDS and copied to its tile partial 16 bytes per
+// thread and access.  The LDS regions start 16-byte aligned and every tile
+// partial is 2,268 floats (9,072 bytes) into a 256-byte aligned buffer.
+static_assert(REGION_NODES % 4 == 0, "partial regions are copied as float4");
+__device__ __forceinline__ void clear_region(float* s, int nreg, int tid) {
+    float4* s4 = reinterpret_cast<float4*>(s);
+    for (int q = tid; q < (nreg >> 2); q += WG) s4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ void store_region(float* __restrict__ dst, const float* s, int nreg, int tid) {
+    float4* d4 = reinterpret_cast<float4*>(dst);
+    const float4* s4 = reinterpret_cast<const float4*>(s);
+    for (int q = tid; q < (nreg >> 2); q += WG) d4[q] = s4[q];
+}
+
 // One particle's stencil sums into the 9 slots x 7 fields of its cell's
 // thread (cell (lx, ly) of the tile, the floor cells' {Gx, Gy, cos theta} in
 // s_ter, 17 x 17 per field).  Both P2G paths call it, so a tile's sums do not
@@ -1231,7 +1246,7 @@ __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slo
         const int tx = local / nty, ty = local - tx * nty;
         // the region's fields; P_z (the last) on output steps only
         const int nreg = (need_vz ? P2G_FIELDS : P2G_PZ) * REGION_NODES;
-        for (int q = threadIdx.x; q < nreg; q += WG) s_acc[q] = 0.f;
+        clear_region(s_acc, nreg, threadIdx.x);
         // the tile's floor cells (base + {0,1}) hold {Gx, Gy, cos theta}: staged
         // once, every load of a thread issued before its first LDS write
         constexpr int TPT = (TCELLS * TCELLS + WG - 1) / WG;
@@ -1298,7 +1313,7 @@ __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slo
             __syncthreads();
         }
         float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
-        for (int q = threadIdx.x; q < nreg; q += WG) dst[q] = s_acc[q];
+        store_region(dst, s_acc, nreg, threadIdx.x);
         __syncthreads();
     }
 }
@@ -1585,7 +1600,7 @@ __device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict_
             const int nreg = (need_vz ? P2G_FIELDS : P2G_PZ) * REGION_NODES;
             float* const s_reg = STAGED ? reinterpret_cast<float*>(s_mem) : s_acc;
             if (!STAGED)
-                for (int q = tid; q < nreg; q += WG) s_acc[q] = 0.f;
+                clear_region(s_acc, nreg, tid);
 #pragma unroll
             for (int i = 0; i < TPT; ++i) {
                 const int q = tid + i * WG;
@@ -1654,7 +1669,7 @@ __device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict_
                 sum_particles(std::false_type{});
             if constexpr (STAGED) {
                 __syncthreads(); // the fields are summed: the region goes over them
-                for (int q = tid; q < nreg; q += WG) s_reg[q] = 0.f;
+                clear_region(s_reg, nreg, tid);
             }
             __syncthreads();
 #pragma unroll
@@ -1666,7 +1681,7 @@ __device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict_
                 __syncthreads();
             }
             float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
-            for (int q = tid; q < nreg; q += WG) dst[q] = s_reg[q];
+            store_region(dst, s_reg, nreg, tid);
             __syncthreads();
         };
         if (n * (need_vz ? ESP_STAGE_B_VZ : ESP_STAGE_B) <= STAGE_BYTES)

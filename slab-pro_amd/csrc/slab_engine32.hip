@@ -1136,19 +1136,26 @@ __device__ __forceinline__ int p2g_cell(int tid) {
     return (((tid >> 5) + ((tid >> 4) & 1) * 8) << 4) | (tid & 15);
 }
 
-// A tile's partial region (nreg floats, a multiple of 4: 6 or 7 fields x
-// 324 nodes) is cleared in LDS and copied to its tile partial 16 bytes per
-// thread and access.  The LDS regions start 16-byte aligned and every tile
-// partial is 2,268 floats (9,072 bytes) into a 256-byte aligned buffer.
-static_assert(REGION_NODES % 4 == 0, "partial regions are copied as float4");
+// A tile's partial region is summed in LDS field-major (field f of node n at
+// f * 324 + n: the 9-phase adds meet no bank conflicts) and stored node-major
+// by field group, so the grid kernel reads each source node with two vector
+// loads: {m, P_x, P_y, H} as a float4 per node, then {F_x, F_y} as a float2
+// per node, then P_z per node (output steps only) -- the same 2,268 floats
+// (9,072 bytes, 16-byte aligned in a 256-byte aligned buffer) per tile.
+static_assert(P2G_FIELDS == 7 && P2G_PZ == 6, "partial groups: 4 + 2 fields, then P_z");
+constexpr int PART_F2 = 4 * REGION_NODES; // float offset of the {F_x, F_y} group
+constexpr int PART_PZ = 6 * REGION_NODES; // float offset of P_z
 __device__ __forceinline__ void clear_region(float* s, int nreg, int tid) {
     float4* s4 = reinterpret_cast<float4*>(s);
     for (int q = tid; q < (nreg >> 2); q += WG) s4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
-__device__ __forceinline__ void store_region(float* __restrict__ dst, const float* s, int nreg, int tid) {
-    float4* d4 = reinterpret_cast<float4*>(dst);
-    const float4* s4 = reinterpret_cast<const float4*>(s);
-    for (int q = tid; q < (nreg >> 2); q += WG) d4[q] = s4[q];
+__device__ __forceinline__ void store_region(float* __restrict__ dst, const float* s, bool vz, int tid) {
+    constexpr int RN = REGION_NODES;
+    for (int n = tid; n < RN; n += WG) {
+        reinterpret_cast<float4*>(dst)[n] = make_float4(s[n], s[RN + n], s[2 * RN + n], s[3 * RN + n]);
+        reinterpret_cast<float2*>(dst + PART_F2)[n] = make_float2(s[4 * RN + n], s[5 * RN + n]);
+        if (vz) dst[PART_PZ + n] = s[PART_PZ + n];
+    }
 }
 
 // One particle's stencil sums into the 9 slots x 7 fields of its cell's
@@ -1313,7 +1320,7 @@ __device__ __forceinline__ void ep2g_tiles(ECtl* c, const Slot* __restrict__ slo
             __syncthreads();
         }
         float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
-        store_region(dst, s_acc, nreg, threadIdx.x);
+        store_region(dst, s_acc, need_vz, threadIdx.x);
         __syncthreads();
     }
 }
@@ -1681,7 +1688,7 @@ __device__ __forceinline__ void esp_small_tiles(ECtl* c, const Slot* __restrict_
                 __syncthreads();
             }
             float* dst = tpart + (size_t)w * (P2G_FIELDS * REGION_NODES);
-            store_region(dst, s_reg, nreg, tid);
+            store_region(dst, s_reg, need_vz, tid);
             __syncthreads();
         };
         if (n * (need_vz ? ESP_STAGE_B_VZ : ESP_STAGE_B) <= STAGE_BYTES)
@@ -1797,9 +1804,16 @@ __global__ void __launch_bounds__(WG) k_egrid(ECtl* c, const Slot* __restrict__ 
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (!ok[q]) continue;
-            const float* pp = tpart + (size_t)pos[q] * (P2G_FIELDS * REGION_NODES) + rn[q];
-#pragma unroll
-            for (int f = 0; f < P2G_FIELDS; ++f) val[q][f] = (f < P2G_PZ || pz) ? pp[f * REGION_NODES] : 0.f;
+            const float* pp = tpart + (size_t)pos[q] * (P2G_FIELDS * REGION_NODES);
+            const float4 m4 = reinterpret_cast<const float4*>(pp)[rn[q]];
+            const float2 f2 = reinterpret_cast<const float2*>(pp + PART_F2)[rn[q]];
+            val[q][0] = m4.x;
+            val[q][1] = m4.y;
+            val[q][2] = m4.z;
+            val[q][3] = m4.w;
+            val[q][4] = f2.x;
+            val[q][5] = f2.y;
+            val[q][P2G_PZ] = pz ? pp[PART_PZ + rn[q]] : 0.f;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
